@@ -1,0 +1,1055 @@
+/*
+ * ddaoracle.c -- CPU restatement of the DDA-FFT solve path (TEST INFRASTRUCTURE).
+ *
+ * This is the oracle: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it, and only as the checker.  The product
+ * (the HIP library) never links or calls it.
+ *
+ * It restates, function by function, the reference's algorithm for the path
+ * (physicist2018/dda-fft v7.2) with the same floating-point operation order, so
+ * that the float build (ORACLE_REAL=float, -ffp-contract=off) reproduces the
+ * reference's own float results bit for bit where the reference is
+ * deterministic, and the double build (ORACLE_REAL=double) is the FP64 build of
+ * the same solve used for error statements.  Each function cites the reference
+ * file:line it follows.
+ *
+ * Array conventions (all flat, row-major, caller-owned):
+ *   field  [ni][nj][nl][3][2]      (cvector per cell, l fastest)
+ *   padded [2ni][2nj][2nl][3][2]
+ *   ahat   [2ni][2nj][2nl][6][2]   (e00 e01 e02 e11 e12 e22)
+ *   occ    [ni][nj][nl] int8 material index, 0 = vacuum
+ *   alpha  [nmat][2] polarizability per material index, alpha[0] = 0
+ *   mv     [nmat] int: 1 if material index m is a dipole (iListMV)
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   1-D kernels     brute-force DFT in float64; bit-exact vs the reference build
+ *   pruned 3-D FFT  numpy fftn of the zero-padded field; bit-exact vs reference
+ *   A-hat           closed-form A(r) + numpy fftn; bit-exact vs reference
+ *   convolution     direct O(N^2) dipole sum on small lattices; bit-exact vs ref
+ *   solvers         textbook dense solve in float64 (residual invariant) and the
+ *                   reference's own iterate/discrepancy sequence
+ *   far field       direct float64 summation; reference output
+ *   cross sections  Cabs = 0 for real m; reference output
+ *   Mueller         S11 >= |S12|, sphere block structure; reference output
+ *   polarizability  closed forms in complex128; reference output
+ */
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef ORACLE_REAL
+#define ORACLE_REAL float
+#endif
+
+typedef ORACLE_REAL real;
+
+#define ORC_IS_FLOAT (sizeof(real) == sizeof(float))
+#define PI_D 3.1415926535897932384626433832795
+
+/* libm entry points matching the reference's overload resolution: a float
+   argument goes to the float function (cosf, sinf, sqrtf). */
+static inline real rcos(real x) { return ORC_IS_FLOAT ? (real)cosf((float)x) : (real)cos((double)x); }
+static inline real rsin(real x) { return ORC_IS_FLOAT ? (real)sinf((float)x) : (real)sin((double)x); }
+static inline real rsqrt_(real x) { return ORC_IS_FLOAT ? (real)sqrtf((float)x) : (real)sqrt((double)x); }
+
+typedef struct { real r, i; } cplx;
+typedef struct { cplx x, y, z; } cvec;
+typedef struct { cplx e00, e01, e02, e11, e12, e22; } cmat;
+typedef struct { real x, y, z; } vec3;
+typedef struct { long double r, i; } ldcplx;
+
+/* complex.cpp:61-64 operator*(cplx,cplx); :11-14 operator+; :32-35 operator- */
+static inline cplx cmul(cplx a, cplx b) { cplx c; c.r = a.r * b.r - a.i * b.i; c.i = a.r * b.i + a.i * b.r; return c; }
+static inline cplx cadd(cplx a, cplx b) { cplx c; c.r = a.r + b.r; c.i = a.i + b.i; return c; }
+static inline cplx csub(cplx a, cplx b) { cplx c; c.r = a.r - b.r; c.i = a.i - b.i; return c; }
+static inline cplx cneg(cplx a) { cplx c; c.r = -a.r; c.i = -a.i; return c; }
+static inline cplx cconj(cplx a) { cplx c; c.r = a.r; c.i = -a.i; return c; }
+/* complex.cpp:67-71 operator*(real,cplx) */
+static inline cplx rmul(real r, cplx a) { cplx c; c.r = r * a.r; c.i = r * a.i; return c; }
+/* complex.cpp:81-90 operator/(cplx,cplx) */
+static inline cplx cdiv(cplx a, cplx b) {
+    real n = b.r * b.r + b.i * b.i;
+    cplx c; c.r = (a.r * b.r + a.i * b.i) / n; c.i = (a.i * b.r - a.r * b.i) / n; return c;
+}
+/* complex.cpp:176-181 exp_i */
+static inline cplx exp_i(real x) { cplx c; c.r = rcos(x); c.i = rsin(x); return c; }
+/* complex.cpp:244-260 long double complex ops */
+static inline ldcplx ldmul(ldcplx a, ldcplx b) { ldcplx c; c.r = a.r * b.r - a.i * b.i; c.i = a.r * b.i + a.i * b.r; return c; }
+static inline ldcplx lddiv(ldcplx a, ldcplx b) {
+    long double n = b.r * b.r + b.i * b.i;
+    ldcplx c; c.r = (a.r * b.r + a.i * b.i) / n; c.i = (a.i * b.r - a.r * b.i) / n; return c;
+}
+static inline cplx ld2c(ldcplx a) { cplx c; c.r = (real)a.r; c.i = (real)a.i; return c; }
+
+static inline size_t idx3(int i, int j, int l, int nj, int nl) { return ((size_t)i * nj + j) * nl + l; }
+
+static int ilog2i(int n) { int k = 0; while ((1 << k) < n) k++; return ((1 << k) == n) ? k : -1; }
+
+/* ------------------------------------------------------------------------ */
+/* Twiddle recurrence: fft_func.cpp:77-88 FourierPhase                       */
+/* ------------------------------------------------------------------------ */
+
+void orc_fourier_phase(int n, real *out /* [n][2] */) {
+    cplx *w = (cplx *)out;
+    cplx cw = exp_i((real)(-2. * PI_D / (2 * n)));
+    w[0].r = 1.; w[0].i = 0.;
+    if (n > 1) w[1] = cw;
+    for (int i = 2; i < n; i++) w[i] = cmul(w[i - 1], cw);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mixed radix 4/8 kernel: fft_kernel.hpp:35-322                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int n, nst, radix[24];
+    int *perm;
+    real *twr, *twi;
+} orc_plan;
+
+/* fft_kernel.hpp:193-208 */
+static int choose_radices(int k, int *r) {
+    int m = 0;
+    while (k > 6) { r[m++] = 8; k -= 3; }
+    switch (k) {
+    case 1: r[m++] = 2; break;
+    case 2: r[m++] = 4; break;
+    case 3: r[m++] = 8; break;
+    case 4: r[m++] = 4; r[m++] = 4; break;
+    case 5: r[m++] = 8; r[m++] = 4; break;
+    case 6: r[m++] = 8; r[m++] = 8; break;
+    default: break;
+    }
+    return m;
+}
+
+/* fft_kernel.hpp:210-246 */
+static int plan_build(orc_plan *p, int n) {
+    int k = ilog2i(n);
+    if (k < 0) return -1;
+    p->n = n;
+    p->nst = choose_radices(k, p->radix);
+    p->perm = (int *)malloc(sizeof(int) * n);
+    p->twr = (real *)malloc(sizeof(real) * n);
+    p->twi = (real *)malloc(sizeof(real) * n);
+    for (int j = 0; j < n; j++) {
+        int tmp = j, L = 1, val = 0;
+        for (int t = 0; t < p->nst; t++) {
+            int rt = p->radix[t], d = tmp % rt;
+            tmp /= rt; L *= rt; val += d * (n / L);
+        }
+        p->perm[j] = val;
+    }
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int q = 0; q < n; q++) {
+        double a = -two_pi * (double)q / (double)n;
+        p->twr[q] = (real)cos(a);
+        p->twi[q] = (real)sin(a);
+    }
+    return 0;
+}
+
+static void plan_free(orc_plan *p) { free(p->perm); free(p->twr); free(p->twi); }
+
+/* fft_kernel.hpp:47-51 */
+static inline void bf2(cplx *a, cplx *b) {
+    real tr = a->r + b->r, ti = a->i + b->i;
+    b->r = a->r - b->r; b->i = a->i - b->i;
+    a->r = tr; a->i = ti;
+}
+/* fft_kernel.hpp:55-66 */
+static inline void bf4(cplx *x0, cplx *x1, cplx *x2, cplx *x3) {
+    real t0r = x0->r + x2->r, t0i = x0->i + x2->i;
+    real t1r = x0->r - x2->r, t1i = x0->i - x2->i;
+    real t2r = x1->r + x3->r, t2i = x1->i + x3->i;
+    real t3r = x1->r - x3->r, t3i = x1->i - x3->i;
+    x0->r = t0r + t2r; x0->i = t0i + t2i;
+    x2->r = t0r - t2r; x2->i = t0i - t2i;
+    x1->r = t1r + t3i; x1->i = t1i - t3r;
+    x3->r = t1r - t3i; x3->i = t1i + t3r;
+}
+/* fft_kernel.hpp:70-106 */
+static inline void bf8(cplx *z) {
+    const real s = ORC_IS_FLOAT ? (real)0.70710678118654752440f : (real)0.70710678118654752440;
+    cplx a0, a1, a2, a3, b0, b1, b2, b3;
+    a0.r = z[0].r + z[4].r; a0.i = z[0].i + z[4].i;
+    a1.r = z[1].r + z[5].r; a1.i = z[1].i + z[5].i;
+    a2.r = z[2].r + z[6].r; a2.i = z[2].i + z[6].i;
+    a3.r = z[3].r + z[7].r; a3.i = z[3].i + z[7].i;
+    b0.r = z[0].r - z[4].r; b0.i = z[0].i - z[4].i;
+    { real dr = z[1].r - z[5].r, di = z[1].i - z[5].i; b1.r = (dr + di) * s; b1.i = (di - dr) * s; }
+    { real dr = z[2].r - z[6].r, di = z[2].i - z[6].i; b2.r = di; b2.i = -dr; }
+    { real dr = z[3].r - z[7].r, di = z[3].i - z[7].i; b3.r = (di - dr) * s; b3.i = -(dr + di) * s; }
+    bf4(&a0, &a1, &a2, &a3);
+    bf4(&b0, &b1, &b2, &b3);
+    z[0] = a0; z[1] = b0; z[2] = a1; z[3] = b1;
+    z[4] = a2; z[5] = b2; z[6] = a3; z[7] = b3;
+}
+/* fft_kernel.hpp:108-113 (one component) */
+static inline void twmul(cplx *v, real wr, real wi) {
+    real tr = v->r * wr - v->i * wi;
+    v->i = v->r * wi + v->i * wr;
+    v->r = tr;
+}
+
+/* fft_kernel.hpp:270-285 run_stages, stage2/4/8 (:119-185), on one component
+   stored with stride `st` (in cplx units). */
+static void run_stages(const orc_plan *p, cplx *v, int st) {
+    const int n = p->n;
+    int span = 1;
+    for (int t = 0; t < p->nst; t++) {
+        int r = p->radix[t];
+        span *= r;
+        int h = span / r, step = n / span;
+        for (int base = 0; base < n; base += span) {
+            for (int i = 0; i < h; i++) {
+                cplx z[8];
+                for (int q = 0; q < r; q++) z[q] = v[(size_t)(base + i + q * h) * st];
+                if (r > 2 && i > 0)
+                    for (int q = 1; q < r; q++) twmul(&z[q], p->twr[q * i * step], p->twi[q * i * step]);
+                if (r == 8) bf8(z);
+                else if (r == 4) bf4(&z[0], &z[1], &z[2], &z[3]);
+                else bf2(&z[0], &z[1]);
+                for (int q = 0; q < r; q++) v[(size_t)(base + i + q * h) * st] = z[q];
+            }
+        }
+    }
+}
+
+/* fft_kernel.hpp:289-296 fft_line / :301-322 ifft_line, on a line of cvectors */
+static void line_fwd(const orc_plan *p, cvec *out, const cvec *in) {
+    for (int k = 0; k < p->n; k++) out[k] = in[p->perm[k]];
+    run_stages(p, &out[0].x, 3); run_stages(p, &out[0].y, 3); run_stages(p, &out[0].z, 3);
+}
+static void line_inv(const orc_plan *p, cvec *out, const cvec *in) {
+    for (int k = 0; k < p->n; k++) {
+        cvec t = in[p->perm[k]];
+        out[k].x.r = t.x.r; out[k].x.i = -t.x.i;
+        out[k].y.r = t.y.r; out[k].y.i = -t.y.i;
+        out[k].z.r = t.z.r; out[k].z.i = -t.z.i;
+    }
+    run_stages(p, &out[0].x, 3); run_stages(p, &out[0].y, 3); run_stages(p, &out[0].z, 3);
+    for (int k = 0; k < p->n; k++) { out[k].x.i = -out[k].x.i; out[k].y.i = -out[k].y.i; out[k].z.i = -out[k].z.i; }
+}
+
+int orc_fft_line(int n, const real *in, real *out, int inverse) {
+    orc_plan p;
+    if (plan_build(&p, n)) return -1;
+    if (inverse) line_inv(&p, (cvec *)out, (const cvec *)in);
+    else line_fwd(&p, (cvec *)out, (const cvec *)in);
+    plan_free(&p);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Legacy radix-2 kernel: fft_func.cpp:44-56 bit reversal, :94-186 FFT       */
+/* Works on `nc` interleaved complex components per point.                   */
+/* ------------------------------------------------------------------------ */
+
+static int bitrev(int x, int bits) { int r = 0; for (int i = 0; i < bits; i++) r = (r << 1) | ((x >> i) & 1); return r; }
+
+static void legacy_fft(int n, const cplx *w, cplx *ini, cplx *dat, int nc) {
+    int nb = ilog2i(n);
+    for (int i = 0; i < n; i++) {
+        int s = bitrev(i, nb);
+        for (int c = 0; c < nc; c++) dat[i * nc + c] = ini[s * nc + c];
+    }
+    int nol = n / 2, step = n / nol;
+    for (int i = 0; i < nol; i++) {
+        int a = step * i;
+        for (int c = 0; c < nc; c++) {
+            cplx x = dat[a * nc + c], y = dat[(a + 1) * nc + c];
+            ini[a * nc + c] = cadd(x, y);
+            ini[(a + 1) * nc + c] = csub(x, y);
+        }
+    }
+    for (int l = 1; l < nb; l++) {
+        nol /= 2; step = n / nol;
+        for (int j = 0; j < nol; j++) {
+            int jn = step * j;
+            for (int i = 0; i < step / 2; i++) {
+                int in_ = jn + i, k = i * n / step, m = in_ + step / 2;
+                for (int c = 0; c < nc; c++) {
+                    cplx t = cmul(w[k], ini[m * nc + c]);
+                    dat[in_ * nc + c] = cadd(t, ini[in_ * nc + c]);
+                    dat[m * nc + c] = csub(ini[in_ * nc + c], t);
+                }
+            }
+        }
+        memcpy(ini, dat, sizeof(cplx) * n * nc);
+    }
+}
+
+int orc_fft_line_legacy(int n, const real *in, real *out, int inverse) {
+    int nb = ilog2i(n);
+    if (nb < 2) return -1;
+    cplx *w = (cplx *)malloc(sizeof(cplx) * (n / 2 + 1));
+    cplx *a = (cplx *)malloc(sizeof(cplx) * 3 * n);
+    orc_fourier_phase(n / 2, (real *)w);
+    memcpy(a, in, sizeof(cplx) * 3 * n);
+    if (inverse) for (int k = 0; k < 3 * n; k++) a[k].i = -a[k].i;      /* fft_func.cpp:284-311 */
+    legacy_fft(n, w, a, (cplx *)out, 3);
+    if (inverse) for (int k = 0; k < 3 * n; k++) ((cplx *)out)[k].i = -((cplx *)out)[k].i;
+    free(w); free(a);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Green tensor and A-hat: dda_func.cpp:85-148, fft_func.cpp:317-416          */
+/* ------------------------------------------------------------------------ */
+
+void orc_amatrix(int ni, int nj, int nl, real d0, real *out) {
+    cmat *A = (cmat *)out;
+    int Ni = 2 * ni, Nj = 2 * nj, Nl = 2 * nl;
+    for (int i = 0; i < Ni; i++) {
+        vec3 r;
+        r.x = (i < ni) ? (real)i * d0 : (real)(i - Ni) * d0;
+        for (int j = 0; j < Nj; j++) {
+            r.y = (j < nj) ? (real)j * d0 : (real)(j - Nj) * d0;
+            for (int l = 0; l < Nl; l++) {
+                r.z = (l < nl) ? (real)l * d0 : (real)(l - Nl) * d0;
+                cmat *a = &A[idx3(i, j, l, Nj, Nl)];
+                /* cvector.cpp:252-263 dyad */
+                real e00 = r.x * r.x, e11 = r.y * r.y, e22 = r.z * r.z;
+                real e01 = r.x * r.y, e02 = r.x * r.z, e12 = r.y * r.z;
+                real r2 = e00 + e11 + e22;
+                if ((double)r2 < 0.0000000001) { memset(a, 0, sizeof(*a)); continue; }
+                real r1 = rsqrt_(r2);
+                cplx c0, c1, c2, c3, c4, c5;
+                c0.r = r2 - (real)1.; c0.i = r1;
+                c1.r = (real)3. - r2; c1.i = (real)-3. * r1;
+                real r3 = r1 * r2;
+                c3 = exp_i(r1);
+                c2.r = c3.r / r3; c2.i = c3.i / r3;                 /* CDdiv */
+                c3 = cmul(c0, c2);
+                c4 = cmul(c1, c2);
+                c5.r = c4.r / r2; c5.i = c4.i / r2;
+                a->e00 = rmul(e00, c5); a->e01 = rmul(e01, c5); a->e02 = rmul(e02, c5);
+                a->e11 = rmul(e11, c5); a->e12 = rmul(e12, c5); a->e22 = rmul(e22, c5);
+                a->e00 = cadd(a->e00, c3); a->e11 = cadd(a->e11, c3); a->e22 = cadd(a->e22, c3);
+            }
+        }
+    }
+}
+
+/* fft_func.cpp:317-416: full unpruned legacy-kernel transform of the 6
+   components, axis l then j then i, times 1/(8N); tables FourierPhase(n). */
+void orc_build_ahat(int ni, int nj, int nl, real d0, real *out) {
+    int Ni = 2 * ni, Nj = 2 * nj, Nl = 2 * nl, Nm = Ni > Nj ? (Ni > Nl ? Ni : Nl) : (Nj > Nl ? Nj : Nl);
+    cmat *A = (cmat *)out;
+    cplx *wi = (cplx *)malloc(sizeof(cplx) * ni), *wj = (cplx *)malloc(sizeof(cplx) * nj),
+         *wl = (cplx *)malloc(sizeof(cplx) * nl);
+    cmat *ba = (cmat *)malloc(sizeof(cmat) * Nm), *bb = (cmat *)malloc(sizeof(cmat) * Nm);
+    orc_amatrix(ni, nj, nl, d0, out);
+    orc_fourier_phase(ni, (real *)wi); orc_fourier_phase(nj, (real *)wj); orc_fourier_phase(nl, (real *)wl);
+    for (int i = 0; i < Ni; i++)
+        for (int j = 0; j < Nj; j++) {
+            for (int l = 0; l < Nl; l++) ba[l] = A[idx3(i, j, l, Nj, Nl)];
+            legacy_fft(Nl, wl, (cplx *)ba, (cplx *)bb, 6);
+            for (int l = 0; l < Nl; l++) A[idx3(i, j, l, Nj, Nl)] = bb[l];
+        }
+    for (int i = 0; i < Ni; i++)
+        for (int l = 0; l < Nl; l++) {
+            for (int j = 0; j < Nj; j++) ba[j] = A[idx3(i, j, l, Nj, Nl)];
+            legacy_fft(Nj, wj, (cplx *)ba, (cplx *)bb, 6);
+            for (int j = 0; j < Nj; j++) A[idx3(i, j, l, Nj, Nl)] = bb[j];
+        }
+    for (int j = 0; j < Nj; j++)
+        for (int l = 0; l < Nl; l++) {
+            for (int i = 0; i < Ni; i++) ba[i] = A[idx3(i, j, l, Nj, Nl)];
+            legacy_fft(Ni, wi, (cplx *)ba, (cplx *)bb, 6);
+            for (int i = 0; i < Ni; i++) A[idx3(i, j, l, Nj, Nl)] = bb[i];
+        }
+    real norma = (real)(1. / (Ni * Nj * Nl));
+    size_t P = (size_t)Ni * Nj * Nl;
+    cplx *c = (cplx *)out;
+    for (size_t k = 0; k < 6 * P; k++) c[k] = rmul(norma, c[k]);
+    free(wi); free(wj); free(wl); free(ba); free(bb);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pruned zero-padded transforms: fft_func.cpp:418-653                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int ni, nj, nl;
+    orc_plan pi, pj, pl, Pi, Pj, Pl;  /* n-point (forward halves) and 2n-point (inverse) */
+    cplx *wi, *wj, *wl;               /* FourierPhase(n): W_2n^k, k < n */
+    cvec *b0, *b1;
+} orc_fft3;
+
+static int fft3_init(orc_fft3 *f, int ni, int nj, int nl) {
+    memset(f, 0, sizeof(*f));
+    f->ni = ni; f->nj = nj; f->nl = nl;
+    if (plan_build(&f->pi, ni) || plan_build(&f->pj, nj) || plan_build(&f->pl, nl) ||
+        plan_build(&f->Pi, 2 * ni) || plan_build(&f->Pj, 2 * nj) || plan_build(&f->Pl, 2 * nl))
+        return -1;
+    f->wi = (cplx *)malloc(sizeof(cplx) * ni); orc_fourier_phase(ni, (real *)f->wi);
+    f->wj = (cplx *)malloc(sizeof(cplx) * nj); orc_fourier_phase(nj, (real *)f->wj);
+    f->wl = (cplx *)malloc(sizeof(cplx) * nl); orc_fourier_phase(nl, (real *)f->wl);
+    int m = 2 * (ni > nj ? (ni > nl ? ni : nl) : (nj > nl ? nj : nl));
+    f->b0 = (cvec *)malloc(sizeof(cvec) * m);
+    f->b1 = (cvec *)malloc(sizeof(cvec) * m);
+    return 0;
+}
+
+static void fft3_free(orc_fft3 *f) {
+    plan_free(&f->pi); plan_free(&f->pj); plan_free(&f->pl);
+    plan_free(&f->Pi); plan_free(&f->Pj); plan_free(&f->Pl);
+    free(f->wi); free(f->wj); free(f->wl); free(f->b0); free(f->b1);
+}
+
+static inline cvec cvtw(cplx w, cvec v) { cvec o; o.x = cmul(w, v.x); o.y = cmul(w, v.y); o.z = cmul(w, v.z); return o; }
+
+/* One pruned forward line: n inputs at stride st -> 2n outputs at stride st,
+   X[2k] = DFT_n(x)[k], X[2k+1] = DFT_n(W_2n^j x_j)[k]  (fft_func.cpp:447-462). */
+static void fwd_line(orc_fft3 *f, const orc_plan *p, const cplx *w, cvec *d, size_t st) {
+    int n = p->n;
+    for (int k = 0; k < n; k++) f->b0[k] = d[k * st];
+    line_fwd(p, f->b1, f->b0);
+    for (int k = 0; k < n; k++) f->b0[k] = cvtw(w[k], d[k * st]);
+    for (int k = 0; k < n; k++) d[2 * k * st] = f->b1[k];
+    line_fwd(p, f->b1, f->b0);
+    for (int k = 0; k < n; k++) d[(2 * k + 1) * st] = f->b1[k];
+}
+
+static void inv_line(orc_fft3 *f, const orc_plan *p, cvec *d, size_t st) {
+    int n = p->n;
+    for (int k = 0; k < n; k++) f->b0[k] = d[k * st];
+    line_inv(p, f->b1, f->b0);
+    for (int k = 0; k < n; k++) d[k * st] = f->b1[k];
+}
+
+/* fft_func.cpp:418-550 on the padded buffer w [2ni][2nj][2nl] */
+static void fft3_forward(orc_fft3 *f, cvec *w) {
+    int ni = f->ni, nj = f->nj, nl = f->nl, Nj = 2 * nj, Nl = 2 * nl;
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++) fwd_line(f, &f->pl, f->wl, &w[idx3(i, j, 0, Nj, Nl)], 1);
+    for (int i = 0; i < ni; i++)
+        for (int l = 0; l < Nl; l++) fwd_line(f, &f->pj, f->wj, &w[idx3(i, 0, l, Nj, Nl)], (size_t)Nl);
+    for (int j = 0; j < Nj; j++)
+        for (int l = 0; l < Nl; l++) fwd_line(f, &f->pi, f->wi, &w[idx3(0, j, l, Nj, Nl)], (size_t)Nj * Nl);
+}
+
+/* fft_func.cpp:556-653 */
+static void fft3_inverse(orc_fft3 *f, cvec *w) {
+    int ni = f->ni, nj = f->nj, Nj = 2 * nj, Nl = 2 * f->nl;
+    for (int j = 0; j < Nj; j++)
+        for (int l = 0; l < Nl; l++) inv_line(f, &f->Pi, &w[idx3(0, j, l, Nj, Nl)], (size_t)Nj * Nl);
+    for (int i = 0; i < ni; i++)
+        for (int l = 0; l < Nl; l++) inv_line(f, &f->Pj, &w[idx3(i, 0, l, Nj, Nl)], (size_t)Nl);
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++) inv_line(f, &f->Pl, &w[idx3(i, j, 0, Nj, Nl)], 1);
+}
+
+/* dda_func.cpp:702-718 CopyingBack */
+static void copy_back(int ni, int nj, int nl, const cvec *x, cvec *w) {
+    int Nj = 2 * nj, Nl = 2 * nl;
+    memset(w, 0, sizeof(cvec) * (size_t)8 * ni * nj * nl);
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++)
+            memcpy(&w[idx3(i, j, 0, Nj, Nl)], &x[idx3(i, j, 0, nj, nl)], sizeof(cvec) * nl);
+}
+
+/* dda_func.cpp:157-182: y = Â x per padded cell (3x3 symmetric), cells that
+   are exactly zero are left alone (dda_func.cpp:160-162). */
+static inline cvec ahat_apply(cmat a, cvec v) {
+    cvec o;
+    cplx c0, c1, c2, c3;
+    c0 = cmul(a.e00, v.x); c1 = cmul(a.e01, v.y); c2 = cmul(a.e02, v.z); c3 = cadd(c0, c1); o.x = cadd(c2, c3);
+    c0 = cmul(a.e01, v.x); c1 = cmul(a.e11, v.y); c2 = cmul(a.e12, v.z); c3 = cadd(c0, c1); o.y = cadd(c2, c3);
+    c0 = cmul(a.e02, v.x); c1 = cmul(a.e12, v.y); c2 = cmul(a.e22, v.z); c3 = cadd(c0, c1); o.z = cadd(c2, c3);
+    return o;
+}
+
+static inline int cvec_nonzero(const cvec *v) {
+    return v->x.r != 0. || v->x.i != 0. || v->y.r != 0. || v->y.i != 0. || v->z.r != 0. || v->z.i != 0.;
+}
+
+static void pointwise(int Ni, int Nj, int Nl, const cmat *A, cvec *w, int conj) {
+    for (int i = 0; i < Ni; i++)
+        for (int j = 0; j < Nj; j++)
+            for (int l = 0; l < Nl; l++) {
+                size_t k = idx3(i, j, l, Nj, Nl);
+                if (!conj) {
+                    if (cvec_nonzero(&w[k])) w[k] = ahat_apply(A[k], w[k]);
+                } else {
+                    /* dda_func.cpp:227-437: conj(Â((P - k) mod P)) for every cell */
+                    cmat a = A[idx3((Ni - i) % Ni, (Nj - j) % Nj, (Nl - l) % Nl, Nj, Nl)];
+                    a.e00 = cconj(a.e00); a.e01 = cconj(a.e01); a.e02 = cconj(a.e02);
+                    a.e11 = cconj(a.e11); a.e12 = cconj(a.e12); a.e22 = cconj(a.e22);
+                    w[k] = ahat_apply(a, w[k]);
+                }
+            }
+}
+
+void orc_fft3d(int ni, int nj, int nl, const real *x, real *spec) {
+    orc_fft3 f;
+    fft3_init(&f, ni, nj, nl);
+    copy_back(ni, nj, nl, (const cvec *)x, (cvec *)spec);
+    fft3_forward(&f, (cvec *)spec);
+    fft3_free(&f);
+}
+
+void orc_ifft3d(int ni, int nj, int nl, const real *spec, real *y) {
+    orc_fft3 f;
+    size_t P = (size_t)8 * ni * nj * nl;
+    cvec *w = (cvec *)malloc(sizeof(cvec) * P);
+    fft3_init(&f, ni, nj, nl);
+    memcpy(w, spec, sizeof(cvec) * P);
+    fft3_inverse(&f, w);
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++)
+            memcpy((cvec *)y + idx3(i, j, 0, nj, nl), &w[idx3(i, j, 0, 2 * nj, 2 * nl)], sizeof(cvec) * nl);
+    fft3_free(&f); free(w);
+}
+
+/* Convolution (dda_func.cpp:150-187) / Convolution_conj (:227-437) on an
+   N-cell field: y = first octant of IFFT(Â ⊙ FFT(pad x)). */
+static void convolve(orc_fft3 *f, const cmat *A, const cvec *x, cvec *w, cvec *y, int conj) {
+    int ni = f->ni, nj = f->nj, nl = f->nl;
+    copy_back(ni, nj, nl, x, w);
+    fft3_forward(f, w);
+    pointwise(2 * ni, 2 * nj, 2 * nl, A, w, conj);
+    fft3_inverse(f, w);
+    if (y)
+        for (int i = 0; i < ni; i++)
+            for (int j = 0; j < nj; j++)
+                memcpy(&y[idx3(i, j, 0, nj, nl)], &w[idx3(i, j, 0, 2 * nj, 2 * nl)], sizeof(cvec) * nl);
+}
+
+void orc_convolution(int ni, int nj, int nl, const real *ahat, const real *x, real *y, int conj) {
+    orc_fft3 f;
+    cvec *w = (cvec *)malloc(sizeof(cvec) * (size_t)8 * ni * nj * nl);
+    fft3_init(&f, ni, nj, nl);
+    convolve(&f, (const cmat *)ahat, (const cvec *)x, w, (cvec *)y, conj);
+    fft3_free(&f); free(w);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solver building blocks: dda_func.cpp:654-974                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int ni, nj, nl;
+    const int *mv;
+    const cplx *alpha;
+    const signed char *occ;
+} orc_sys;
+
+static inline int occupied(const orc_sys *s, size_t k) { return s->mv[(int)s->occ[k]] != 0; }
+static inline size_t ncell(const orc_sys *s) { return (size_t)s->ni * s->nj * s->nl; }
+
+/* dda_func.cpp:654-689 */
+void orc_first_approx(int ni, int nj, int nl, const int *mv, const real *alpha, const signed char *occ,
+                      real d0, const real *kinc, const real *einc, real *p0) {
+    vec3 c = {(real)((ni - 1) * d0 / 2.), (real)((nj - 1) * d0 / 2.), (real)((nl - 1) * d0 / 2.)};
+    const cplx *al = (const cplx *)alpha;
+    const cvec *E = (const cvec *)einc;
+    cvec *P = (cvec *)p0;
+    for (int i = 0; i < ni; i++) {
+        real rx = (real)i * d0;
+        for (int j = 0; j < nj; j++) {
+            real ry = (real)j * d0;
+            for (int l = 0; l < nl; l++) {
+                real rz = (real)l * d0;
+                size_t k = idx3(i, j, l, nj, nl);
+                int m = (int)occ[k];
+                if (mv[m] != 0) {
+                    real dx = rx - c.x, dy = ry - c.y, dz = rz - c.z;
+                    real ph = kinc[0] * dx + kinc[1] * dy + kinc[2] * dz;   /* cvector.cpp:93-98 */
+                    cplx c0 = exp_i(ph), c1 = cmul(c0, al[m]);
+                    P[k].x = cmul(c1, E->x); P[k].y = cmul(c1, E->y); P[k].z = cmul(c1, E->z);
+                } else memset(&P[k], 0, sizeof(cvec));
+            }
+        }
+    }
+}
+
+/* dda_func.cpp:724-755 ldcScalMul: Σ x·y unconjugated, float products, long double sum */
+static ldcplx dot_u(const orc_sys *s, const cvec *a, const cvec *b) {
+    ldcplx t = {0, 0};
+    size_t n = ncell(s);
+    for (size_t k = 0; k < n; k++)
+        if (occupied(s, k)) {
+            long double c0r = (long double)(a[k].x.r * b[k].x.r - a[k].x.i * b[k].x.i);
+            long double c0i = (long double)(a[k].x.r * b[k].x.i + a[k].x.i * b[k].x.r);
+            long double c1r = (long double)(a[k].y.r * b[k].y.r - a[k].y.i * b[k].y.i);
+            long double c1i = (long double)(a[k].y.r * b[k].y.i + a[k].y.i * b[k].y.r);
+            long double c2r = (long double)(a[k].z.r * b[k].z.r - a[k].z.i * b[k].z.i);
+            long double c2i = (long double)(a[k].z.r * b[k].z.i + a[k].z.i * b[k].z.r);
+            t.r += (c0r + c1r + c2r);
+            t.i += (c0i + c1i + c2i);
+        }
+    return t;
+}
+
+/* dda_func.cpp:757-782 ldScalMulConj: Re Σ x·conj(y) */
+static long double dot_c(const orc_sys *s, const cvec *a, const cvec *b) {
+    long double t = 0;
+    size_t n = ncell(s);
+    for (size_t k = 0; k < n; k++)
+        if (occupied(s, k)) {
+            long double l0 = (long double)(a[k].x.r * b[k].x.r + a[k].x.i * b[k].x.i);
+            long double l1 = (long double)(a[k].y.r * b[k].y.r + a[k].y.i * b[k].y.i);
+            long double l2 = (long double)(a[k].z.r * b[k].z.r + a[k].z.i * b[k].z.i);
+            t += (l0 + l1 + l2);
+        }
+    return t;
+}
+
+/* dda_func.cpp:784-811 ldError: mean over dipoles of |R_j| */
+static long double err_mean(const orc_sys *s, const cvec *R) {
+    long double t = 0;
+    long n = 0;
+    size_t nc = ncell(s);
+    for (size_t k = 0; k < nc; k++)
+        if (occupied(s, k)) {
+            n++;
+            long double l0 = (long double)(R[k].x.r * R[k].x.r + R[k].x.i * R[k].x.i);
+            long double l1 = (long double)(R[k].y.r * R[k].y.r + R[k].y.i * R[k].y.i);
+            long double l2 = (long double)(R[k].z.r * R[k].z.r + R[k].z.i * R[k].z.i);
+            t = t + (long double)sqrtl(l0 + l1 + l2);
+        }
+    return t / (long double)n;
+}
+
+static inline cvec cv_add(cvec a, cvec b) { cvec o; o.x = cadd(a.x, b.x); o.y = cadd(a.y, b.y); o.z = cadd(a.z, b.z); return o; }
+static inline cvec cv_sub(cvec a, cvec b) { cvec o; o.x = csub(a.x, b.x); o.y = csub(a.y, b.y); o.z = csub(a.z, b.z); return o; }
+static inline cvec cv_cmul(cplx c, cvec v) { cvec o; o.x = cmul(c, v.x); o.y = cmul(c, v.y); o.z = cmul(c, v.z); return o; }
+static inline cvec cv_rmul(real r, cvec v) { cvec o; o.x = rmul(r, v.x); o.y = rmul(r, v.y); o.z = rmul(r, v.z); return o; }
+static const cvec CV0 = {{0, 0}, {0, 0}, {0, 0}};
+
+/* dda_func.cpp:813-841 LinearComb(c, cv1, cv2, cv3): cv3 = cv1 + c·cv2, masked */
+static void lc3(const orc_sys *s, cplx c, const cvec *a, const cvec *b, cvec *o) {
+    size_t n = ncell(s);
+    for (size_t k = 0; k < n; k++) o[k] = occupied(s, k) ? cv_add(a[k], cv_cmul(c, b[k])) : CV0;
+}
+/* dda_func.cpp:843-882 LinearComb(c1, c2, cv1..cv4): cv4 = c2·cv3 + (cv1 + c1·cv2) */
+static void lc4(const orc_sys *s, cplx c1, cplx c2, const cvec *a, const cvec *b, const cvec *c, cvec *o) {
+    size_t n = ncell(s);
+    for (size_t k = 0; k < n; k++)
+        if (occupied(s, k)) {
+            cvec t0 = cv_cmul(c1, b[k]), t1 = cv_cmul(c2, c[k]), t2 = cv_add(a[k], t0);
+            o[k] = cv_add(t1, t2);
+        } else o[k] = CV0;
+}
+/* dda_func.cpp:884-925 mode 1: cv1 += c·cv2 ; mode 2: cv2 = cv1 + c·cv2 */
+static void lc_c(const orc_sys *s, cplx c, cvec *a, cvec *b, int mode) {
+    size_t n = ncell(s);
+    for (size_t k = 0; k < n; k++)
+        if (occupied(s, k)) {
+            if (mode == 1) a[k] = cv_add(a[k], cv_cmul(c, b[k]));
+            else b[k] = cv_add(a[k], cv_cmul(c, b[k]));
+        }
+}
+/* dda_func.cpp:927-974 real coefficient */
+static void lc_r(const orc_sys *s, real c, cvec *a, cvec *b, int mode) {
+    size_t n = ncell(s);
+    for (size_t k = 0; k < n; k++)
+        if (occupied(s, k)) {
+            if (mode == 1) a[k] = cv_add(a[k], cv_rmul(c, b[k]));
+            else b[k] = cv_add(a[k], cv_rmul(c, b[k]));
+        }
+}
+
+/* Matvec epilogue used throughout DDA_FFT (e.g. dda_func.cpp:1335-1356):
+   out = x - α_m·(A x) on dipoles, 0 elsewhere; conj uses conj(α_m). */
+static void matvec_apply(orc_fft3 *f, const cmat *A, const orc_sys *s, const cvec *x, cvec *w, cvec *out,
+                         int conj) {
+    int ni = s->ni, nj = s->nj, nl = s->nl;
+    convolve(f, A, x, w, NULL, conj);
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++)
+            for (int l = 0; l < nl; l++) {
+                size_t k = idx3(i, j, l, nj, nl);
+                int m = (int)s->occ[k];
+                cplx a = conj ? cconj(s->alpha[m]) : s->alpha[m];
+                if (s->mv[m] != 0) out[k] = cv_sub(x[k], cv_cmul(a, w[idx3(i, j, l, 2 * nj, 2 * nl)]));
+                else out[k] = CV0;
+            }
+}
+
+void orc_matvec(int ni, int nj, int nl, const real *ahat, const int *mv, const real *alpha,
+                const signed char *occ, const real *x, real *y, int conj) {
+    orc_sys s = {ni, nj, nl, mv, (const cplx *)alpha, occ};
+    orc_fft3 f;
+    cvec *w = (cvec *)malloc(sizeof(cvec) * (size_t)8 * ni * nj * nl);
+    fft3_init(&f, ni, nj, nl);
+    matvec_apply(&f, (const cmat *)ahat, &s, (const cvec *)x, w, (cvec *)y, conj);
+    fft3_free(&f); free(w);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cross sections and far field: dda_func.cpp:1576-1625, 1679-1727          */
+/* ------------------------------------------------------------------------ */
+
+void orc_cabs_cext(real d0, const real *kinc, const real *einc, int ni, int nj, int nl, const real *alpha,
+                   const signed char *occ, const real *p, real *cabs, real *cext) {
+    const cvec *E = (const cvec *)einc;
+    const cvec *P = (const cvec *)p;
+    const cplx *al = (const cplx *)alpha;
+    /* r0 = 4π / Re(E·conj(E)), computed in double then stored as real */
+    cvec ce; ce.x = cconj(E->x); ce.y = cconj(E->y); ce.z = cconj(E->z);
+    cplx ee = cadd(cadd(cmul(E->x, ce.x), cmul(E->y, ce.y)), cmul(E->z, ce.z));
+    real r0 = (real)((4. * PI_D) / ee.r), r1 = 0., r2 = 0., r4 = (real)(2. / 3.);
+    vec3 c = {(real)(ni * d0 / 2.), (real)(nj * d0 / 2.), (real)(nl * d0 / 2.)};
+    for (int i = 0; i < ni; i++) {
+        real rx = (real)i * d0;
+        for (int j = 0; j < nj; j++) {
+            real ry = (real)j * d0;
+            for (int l = 0; l < nl; l++) {
+                real rz = (real)l * d0;
+                size_t k = idx3(i, j, l, nj, nl);
+                int m = (int)occ[k];
+                if (al[m].r == 0. && al[m].i == 0.) continue;
+                real r3 = -(kinc[0] * (rx - c.x) + kinc[1] * (ry - c.y) + kinc[2] * (rz - c.z));
+                cplx c0 = exp_i(r3);
+                cplx c1 = cmul(P[k].x, ce.x), c2 = cmul(P[k].y, ce.y), c3 = cmul(P[k].z, ce.z);
+                cplx c4 = cadd(c1, c2), c5 = cadd(c3, c4);
+                c1 = cmul(c0, c5);
+                r1 += c1.i;
+                /* Cmuls(x,x): |x|^2 in the real part */
+                c1.r = P[k].x.r * P[k].x.r + P[k].x.i * P[k].x.i; c1.i = P[k].x.i * P[k].x.r - P[k].x.r * P[k].x.i;
+                c2.r = P[k].y.r * P[k].y.r + P[k].y.i * P[k].y.i; c2.i = P[k].y.i * P[k].y.r - P[k].y.r * P[k].y.i;
+                c3.r = P[k].z.r * P[k].z.r + P[k].z.i * P[k].z.i; c3.i = P[k].z.i * P[k].z.r - P[k].z.r * P[k].z.i;
+                c4 = cadd(c1, c2); c0 = cadd(c3, c4);
+                c1.r = al[m].r; c1.i = -al[m].i;
+                /* cplx.h Cdiv */
+                real nn = c1.r * c1.r + c1.i * c1.i;
+                c2.r = (c0.r * c1.r + c0.i * c1.i) / nn;
+                c2.i = (-c0.r * c1.i + c0.i * c1.r) / nn;
+                r2 += (c2.i - r4 * c0.r);
+            }
+        }
+    }
+    *cext = r0 * r1;
+    *cabs = r0 * r2;
+}
+
+void orc_far_field(int ni, int nj, int nl, const int *mv, const signed char *occ, real d0, const real *p,
+                   const real *dir, real *esc) {
+    vec3 c = {(real)((ni - 1) * d0 / 2.), (real)((nj - 1) * d0 / 2.), (real)((nl - 1) * d0 / 2.)};
+    const cvec *P = (const cvec *)p;
+    real kx = dir[0], ky = dir[1], kz = dir[2];
+    cvec t = CV0;
+    for (int i = 0; i < ni; i++) {
+        real rx = (real)i * d0 - c.x;
+        for (int j = 0; j < nj; j++) {
+            real ry = (real)j * d0 - c.y;
+            for (int l = 0; l < nl; l++) {
+                size_t k = idx3(i, j, l, nj, nl);
+                if (mv[(int)occ[k]] == 0) continue;
+                real rz = (real)l * d0 - c.z;
+                real r0 = kx * rx + ky * ry + kz * rz;
+                cplx c0; c0.r = rcos(r0); c0.i = rsin(r0);
+                cplx c1 = rmul(kx, P[k].x), c2 = rmul(ky, P[k].y), c3 = rmul(kz, P[k].z);
+                cplx c4 = cadd(c1, c2); c1 = cadd(c3, c4);
+                c2 = rmul(kx, c1); c3 = rmul(ky, c1); c4 = rmul(kz, c1);
+                cplx c5 = csub(P[k].x, c2), c6 = csub(P[k].y, c3), c7 = csub(P[k].z, c4);
+                c2 = cmul(c0, c5); c3 = cmul(c0, c6); c4 = cmul(c0, c7);
+                t.x = cadd(t.x, c2); t.y = cadd(t.y, c3); t.z = cadd(t.z, c4);
+            }
+        }
+    }
+    memcpy(esc, &t, sizeof(t));
+}
+
+/* ------------------------------------------------------------------------ */
+/* DDA_FFT: dda_func.cpp:1075-1570                                           */
+/* ------------------------------------------------------------------------ */
+
+/* err_hist[0..max_step] receives the discrepancy printed after each step;
+   info[0] = last step executed, info[1] = step_min, info[2] = 1 if the best
+   iterate was restored, info[3] = number of matvecs. p_out (optional) gets the
+   final dipole moments. */
+int orc_dda_fft(int solver, real tau, int ni, int nj, int nl, const real *alpha, const int *mv,
+                const signed char *occ, int max_step, real err_max, real d0, const real *kinc,
+                const real *einc, const real *ahat, int ndirs, const real *dirs, real *esc, real *cabs,
+                real *cext, double *err_hist, int *info, real *p_out) {
+    orc_sys s = {ni, nj, nl, mv, (const cplx *)alpha, occ};
+    size_t n = ncell(&s);
+    const cmat *A = (const cmat *)ahat;
+    orc_fft3 f;
+    if (fft3_init(&f, ni, nj, nl)) return -1;
+    cvec *P = (cvec *)calloc(n, sizeof(cvec)), *Pmin = (cvec *)calloc(n, sizeof(cvec)),
+         *R = (cvec *)calloc(n, sizeof(cvec)), *S = (cvec *)calloc(n, sizeof(cvec)),
+         *G = (cvec *)calloc(n, sizeof(cvec)), *H = (cvec *)calloc(n, sizeof(cvec)),
+         *Wk = (cvec *)calloc(n, sizeof(cvec)), *w = (cvec *)calloc((size_t)8 * n, sizeof(cvec));
+    int nmv = 0;
+    long double Error, Error_min, RR_1 = 0, RR, SS;
+    ldcplx ldcA = {0, 0}, ldcB, ldcW = {0, 0}, ldcRho_1 = {0, 0}, ldcRho, ldc0, ldc1, den;
+    cplx cA, cB, cW;
+    int step_min = 0, o, last = 0;
+
+    orc_first_approx(ni, nj, nl, mv, alpha, occ, d0, kinc, einc, (real *)P);
+    /* :1105-1135 R = -α·(A P) */
+    convolve(&f, A, P, w, NULL, 0); nmv++;
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++)
+            for (int l = 0; l < nl; l++) {
+                size_t k = idx3(i, j, l, nj, nl);
+                int m = (int)occ[k];
+                if (mv[m] != 0) {
+                    cvec t = cv_cmul(s.alpha[m], w[idx3(i, j, l, 2 * nj, 2 * nl)]);
+                    R[k].x = cneg(t.x); R[k].y = cneg(t.y); R[k].z = cneg(t.z);
+                } else R[k] = CV0;
+            }
+
+    if (solver == 0) memcpy(S, R, sizeof(cvec) * n);                       /* :1137-1138 */
+    if (solver == 1) {                                                    /* :1140-1176 */
+        RR_1 = dot_c(&s, R, R);
+        matvec_apply(&f, A, &s, R, w, S, 1); nmv++;
+    }
+    if (solver == 2) {                                                    /* :1178-1264 */
+        memcpy(G, R, sizeof(cvec) * n);
+        memcpy(S, R, sizeof(cvec) * n);
+        ldcRho_1 = dot_u(&s, R, G);
+        matvec_apply(&f, A, &s, S, w, H, 0); nmv++;
+        ldc0 = dot_u(&s, G, H);
+        ldcA = lddiv(ldcRho_1, ldc0);
+        cA = ld2c(ldcA);
+        lc_c(&s, cneg(cA), R, H, 1);
+        matvec_apply(&f, A, &s, R, w, Wk, 0); nmv++;
+        ldc0 = dot_u(&s, Wk, R);
+        ldc1 = dot_u(&s, Wk, Wk);
+        ldcW = lddiv(ldc0, ldc1);
+        cW = ld2c(ldcW);
+        lc4(&s, cneg(cA), cneg(cW), P, S, R, P);
+        lc_c(&s, cneg(cW), R, Wk, 1);
+    }
+    if (solver == 3) {                                                    /* :1266-1278 */
+        memcpy(S, P, sizeof(cvec) * n);
+        for (size_t k = 0; k < n; k++) P[k] = cv_sub(S[k], R[k]);
+    }
+
+    Error = err_mean(&s, R);                                              /* :1280 */
+    if (err_hist) err_hist[0] = (double)Error;
+    Error_min = Error; step_min = 0;
+    memcpy(Pmin, P, sizeof(cvec) * n);                                    /* :1290-1293 */
+
+    for (o = 1; o <= max_step; o++) {                                     /* :1301-1535 */
+        if (Error <= (long double)err_max) break;
+        if (Error < Error_min) { Error_min = Error; step_min = o - 1; memcpy(Pmin, P, sizeof(cvec) * n); }
+        if (solver == 2) {
+            ldcRho = dot_u(&s, R, G);
+            ldcB = lddiv(ldmul(ldcRho, ldcA), ldmul(ldcRho_1, ldcW));
+            cB = ld2c(ldcB);
+            lc4(&s, cB, cneg(cmul(cB, cW)), R, S, H, S);
+        }
+        if (solver == 0 || solver == 1 || solver == 2) { matvec_apply(&f, A, &s, S, w, Wk, 0); nmv++; }
+        if (solver == 0) {                                                /* :1359-1371 */
+            den = dot_u(&s, S, Wk);
+            ldcA = lddiv(dot_u(&s, R, S), den);
+            cA = ld2c(ldcA);
+            lc_c(&s, cneg(cA), P, S, 1);
+            lc_c(&s, cneg(cA), R, Wk, 1);
+            ldcB = lddiv(dot_u(&s, R, Wk), den);
+            cB = ld2c(ldcB);
+            lc_c(&s, cneg(cB), R, S, 2);
+        }
+        if (solver == 1) {                                                /* :1373-1418 */
+            SS = dot_c(&s, S, S);
+            real rA = (real)(-RR_1 / SS);
+            lc_r(&s, rA, R, Wk, 1);
+            lc_r(&s, rA, P, S, 1);
+            RR = dot_c(&s, R, R);
+            real rB = (real)(RR / RR_1); RR_1 = RR;
+            matvec_apply(&f, A, &s, R, w, Wk, 1); nmv++;
+            lc_r(&s, rB, Wk, S, 2);
+        }
+        if (solver == 2) {                                                /* :1420-1469 */
+            memcpy(H, Wk, sizeof(cvec) * n);
+            ldc0 = dot_u(&s, G, H);
+            ldcA = lddiv(ldcRho, ldc0);
+            cA = ld2c(ldcA);
+            lc_c(&s, cneg(cA), R, H, 1);
+            matvec_apply(&f, A, &s, R, w, Wk, 0); nmv++;
+            ldc0 = dot_u(&s, Wk, R);
+            ldc1 = dot_u(&s, Wk, Wk);
+            ldcW = lddiv(ldc0, ldc1);
+            cW = ld2c(ldcW);
+            lc4(&s, cneg(cA), cneg(cW), P, S, R, P);
+            lc_c(&s, cneg(cW), R, Wk, 1);
+            ldcRho_1 = ldcRho;
+        }
+        if (solver == 3) {                                                /* :1471-1521 */
+            convolve(&f, A, P, w, NULL, 0); nmv++;
+            for (int i = 0; i < ni; i++)
+                for (int j = 0; j < nj; j++)
+                    for (int l = 0; l < nl; l++) {
+                        size_t k = idx3(i, j, l, nj, nl);
+                        int m = (int)occ[k];
+                        if (mv[m] != 0) {
+                            cvec t = cv_sub(P[k], cv_cmul(s.alpha[m], w[idx3(i, j, l, 2 * nj, 2 * nl)]));
+                            R[k] = cv_sub(t, S[k]);
+                            P[k] = cv_sub(P[k], cv_rmul(tau, R[k]));
+                        } else { P[k] = CV0; R[k] = CV0; }
+                    }
+        }
+        Error = err_mean(&s, R);
+        if (err_hist) err_hist[o] = (double)Error;
+        last = o;
+    }
+    int restored = 0;
+    if (Error > Error_min) { memcpy(P, Pmin, sizeof(cvec) * n); restored = 1; }   /* :1537-1548 */
+
+    orc_cabs_cext(d0, kinc, einc, ni, nj, nl, alpha, occ, (const real *)P, cabs, cext);
+    for (int d = 0; d < ndirs; d++) orc_far_field(ni, nj, nl, mv, occ, d0, (const real *)P, dirs + 3 * d, esc + 6 * d);
+    if (info) { info[0] = last; info[1] = step_min; info[2] = restored; info[3] = nmv; }
+    if (p_out) memcpy(p_out, P, sizeof(cvec) * n);
+
+    fft3_free(&f);
+    free(P); free(Pmin); free(R); free(S); free(G); free(H); free(Wk); free(w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Polarizability: dda_func.cpp:1631-1677; selection main.cpp:606-631        */
+/* ------------------------------------------------------------------------ */
+
+static cplx pol_cm(real d0, cplx m) {
+    cplx mm = cmul(m, m);
+    real f = (real)((double)(d0 * d0 * d0) * (0.75 / PI_D));
+    cplx num = mm; num.r = mm.r - (real)1;
+    cplx den = mm; den.r = (real)2 + mm.r;
+    return cdiv(rmul(f, num), den);
+}
+
+static cplx radiative(cplx a) {     /* a / (1 - (2/3) i a) */
+    cplx i23 = {0, 0}, ci = {0, 1};
+    i23 = rmul((real)(2. / 3.), ci);
+    cplx t = cmul(i23, a);
+    cplx d; d.r = (real)1. - t.r; d.i = -t.i;
+    return cdiv(a, d);
+}
+
+static cplx ldr_core(real d0, cplx m, cplx a, real S) {
+    const real b1 = (real)-1.8915316, b2 = (real)0.1648469, b3 = (real)-1.7700004;
+    cplx eps = cmul(m, m), ci = {0, 1};
+    cplx t = rmul(b2, eps); t.r = b1 + t.r;                      /* b1 + b2 ε */
+    t = cadd(t, rmul(S * b3, eps));                                /* + S b3 ε */
+    t = csub(t, rmul((real)((2. / 3.) * d0), ci));                 /* - (2/3) d i */
+    cplx ad; ad.r = a.r / d0; ad.i = a.i / d0;
+    cplx u = cmul(ad, t); u.r = (real)1 + u.r;
+    return cdiv(a, u);
+}
+
+void orc_polarizability(int model, real d0, const real *m, const real *kinc, const real *einc, real *alpha) {
+    cplx n = {m[0], m[1]};
+    cplx a = pol_cm(d0, n);
+    if (model == 2) a = radiative(a);
+    if (model == 3) {
+        real b = (real)(-sqrt(4. * PI_D / 3.) / d0);
+        cplx t = rmul(b, a); t.r = (real)1. + t.r;
+        a = radiative(cdiv(a, t));
+    }
+    if (model == 4) a = ldr_core(d0, n, a, (real).2);
+    if (model == 5) {
+        real rx = kinc[0] * einc[0], ry = kinc[1] * einc[2], rz = kinc[2] * einc[4];
+        a = ldr_core(d0, n, a, rx * rx + ry * ry + rz * rz);
+    }
+    alpha[0] = a.r; alpha[1] = a.i;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Jones / Mueller: mmuller.cpp:175-314                                      */
+/* ------------------------------------------------------------------------ */
+
+static vec3 cross(vec3 a, vec3 b) {
+    vec3 c; c.x = a.y * b.z - a.z * b.y; c.y = a.z * b.x - a.x * b.z; c.z = a.x * b.y - a.y * b.x; return c;
+}
+static cplx proj(vec3 v, const cplx *e) {      /* DCmul x3, then c2 + (c0 + c1) */
+    cplx c0 = rmul(v.x, e[0]), c1 = rmul(v.y, e[1]), c2 = rmul(v.z, e[2]);
+    return cadd(c2, cadd(c0, c1));
+}
+static cplx cdiv_m(cplx x, cplx y) {            /* cplx.h Cdiv */
+    real n = y.r * y.r + y.i * y.i;
+    cplx c; c.r = (x.r * y.r + x.i * y.i) / n; c.i = (-x.r * y.i + x.i * y.r) / n; return c;
+}
+
+void orc_muller(const real *kinc, const real *ksc, const real *normal, const real *einc0, const real *einc1,
+                const real *esc0, const real *esc1, real *mm) {
+    vec3 k = {kinc[0], kinc[1], kinc[2]}, s = {ksc[0], ksc[1], ksc[2]}, nv = {normal[0], normal[1], normal[2]};
+    vec3 v0 = cross(k, nv);
+    real a0 = rsqrt_(v0.x * v0.x + v0.y * v0.y + v0.z * v0.z);
+    real f0 = (real)(1. / a0);
+    vec3 pin = {f0 * v0.x, f0 * v0.y, f0 * v0.z};
+    v0 = cross(s, nv);
+    real a1 = rsqrt_(v0.x * v0.x + v0.y * v0.y + v0.z * v0.z);
+    real f1 = (real)(-1. / a1);
+    vec3 psc = {f1 * v0.x, f1 * v0.y, f1 * v0.z};
+    const cplx *E0 = (const cplx *)einc0, *E1 = (const cplx *)einc1, *S0 = (const cplx *)esc0, *S1 = (const cplx *)esc1;
+    cplx cc0 = proj(nv, E0), cc1 = proj(nv, E1), cc2 = proj(pin, E0), cc3 = proj(pin, E1);
+    cplx c3 = csub(cmul(cc0, cc3), cmul(cc1, cc2));
+    cplx den; den.r = -c3.i; den.i = c3.r;
+    cplx cc4 = proj(nv, S0), cc5 = proj(nv, S1), cc6 = proj(psc, S0), cc7 = proj(psc, S1);
+    cplx J00 = cdiv_m(csub(cmul(cc7, cc0), cmul(cc6, cc1)), den);
+    cplx J01 = cdiv_m(csub(cmul(cc6, cc3), cmul(cc7, cc2)), den);
+    cplx J10 = cdiv_m(csub(cmul(cc5, cc0), cmul(cc4, cc1)), den);
+    cplx J11 = cdiv_m(csub(cmul(cc4, cc3), cmul(cc5, cc2)), den);
+    cplx a = cmul(J00, cconj(J00)), b = cmul(J01, cconj(J01)), c = cmul(J10, cconj(J10)), d = cmul(J11, cconj(J11));
+    cplx x01 = cmul(J00, cconj(J01)), x1110 = cmul(J11, cconj(J10));
+    cplx x0010 = cmul(J00, cconj(J10)), x1101 = cmul(J11, cconj(J01));
+    cplx x1100 = cmul(J11, cconj(J00)), x0110 = cmul(J01, cconj(J10));
+    cplx x0011 = cmul(J00, cconj(J11)), x1001 = cmul(J10, cconj(J01));
+    cplx x1000 = cmul(J10, cconj(J00));
+    real (*M)[4] = (real(*)[4])mm;
+    M[0][0] = (real)(.5 * cadd(cadd(cadd(a, b), c), d).r);
+    M[0][1] = (real)(.5 * csub(cadd(csub(a, b), c), d).r);
+    M[0][2] = cadd(x01, x1110).r;
+    M[0][3] = csub(x01, x1110).i;
+    M[1][0] = (real)(.5 * csub(csub(cadd(a, b), c), d).r);
+    M[1][1] = (real)(.5 * cadd(csub(csub(a, b), c), d).r);
+    M[1][2] = csub(x01, x1110).r;
+    M[1][3] = cadd(x01, x1110).i;
+    M[2][0] = cadd(x0010, x1101).r;
+    M[2][1] = csub(x0010, x1101).r;
+    M[2][2] = cadd(x1100, x0110).r;
+    M[2][3] = cadd(x0011, x1001).i;
+    M[3][0] = cadd(x1000, x1101).i;
+    M[3][1] = csub(x1000, x1101).i;
+    M[3][2] = csub(x1100, x0110).i;
+    M[3][3] = csub(x1100, x0110).r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deterministic targets: particle.cpp:408-527                               */
+/* ------------------------------------------------------------------------ */
+
+int orc_target_ellipsoid(int ni, int nj, int nl, real b_a, real c_a, signed char *occ) {
+    vec3 c = {(real)((real)ni / 2. - .5), (real)((real)nj / 2. - .5), (real)((real)nl / 2. - .5)};
+    real a = (real)ni, b = a * b_a, cc = a * c_a;
+    int cnt = 0;
+    if (b > (real)nj || cc > (real)nl) return -1;
+    a = (real)(a / 2.); b = (real)(b / 2.); cc = (real)(cc / 2.);
+    a = a * a; b = b * b; cc = cc * cc;
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++)
+            for (int l = 0; l < nl; l++) {
+                real x = (real)i - c.x, y = (real)j - c.y, z = (real)l - c.z;
+                int in = (x * x) / a + (y * y) / b + (z * z) / cc <= (real)1.;
+                occ[idx3(i, j, l, nj, nl)] = in ? 1 : 0;
+                cnt += in;
+            }
+    return cnt;
+}
+
+int orc_target_parallelepiped(int ni, int nj, int nl, real b_a, real c_a, signed char *occ) {
+    real a = (real)ni, b = a * b_a, c = a * c_a;
+    int cnt = 0;
+    if (b > (real)nj || c > (real)nl) return -1;
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++)
+            for (int l = 0; l < nl; l++) {
+                int in = (double)(a - (real)i) >= 0.001 && (double)(b - (real)j) >= 0.001 &&
+                         (double)(c - (real)l) >= 0.001;
+                occ[idx3(i, j, l, nj, nl)] = in ? 1 : 0;
+                cnt += in;
+            }
+    return cnt;
+}
+
+int orc_real_bytes(void) { return (int)sizeof(real); }
