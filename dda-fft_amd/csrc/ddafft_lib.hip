@@ -1,0 +1,1017 @@
+// libddafft_amd: the DDA-FFT solve path on MI355X behind the C ABI of
+// include/ddafft.h.  Host code here only stages tiny inputs, builds the
+// constant tables and sequences kernels on the context's stream; every
+// pass over lattice-sized data runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ddafft.h"
+#include "ahat_kernels.hpp"
+#include "device_types.hpp"
+#include "fft_common.hpp"
+#include "matvec_kernels.hpp"
+#include "solver_kernels.hpp"
+
+using namespace ddafft;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHECK(expr)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(DDAFFT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+const double kPi = 3.1415926535897932384626433832795;
+
+bool pow2_ok(int n) { return n >= 4 && n <= 256 && (n & (n - 1)) == 0; }
+
+// FourierPhase (fft_func.cpp:77-88): W_2n^k for k < n by a float recurrence.
+std::vector<float2> fourier_phase(int n) {
+    std::vector<float2> w(n > 2 ? n : 2);
+    const float a = (float)(-2. * kPi / (2 * n));
+    const float2 cw = make_float2(std::cos(a), std::sin(a));
+    w[0] = make_float2(1.f, 0.f);
+    w[1] = cw;
+    for (int i = 2; i < n; ++i)
+        w[i] = make_float2(w[i - 1].x * cw.x - w[i - 1].y * cw.y, w[i - 1].x * cw.y + w[i - 1].y * cw.x);
+    w.resize(n);
+    return w;
+}
+
+// Exact float twiddles W_N^q (fft_kernel.hpp:238-243).
+std::vector<float2> plan_twiddles(int n) {
+    std::vector<float2> t(n);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int q = 0; q < n; ++q) {
+        const double a = -two_pi * (double)q / (double)n;
+        t[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    return t;
+}
+
+// One octant of the Green tensor (dda_func.cpp:92-147) at r = (a d, b d, c d).
+void green_octant(int ni, int nj, int nl, float d0, std::vector<float2>& oct) {
+    oct.assign((size_t)(ni + 1) * (nj + 1) * (nl + 1) * 6, make_float2(0.f, 0.f));
+    for (int a = 0; a <= ni; ++a) {
+        const float rx = (float)a * d0;
+        for (int b = 0; b <= nj; ++b) {
+            const float ry = (float)b * d0;
+            for (int c = 0; c <= nl; ++c) {
+                const float rz = (float)c * d0;
+                float2* o = &oct[(((size_t)a * (nj + 1) + b) * (nl + 1) + c) * 6];
+                const float e00 = rx * rx, e11 = ry * ry, e22 = rz * rz;
+                const float e01 = rx * ry, e02 = rx * rz, e12 = ry * rz;
+                const float r2 = e00 + e11 + e22;
+                if ((double)r2 < 0.0000000001) continue;
+                const float r1 = std::sqrt(r2);
+                const float2 c0 = make_float2(r2 - 1.f, r1);
+                const float2 c1 = make_float2(3.f - r2, -3.f * r1);
+                const float r3 = r1 * r2;
+                float2 c3 = make_float2(std::cos(r1), std::sin(r1));
+                const float2 c2 = make_float2(c3.x / r3, c3.y / r3);
+                c3 = make_float2(c0.x * c2.x - c0.y * c2.y, c0.x * c2.y + c0.y * c2.x);
+                const float2 c4 = make_float2(c1.x * c2.x - c1.y * c2.y, c1.x * c2.y + c1.y * c2.x);
+                const float2 c5 = make_float2(c4.x / r2, c4.y / r2);
+                const float e[6] = {e00, e01, e02, e11, e12, e22};
+                for (int q = 0; q < 6; ++q) o[q] = make_float2(e[q] * c5.x, e[q] * c5.y);
+                o[0] = make_float2(o[0].x + c3.x, o[0].y + c3.y);
+                o[3] = make_float2(o[3].x + c3.x, o[3].y + c3.y);
+                o[5] = make_float2(o[5].x + c3.x, o[5].y + c3.y);
+            }
+        }
+    }
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return DDAFFT_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(DDAFFT_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return DDAFFT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+struct ddafft_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Dims d{};
+    int nrhs_max = 1;
+    float kd = 0.f;
+
+    float2* ahat = nullptr;            // [2nj][2nl][2ni][6]
+    float2* tables = nullptr;
+    const float2 *wpre_i = nullptr, *wpre_j = nullptr, *wpre_l = nullptr;
+    const float2 *tw_i = nullptr, *tw_j = nullptr, *tw_l = nullptr;      // n-point plans
+    const float2 *tw_2i = nullptr, *tw_2j = nullptr, *tw_2l = nullptr;   // 2n-point plans
+
+    // target
+    int8_t* occ = nullptr;
+    int* mv = nullptr;
+    int nmat = 0;
+    int* cells = nullptr;   // dipole cell indices (far field)
+    int ncells = 0;
+    long long ndip = 0;
+    bool have_target = false;
+
+    // matvec scratch
+    float2 *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
+    // solver fields [nrhs][3][n]
+    float2 *P = nullptr, *Pmin = nullptr, *R = nullptr, *S = nullptr, *G = nullptr, *H = nullptr, *W = nullptr;
+    float2 *X = nullptr, *Y = nullptr;
+    float2* alpha = nullptr;           // [nrhs][nmat]
+    float2* einc = nullptr;            // [nrhs][3]
+    SolveState* st = nullptr;          // [nrhs]
+    double* hist = nullptr;
+    int hist_cap = 0;
+    double* partials = nullptr;
+    size_t partials_cap = 0;
+    float* dirs = nullptr;
+    int ndirs = 0, dirs_cap = 0;
+    float2* esc = nullptr;
+    int esc_cap = 0;
+    int* stop_dev = nullptr;
+    int* stop_host = nullptr;          // pinned ring
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+int ensure_partials(ddafft_ctx* c, size_t doubles) {
+    if (doubles <= c->partials_cap) return DDAFFT_OK;
+    if (c->stream) HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->partials) HIPCHECK(hipFree(c->partials));
+    c->partials = nullptr;
+    c->partials_cap = 0;
+    int rc = dev_alloc(&c->partials, doubles);
+    if (rc) return rc;
+    c->partials_cap = doubles;
+    return DDAFFT_OK;
+}
+
+// grid sizes of the five passes (tile widths as in matvec_kernels.hpp)
+int tile_zf(int n) {
+    switch (n) {
+        case 4: return TileZF<4>::T;
+        case 8: return TileZF<8>::T;
+        case 16: return TileZF<16>::T;
+        case 32: return TileZF<32>::T;
+        case 64: return TileZF<64>::T;
+        case 128: return TileZF<128>::T;
+        case 256: return TileZF<256>::T;
+    }
+    return 1;
+}
+int tile_x(int n) {
+    switch (n) {
+        case 4: return TileX<4>::T;
+        case 8: return TileX<8>::T;
+        case 16: return TileX<16>::T;
+        case 32: return TileX<32>::T;
+        case 64: return TileX<64>::T;
+        case 128: return TileX<128>::T;
+        case 256: return TileX<256>::T;
+    }
+    return 1;
+}
+int tile_yi(int n2) {
+    switch (n2) {
+        case 8: return TileYI<8>::T;
+        case 16: return TileYI<16>::T;
+        case 32: return TileYI<32>::T;
+        case 64: return TileYI<64>::T;
+        case 128: return TileYI<128>::T;
+        case 256: return TileYI<256>::T;
+        case 512: return TileYI<512>::T;
+    }
+    return 1;
+}
+int tile_zi(int n2) {
+    switch (n2) {
+        case 8: return TileZI<8>::T;
+        case 16: return TileZI<16>::T;
+        case 32: return TileZI<32>::T;
+        case 64: return TileZI<64>::T;
+        case 128: return TileZI<128>::T;
+        case 256: return TileZI<256>::T;
+        case 512: return TileZI<512>::T;
+    }
+    return 1;
+}
+
+int zi_blocks(const Dims& d) {
+    const int tj = std::min(tile_zi(2 * d.nl), d.nj);
+    return d.ni * (d.nj / tj);
+}
+
+#define DDA_CASE_N(N, ...)    \
+    case N: {                 \
+        constexpr int NN = N; \
+        __VA_ARGS__;          \
+    } break;
+
+#define DDA_SWITCH_SMALL(n, ...)                                                                        \
+    switch (n) {                                                                                         \
+        DDA_CASE_N(4, __VA_ARGS__) DDA_CASE_N(8, __VA_ARGS__) DDA_CASE_N(16, __VA_ARGS__)                \
+        DDA_CASE_N(32, __VA_ARGS__) DDA_CASE_N(64, __VA_ARGS__) DDA_CASE_N(128, __VA_ARGS__)             \
+        DDA_CASE_N(256, __VA_ARGS__)                                                                     \
+        default: return fail(DDAFFT_EINVAL, "unsupported line length");                                 \
+    }
+
+#define DDA_SWITCH_LARGE(n, ...)                                                                        \
+    switch (n) {                                                                                         \
+        DDA_CASE_N(8, __VA_ARGS__) DDA_CASE_N(16, __VA_ARGS__) DDA_CASE_N(32, __VA_ARGS__)               \
+        DDA_CASE_N(64, __VA_ARGS__) DDA_CASE_N(128, __VA_ARGS__) DDA_CASE_N(256, __VA_ARGS__)            \
+        DDA_CASE_N(512, __VA_ARGS__)                                                                     \
+        default: return fail(DDAFFT_EINVAL, "unsupported line length");                                 \
+    }
+
+struct PassEvents {
+    hipEvent_t* ev;  // 6 events bracketing the 5 passes, or nullptr
+};
+
+// Host-side check that every pointer a pass will dereference is set, so that
+// a sequencing mistake returns an error instead of faulting the GPU.
+int check_args(const ProArgs& pa, const EpiArgs& ea, bool solver) {
+    bool ok = pa.A != nullptr;
+    if (pa.mode == PRO_LC1 || pa.mode == PRO_LC2) ok = ok && pa.B;
+    if (pa.mode == PRO_LC4) ok = ok && pa.B && pa.D;
+    if ((pa.P == nullptr) != (pa.Pmin == nullptr)) ok = false;
+    switch (ea.kind) {
+        case EK_CONV: ok = ok && ea.out; break;
+        case EK_MV: case EK_BICG_W: case EK_AA_S0: case EK_AA_W: ok = ok && ea.x && ea.out; break;
+        case EK_BICG_H: case EK_CG_W: ok = ok && ea.x && ea.out && ea.y; break;
+        case EK_BICG_R0: ok = ok && ea.out && ea.out2 && ea.out3; break;
+        case EK_CG_R0: ok = ok && ea.out && ea.out2; break;
+        case EK_AA_R0: ok = ok && ea.out; break;
+        case EK_SIM_R0: ok = ok && ea.x && ea.out && ea.out2 && ea.out3; break;
+        case EK_AA_SUPD: ok = ok && ea.x && ea.out2; break;
+        case EK_SIM: ok = ok && ea.x && ea.out && ea.out2 && ea.y; break;
+        default: ok = false;
+    }
+    if (ea.kind != EK_CONV) ok = ok && ea.alpha && ea.nmat > 0;
+    if (solver && ea.kind != EK_CONV && ea.kind != EK_MV && ea.kind != EK_AA_SUPD) ok = ok && ea.partials;
+    if (!ok) return fail(DDAFFT_ESTATE, "internal: matvec launched with a missing buffer");
+    return DDAFFT_OK;
+}
+
+// One matvec of nrhs right-hand sides: ZF -> YF -> X -> YI -> ZI.
+int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, const SolveState* st, int iter, const EpiArgs& ea,
+                  int conj, hipEvent_t* ev = nullptr) {
+    {
+        const int rc = check_args(pa, ea, st != nullptr);
+        if (rc) return rc;
+        if (!c->occ || !c->mv || !c->b1 || !c->b2 || !c->b3 || !c->ahat || !c->tables)
+            return fail(DDAFFT_ESTATE, "internal: context buffers missing");
+    }
+    Dims d = c->d;
+    d.nrhs = nrhs;
+    const int C = 3 * nrhs;
+    hipStream_t s = c->stream;
+    if (ev) HIPCHECK(hipEventRecord(ev[0], s));
+    {
+        const int tj = std::min(tile_zf(d.nl), d.nj);
+        dim3 grid(d.ni * (d.nj / tj), C);
+        DDA_SWITCH_SMALL(d.nl, hipLaunchKernelGGL((k_zf<NN>), grid, kBlock, 0, s, d, pa, st, iter, c->occ, c->mv, c->b1, c->wpre_l,
+                                                                  c->tw_l));
+    }
+    if (ev) HIPCHECK(hipEventRecord(ev[1], s));
+    {
+        const int ti = std::min(tile_zf(d.nj), d.ni);
+        dim3 grid(2 * d.nl * (d.ni / ti), C);
+        DDA_SWITCH_SMALL(d.nj, hipLaunchKernelGGL((k_yf<NN>), grid, kBlock, 0, s, d, st, iter, c->b1, c->b2, c->wpre_j, c->tw_j));
+    }
+    if (ev) HIPCHECK(hipEventRecord(ev[2], s));
+    {
+        const int tj = std::min(tile_x(d.ni), 2 * d.nj);
+        dim3 grid(2 * d.nl * (2 * d.nj / tj), nrhs);
+        DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN>), grid, kBlock, 0, s, d, st, iter, c->b2, c->b3, c->ahat, conj, c->wpre_i,
+                                                                 c->tw_i, c->tw_2i));
+    }
+    if (ev) HIPCHECK(hipEventRecord(ev[3], s));
+    {
+        const int tl = std::min(tile_yi(2 * d.nj), 2 * d.nl);
+        dim3 grid(d.ni * (2 * d.nl / tl), C);
+        DDA_SWITCH_LARGE(2 * d.nj, hipLaunchKernelGGL((k_yi<NN>), grid, kBlock, 0, s, d, st, iter, c->b3, c->b1, c->tw_2j));
+    }
+    if (ev) HIPCHECK(hipEventRecord(ev[4], s));
+    {
+        dim3 grid(zi_blocks(d), nrhs);
+        DDA_SWITCH_LARGE(2 * d.nl, hipLaunchKernelGGL((k_zi<NN>), grid, kBlock, 0, s, d, st, iter, c->b1, ea, c->occ, c->mv,
+                                                                      c->tw_2l));
+    }
+    if (ev) HIPCHECK(hipEventRecord(ev[5], s));
+    HIPCHECK(hipGetLastError());
+    return DDAFFT_OK;
+}
+
+int build_ahat(ddafft_ctx* c) {
+    const int ni = c->d.ni, nj = c->d.nj, nl = c->d.nl;
+    std::vector<float2> oct;
+    green_octant(ni, nj, nl, c->kd, oct);
+    float2 *doct = nullptr, *grid = nullptr, *tw = nullptr;
+    const size_t P = (size_t)8 * ni * nj * nl;
+    int rc = dev_alloc(&doct, oct.size());
+    if (!rc) rc = dev_alloc(&grid, P * 6);
+    if (!rc) rc = dev_alloc(&tw, (size_t)(ni + nj + nl));
+    if (rc) {
+        hipFree(doct);
+        hipFree(grid);
+        hipFree(tw);
+        return rc;
+    }
+    std::vector<float2> wi = fourier_phase(ni), wj = fourier_phase(nj), wl = fourier_phase(nl);
+    hipStream_t s = c->stream;
+    hipError_t e = hipMemcpyAsync(doct, oct.data(), oct.size() * sizeof(float2), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tw, wi.data(), ni * sizeof(float2), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tw + ni, wj.data(), nj * sizeof(float2), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tw + ni + nj, wl.data(), nl * sizeof(float2), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        const long long total = (long long)P;
+        const int blocks = (int)std::min<long long>((total + kBlock - 1) / kBlock, 65536);
+        hipLaunchKernelGGL(k_ahat_expand, blocks, kBlock, 0, s, ni, nj, nl, doct, grid);
+        const float norma = (float)(1. / ((2 * ni) * (2 * nj) * (2 * nl)));
+        auto axis_pass = [&](int axis, int na, const float2* w) -> int {
+            int ext = axis == 0 ? 2 * nj : 2 * nl;
+            int outer = axis == 2 ? 2 * nj : 2 * ni;
+            int tt = 1;
+            switch (na) {
+                DDA_CASE_N(8, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
+                DDA_CASE_N(16, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
+                DDA_CASE_N(32, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
+                DDA_CASE_N(64, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
+                DDA_CASE_N(128, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
+                DDA_CASE_N(256, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
+                DDA_CASE_N(512, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
+                default: return fail(DDAFFT_EINVAL, "unsupported axis length");
+            }
+            dim3 g(outer * (ext / tt));
+            DDA_SWITCH_LARGE(na, hipLaunchKernelGGL((k_legacy_axis<NN>), g, kBlock, 0, s, ni, nj, nl, axis, grid, c->ahat, w, norma));
+            return DDAFFT_OK;
+        };
+        rc = axis_pass(0, 2 * nl, tw + ni + nj);
+        if (!rc) rc = axis_pass(1, 2 * nj, tw + ni);
+        if (!rc) rc = axis_pass(2, 2 * ni, tw);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    hipFree(doct);
+    hipFree(grid);
+    hipFree(tw);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(DDAFFT_EHIP, std::string("Â build: ") + hipGetErrorString(e));
+    return DDAFFT_OK;
+}
+
+int upload(ddafft_ctx* c, void* dst, const void* src, size_t bytes) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return DDAFFT_OK;
+}
+
+int field_in(ddafft_ctx* c, const float* host_aos, float2* dev_soa, float2* staging) {
+    const long long n = c->d.n;
+    HIPCHECK(hipMemcpyAsync(staging, host_aos, (size_t)n * 3 * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_aos_to_soa, (int)std::min<long long>((n + kBlock - 1) / kBlock, 4096), kBlock, 0, c->stream, n, staging,
+                                                                                                     dev_soa);
+    HIPCHECK(hipGetLastError());
+    return DDAFFT_OK;
+}
+
+int field_out(ddafft_ctx* c, const float2* dev_soa, float* host_aos, float2* staging) {
+    const long long n = c->d.n;
+    hipLaunchKernelGGL(k_soa_to_aos, (int)std::min<long long>((n + kBlock - 1) / kBlock, 4096), kBlock, 0, c->stream, n, dev_soa,
+                                                                                                     staging);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(host_aos, staging, (size_t)n * 3 * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return DDAFFT_OK;
+}
+
+Geometry make_geometry(const ddafft_ctx* c, const float* kinc) {
+    Geometry g;
+    const float d0 = c->kd;
+    g.d0 = d0;
+    g.kx = kinc ? kinc[0] : 0.f;
+    g.ky = kinc ? kinc[1] : 0.f;
+    g.kz = kinc ? kinc[2] : 1.f;
+    g.cx = (float)((c->d.ni - 1) * d0 / 2.);   // dda_func.cpp:659, 1581
+    g.cy = (float)((c->d.nj - 1) * d0 / 2.);
+    g.cz = (float)((c->d.nl - 1) * d0 / 2.);
+    g.ex = (float)(c->d.ni * d0 / 2.);         // dda_func.cpp:1685
+    g.ey = (float)(c->d.nj * d0 / 2.);
+    g.ez = (float)(c->d.nl * d0 / 2.);
+    return g;
+}
+
+// 4 pi / Re(E . conj(E)) rounded to float (dda_func.cpp:1683)
+float cabs_prefactor(const float* e) {
+    auto cm = [](float ar, float ai, float br, float bi, float& cr, float& ci) {
+        cr = ar * br - ai * bi;
+        ci = ar * bi + ai * br;
+    };
+    float xr, xi, yr, yi, zr, zi;
+    cm(e[0], e[1], e[0], -e[1], xr, xi);
+    cm(e[2], e[3], e[2], -e[3], yr, yi);
+    cm(e[4], e[5], e[4], -e[5], zr, zi);
+    const float re = (xr + yr) + zr;
+    return (float)((4. * kPi) / re);
+}
+
+int update_blocks(const Dims& d) { return (int)std::min<long long>((d.n + kBlock - 1) / kBlock, 1024); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* ddafft_last_error(void) { return g_err.c_str(); }
+
+int ddafft_version(void) { return 100; }
+
+int ddafft_destroy(ddafft_ctx* c) {
+    if (!c) return DDAFFT_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void* bufs[] = {c->ahat, c->tables, c->occ, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
+                    c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
+                    c->dirs, c->esc, c->stop_dev};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    if (c->stop_host) hipHostFree(c->stop_host);
+    for (hipEvent_t& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return DDAFFT_OK;
+}
+
+int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** out) {
+    if (!out) return fail(DDAFFT_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!pow2_ok(ni) || !pow2_ok(nj) || !pow2_ok(nl))
+        return fail(DDAFFT_EINVAL, "lattice dimensions must be powers of two in [4, 256]");
+    if (!(kd > 0.f) || nrhs_max < 1 || nrhs_max > 64) return fail(DDAFFT_EINVAL, "bad kd or nrhs_max");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(DDAFFT_ENODEV, "no HIP device");
+    ddafft_ctx* c = new ddafft_ctx();
+    c->d.ni = ni;
+    c->d.nj = nj;
+    c->d.nl = nl;
+    c->d.nrhs = nrhs_max;
+    c->d.n = (long long)ni * nj * nl;
+    c->nrhs_max = nrhs_max;
+    c->kd = kd;
+    int rc = DDAFFT_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == DDAFFT_OK) rc = fail(DDAFFT_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    chk(hipGetDevice(&c->device), "hipGetDevice");
+    chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (hipEvent_t& e : c->ev) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    if (rc) {
+        ddafft_destroy(c);
+        return rc;
+    }
+    const long long n = c->d.n, P = 8 * n;
+    const long long C = 3LL * nrhs_max;
+    // tables: wpre(ni,nj,nl), tw(ni,nj,nl), tw(2ni,2nj,2nl)
+    const size_t tab = (size_t)4 * (ni + nj + nl);
+    rc = dev_alloc(&c->tables, tab);
+    if (!rc) rc = dev_alloc(&c->ahat, (size_t)P * 6);
+    if (!rc) rc = dev_alloc(&c->b1, (size_t)(2 * n * C));
+    if (!rc) rc = dev_alloc(&c->b2, (size_t)(4 * n * C));
+    if (!rc) rc = dev_alloc(&c->b3, (size_t)(4 * n * C));
+    float2** fields[] = {&c->P, &c->Pmin, &c->R, &c->S, &c->G, &c->H, &c->W, &c->X, &c->Y};
+    for (float2** f : fields)
+        if (!rc) rc = dev_alloc(f, (size_t)(n * C));
+    if (!rc) rc = dev_alloc(&c->einc, (size_t)3 * nrhs_max);
+    if (!rc) rc = dev_alloc(&c->st, (size_t)nrhs_max);
+    if (!rc) rc = dev_alloc(&c->occ, (size_t)n);
+    if (!rc) rc = dev_alloc(&c->stop_dev, 1);
+    if (!rc && hipHostMalloc((void**)&c->stop_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+        rc = fail(DDAFFT_ENOMEM, "hipHostMalloc");
+    if (!rc) rc = ensure_partials(c, (size_t)nrhs_max * std::max<long long>(zi_blocks(c->d), 1024) * kSlots);
+    if (rc) {
+        ddafft_destroy(c);
+        return rc;
+    }
+    {
+        std::vector<float2> t;
+        auto app = [&](const std::vector<float2>& v) { t.insert(t.end(), v.begin(), v.end()); };
+        app(fourier_phase(ni));
+        app(fourier_phase(nj));
+        app(fourier_phase(nl));
+        app(plan_twiddles(ni));
+        app(plan_twiddles(nj));
+        app(plan_twiddles(nl));
+        app(plan_twiddles(2 * ni));
+        app(plan_twiddles(2 * nj));
+        app(plan_twiddles(2 * nl));
+        const float2* b = c->tables;
+        c->wpre_i = b;
+        c->wpre_j = b + ni;
+        c->wpre_l = b + ni + nj;
+        c->tw_i = b + ni + nj + nl;
+        c->tw_j = c->tw_i + ni;
+        c->tw_l = c->tw_j + nj;
+        c->tw_2i = c->tw_l + nl;
+        c->tw_2j = c->tw_2i + 2 * ni;
+        c->tw_2l = c->tw_2j + 2 * nj;
+        hipError_t e = hipMemcpy(c->tables, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            ddafft_destroy(c);
+            return fail(DDAFFT_EHIP, std::string("table upload: ") + hipGetErrorString(e));
+        }
+    }
+    rc = build_ahat(c);
+    if (rc) {
+        ddafft_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return DDAFFT_OK;
+}
+
+int ddafft_get_ahat(ddafft_ctx* c, float* out) {
+    if (!c || !out) return fail(DDAFFT_EINVAL, "NULL argument");
+    const int NI2 = 2 * c->d.ni, NJ2 = 2 * c->d.nj, NL2 = 2 * c->d.nl;
+    const size_t P = (size_t)NI2 * NJ2 * NL2;
+    std::vector<float2> h(P * 6);
+    HIPCHECK(hipMemcpyAsync(h.data(), c->ahat, h.size() * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    float2* o = reinterpret_cast<float2*>(out);
+    for (int J = 0; J < NJ2; ++J)
+        for (int L = 0; L < NL2; ++L)
+            for (int I = 0; I < NI2; ++I)
+                memcpy(&o[(((size_t)I * NJ2 + J) * NL2 + L) * 6], &h[(((size_t)J * NL2 + L) * NI2 + I) * 6],
+                       6 * sizeof(float2));
+    return DDAFFT_OK;
+}
+
+int ddafft_set_target(ddafft_ctx* c, const signed char* occ, int nmat, const int* mv) {
+    if (!c || !occ || !mv || nmat < 1 || nmat > 128) return fail(DDAFFT_EINVAL, "bad target arguments");
+    const long long n = c->d.n;
+    std::vector<int> cells;
+    cells.reserve((size_t)n);
+    for (long long k = 0; k < n; ++k) {
+        const int m = occ[k];
+        if (m < 0 || m >= nmat) return fail(DDAFFT_EINVAL, "occupancy index outside the material table");
+        if (mv[m]) cells.push_back((int)k);
+    }
+    if (cells.empty()) return fail(DDAFFT_EINVAL, "target has no dipoles");
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->mv) hipFree(c->mv);
+    if (c->cells) hipFree(c->cells);
+    if (c->alpha) hipFree(c->alpha);
+    c->mv = nullptr;
+    c->cells = nullptr;
+    c->alpha = nullptr;
+    int rc = dev_alloc(&c->mv, (size_t)nmat);
+    if (!rc) rc = dev_alloc(&c->cells, cells.size());
+    if (!rc) rc = dev_alloc(&c->alpha, (size_t)nmat * c->nrhs_max);
+    if (rc) return rc;
+    c->nmat = nmat;
+    c->ncells = (int)cells.size();
+    c->ndip = (long long)cells.size();
+    if ((rc = upload(c, c->occ, occ, (size_t)n))) return rc;
+    if ((rc = upload(c, c->mv, mv, (size_t)nmat * sizeof(int)))) return rc;
+    if ((rc = upload(c, c->cells, cells.data(), cells.size() * sizeof(int)))) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->have_target = true;
+    return DDAFFT_OK;
+}
+
+int ddafft_set_directions(ddafft_ctx* c, int ndirs, const float* dirs) {
+    if (!c || ndirs < 0 || (ndirs > 0 && !dirs)) return fail(DDAFFT_EINVAL, "bad directions");
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (ndirs > c->dirs_cap) {
+        if (c->dirs) hipFree(c->dirs);
+        c->dirs = nullptr;
+        c->dirs_cap = 0;
+        int rc = dev_alloc(&c->dirs, (size_t)ndirs * 3);
+        if (rc) return rc;
+        c->dirs_cap = ndirs;
+    }
+    c->ndirs = ndirs;
+    if (ndirs) {
+        HIPCHECK(hipMemcpy(c->dirs, dirs, (size_t)ndirs * 3 * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return DDAFFT_OK;
+}
+
+int ddafft_convolution(ddafft_ctx* c, const float* x, float* y, int conj) {
+    if (!c || !x || !y) return fail(DDAFFT_EINVAL, "NULL argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    int rc = field_in(c, x, c->X, c->Y);
+    if (rc) return rc;
+    ProArgs pa{PRO_COPY, c->X, nullptr, nullptr, nullptr, nullptr};
+    EpiArgs ea{};
+    ea.kind = EK_CONV;
+    ea.out = c->W;
+    ea.alpha = c->alpha;
+    ea.nmat = c->nmat;
+    rc = launch_matvec(c, 1, pa, nullptr, 0, ea, conj);
+    if (rc) return rc;
+    return field_out(c, c->W, y, c->Y);
+}
+
+int ddafft_matvec(ddafft_ctx* c, const float* alpha, const float* x, float* y, int conj) {
+    if (!c || !alpha || !x || !y) return fail(DDAFFT_EINVAL, "NULL argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    int rc = upload(c, c->alpha, alpha, (size_t)c->nmat * sizeof(float2));
+    if (!rc) rc = field_in(c, x, c->X, c->Y);
+    if (rc) return rc;
+    ProArgs pa{PRO_COPY, c->X, nullptr, nullptr, nullptr, nullptr};
+    EpiArgs ea{};
+    ea.kind = EK_MV;
+    ea.conj = conj;
+    ea.x = c->X;
+    ea.out = c->W;
+    ea.alpha = c->alpha;
+    ea.nmat = c->nmat;
+    rc = launch_matvec(c, 1, pa, nullptr, 0, ea, conj);
+    if (rc) return rc;
+    return field_out(c, c->W, y, c->Y);
+}
+
+// --- the solve ---------------------------------------------------------------
+
+namespace {
+
+struct Seq {
+    ddafft_ctx* c;
+    int nrhs;
+    FinArgs fa;
+
+    int fin(int kind, int iter, int nblk) {
+        FinArgs f = fa;
+        f.kind = kind;
+        f.nblk = nblk;
+        f.partials = c->partials;
+        f.hist = c->hist;
+        if (!f.partials || !f.hist) return fail(DDAFFT_ESTATE, "solver scratch not allocated");
+        hipLaunchKernelGGL(k_finalize, nrhs, kBlock, 0, c->stream, c->st, iter, f);
+        HIPCHECK(hipGetLastError());
+        return DDAFFT_OK;
+    }
+    int mv(int iter, ProArgs pa, EpiArgs ea, int conj) {
+        ea.alpha = c->alpha;
+        ea.nmat = c->nmat;
+        ea.partials = c->partials;
+        ea.conj = conj;
+        return launch_matvec(c, nrhs, pa, c->st, iter, ea, conj);
+    }
+    int upd(int iter, UpdArgs ua) {
+        Dims d = c->d;
+        d.nrhs = nrhs;
+        ua.partials = c->partials;
+        if (!ua.P || !ua.R || !ua.S || !ua.W || !ua.partials || (ua.kind == UK_BICG && !ua.G))
+            return fail(DDAFFT_ESTATE, "internal: update launched with a missing buffer");
+        dim3 g(update_blocks(d), nrhs);
+        hipLaunchKernelGGL(k_update, g, kBlock, 0, c->stream, d, c->st, iter, ua, c->occ, c->mv);
+        HIPCHECK(hipGetLastError());
+        return DDAFFT_OK;
+    }
+};
+
+#define TRY(x)                  \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_) return rc_;    \
+    } while (0)
+
+ProArgs pro(int mode, float2* A, float2* B = nullptr, const float2* D = nullptr) {
+    return ProArgs{mode, A, B, D, nullptr, nullptr};
+}
+
+EpiArgs epi(int kind, const float2* x, float2* out, float2* out2 = nullptr, float2* out3 = nullptr,
+            const float2* y = nullptr, float tau = 0.f) {
+    EpiArgs e{};
+    e.kind = kind;
+    e.x = x;
+    e.out = out;
+    e.out2 = out2;
+    e.out3 = out3;
+    e.y = y;
+    e.tau = tau;
+    return e;
+}
+
+int step0(Seq& q, int solver) {
+    ddafft_ctx* c = q.c;
+    const int zb = zi_blocks(c->d);
+    const int ub = update_blocks(c->d);
+    switch (solver) {
+        case DDAFFT_SOLVER_CG:
+            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_CG_R0, nullptr, c->R, c->S), 0));
+            TRY(q.fin(FK_CG_E0, 0, zb));
+            break;
+        case DDAFFT_SOLVER_AA:
+            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_AA_R0, nullptr, c->R), 0));
+            TRY(q.fin(FK_AA_0, 0, zb));
+            TRY(q.mv(0, pro(PRO_COPY, c->R), epi(EK_AA_S0, c->R, c->S), 1));
+            TRY(q.fin(FK_AA_E0, 0, zb));
+            break;
+        case DDAFFT_SOLVER_BICGSTAB:
+            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_BICG_R0, nullptr, c->R, c->S, c->G), 0));
+            TRY(q.fin(FK_BICG_0, 0, zb));
+            TRY(q.mv(0, pro(PRO_COPY, c->S), epi(EK_BICG_H, c->S, c->H, nullptr, nullptr, c->G), 0));
+            TRY(q.fin(FK_BICG_A, 0, zb));
+            TRY(q.mv(0, pro(PRO_LC1, c->R, c->H), epi(EK_BICG_W, c->R, c->W), 0));
+            TRY(q.fin(FK_BICG_W, 0, zb));
+            TRY(q.upd(0, UpdArgs{UK_BICG, c->P, c->R, c->S, c->W, c->G, nullptr}));
+            TRY(q.fin(FK_BICG_E, 0, ub));
+            break;
+        case DDAFFT_SOLVER_SIMPLE:
+            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_SIM_R0, c->P, c->R, c->S, c->P), 0));
+            TRY(q.fin(FK_SIM_E, 0, zb));
+            break;
+    }
+    return DDAFFT_OK;
+}
+
+int iteration(Seq& q, int solver, int o, float tau) {
+    ddafft_ctx* c = q.c;
+    const int zb = zi_blocks(c->d);
+    const int ub = update_blocks(c->d);
+    auto snap = [&](ProArgs p) {
+        p.P = c->P;
+        p.Pmin = c->Pmin;
+        return p;
+    };
+    switch (solver) {
+        case DDAFFT_SOLVER_CG: {
+            ProArgs p = (o == 1) ? pro(PRO_COPY, c->S) : pro(PRO_LC2, c->R, c->S);
+            TRY(q.mv(o, snap(p), epi(EK_CG_W, c->S, c->W, nullptr, nullptr, c->R), 0));
+            TRY(q.fin(FK_CG_A, o, zb));
+            TRY(q.upd(o, UpdArgs{UK_CG, c->P, c->R, c->S, c->W, nullptr, nullptr}));
+            TRY(q.fin(FK_CG_E, o, ub));
+            break;
+        }
+        case DDAFFT_SOLVER_AA:
+            TRY(q.mv(o, snap(pro(PRO_COPY, c->S)), epi(EK_AA_W, c->S, c->W), 0));
+            TRY(q.fin(FK_AA_A, o, zb));
+            TRY(q.upd(o, UpdArgs{UK_AA, c->P, c->R, c->S, c->W, nullptr, nullptr}));
+            TRY(q.fin(FK_AA_E, o, ub));
+            TRY(q.mv(o, pro(PRO_COPY, c->R), epi(EK_AA_SUPD, c->R, nullptr, c->S), 1));
+            break;
+        case DDAFFT_SOLVER_BICGSTAB:
+            TRY(q.mv(o, snap(pro(PRO_LC4, c->R, c->S, c->H)), epi(EK_BICG_H, c->S, c->H, nullptr, nullptr, c->G), 0));
+            TRY(q.fin(FK_BICG_A, o, zb));
+            TRY(q.mv(o, pro(PRO_LC1, c->R, c->H), epi(EK_BICG_W, c->R, c->W), 0));
+            TRY(q.fin(FK_BICG_W, o, zb));
+            TRY(q.upd(o, UpdArgs{UK_BICG, c->P, c->R, c->S, c->W, c->G, nullptr}));
+            TRY(q.fin(FK_BICG_E, o, ub));
+            break;
+        case DDAFFT_SOLVER_SIMPLE:
+            TRY(q.mv(o, snap(pro(PRO_COPY, c->P)), epi(EK_SIM, c->P, c->R, c->P, nullptr, c->S, tau), 0));
+            TRY(q.fin(FK_SIM_E, o, zb));
+            break;
+    }
+    return DDAFFT_OK;
+}
+
+__global__ void k_stop_max(const SolveState* st, int nrhs, int* out) {
+    int m = 0;
+    for (int r = 0; r < nrhs; ++r) m = max(m, st[r].stop_at);
+    *out = m;
+}
+
+int observables(ddafft_ctx* c, int nrhs, const float* kinc, const float* einc, int ndirs, const float* ddirs,
+                float* esc, float* cabs, float* cext) {
+    Dims d = c->d;
+    d.nrhs = nrhs;
+    const Geometry g = make_geometry(c, kinc);
+    // cross sections
+    const int cb = (int)std::min<long long>((d.n + kBlock - 1) / kBlock, 512);
+    const int fchunk = ndirs > 0 ? (int)std::max(1, std::min(64, (int)((c->ncells + 8191) / 8192))) : 0;
+    TRY(ensure_partials(c, std::max<size_t>((size_t)nrhs * cb * 2, (size_t)nrhs * fchunk * std::max(ndirs, 1) * 6)));
+    dim3 gc(cb, nrhs);
+    hipLaunchKernelGGL(k_cabs_cext, gc, kBlock, 0, c->stream, d, g, c->occ, c->alpha, c->nmat, c->einc, c->P, c->partials);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> part((size_t)nrhs * cb * 2);
+    HIPCHECK(hipMemcpyAsync(part.data(), c->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (ndirs > 0) {
+        if (nrhs * ndirs * 3 > c->esc_cap) {
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            if (c->esc) hipFree(c->esc);
+            c->esc = nullptr;
+            c->esc_cap = 0;
+            TRY(dev_alloc(&c->esc, (size_t)nrhs * ndirs * 3));
+            c->esc_cap = nrhs * ndirs * 3;
+        }
+        // partials for cabs were copied above (stream-ordered), so reuse is safe
+        dim3 gf(fchunk, (ndirs + kFD - 1) / kFD, nrhs);
+        hipLaunchKernelGGL(k_far_field, gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, ddirs, ndirs, c->P, c->partials);
+        HIPCHECK(hipGetLastError());
+        const int tot = nrhs * ndirs * 3;
+        hipLaunchKernelGGL(k_far_field_sum, (tot + kBlock - 1) / kBlock, kBlock, 0, c->stream, c->partials, fchunk, ndirs, nrhs,
+                                                                                 c->esc);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(esc, c->esc, (size_t)tot * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < nrhs; ++r) {
+        double s1 = 0, s2 = 0;
+        for (int b = 0; b < cb; ++b) {
+            s1 += part[((size_t)r * cb + b) * 2];
+            s2 += part[((size_t)r * cb + b) * 2 + 1];
+        }
+        const float r0 = cabs_prefactor(einc + 6 * r);
+        if (cext) cext[r] = r0 * (float)s1;
+        if (cabs) cabs[r] = r0 * (float)s2;
+    }
+    return DDAFFT_OK;
+}
+
+}  // namespace
+
+int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, const float* alpha,
+                 const float* kinc, const float* einc, int ndirs, const float* dirs, float* esc, float* cabs,
+                 float* cext, double* err_hist, int* info) {
+    if (!c || !alpha || !kinc || !einc) return fail(DDAFFT_EINVAL, "NULL argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDAFFT_EINVAL, "nrhs outside [1, nrhs_max]");
+    if (solver < 0 || solver > 3 || max_step < 0) return fail(DDAFFT_EINVAL, "bad solver or max_step");
+    if (ndirs < 0 || (ndirs > 0 && !esc)) return fail(DDAFFT_EINVAL, "bad direction arguments");
+    const float* ddirs = nullptr;
+    if (ndirs > 0) {
+        if (dirs) {
+            TRY(ddafft_set_directions(c, ndirs, dirs));
+        } else if (ndirs != c->ndirs) {
+            return fail(DDAFFT_EINVAL, "dirs == NULL but ndirs differs from the resident directions");
+        }
+        ddirs = c->dirs;
+    }
+    Dims d = c->d;
+    d.nrhs = nrhs;
+    // inputs
+    TRY(upload(c, c->alpha, alpha, (size_t)nrhs * c->nmat * sizeof(float2)));
+    TRY(upload(c, c->einc, einc, (size_t)nrhs * 3 * sizeof(float2)));
+    if (max_step + 1 > c->hist_cap || c->hist == nullptr) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->hist) hipFree(c->hist);
+        c->hist = nullptr;
+        TRY(dev_alloc(&c->hist, (size_t)c->nrhs_max * (max_step + 1)));
+        c->hist_cap = max_step + 1;
+    }
+    std::vector<SolveState> s0(nrhs);
+    for (auto& s : s0) {
+        memset(&s, 0, sizeof(s));
+        s.stop_at = max_step + 1;
+    }
+    TRY(upload(c, c->st, s0.data(), nrhs * sizeof(SolveState)));
+    TRY(ensure_partials(c, (size_t)nrhs * std::max(zi_blocks(d), update_blocks(d)) * kSlots));
+
+    const Geometry g = make_geometry(c, kinc);
+    {
+        dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), nrhs);
+        hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->occ, c->mv, c->alpha, c->nmat, c->einc, c->P);
+        HIPCHECK(hipGetLastError());
+    }
+    Seq q{c, nrhs, FinArgs{}};
+    q.fa.hist = c->hist;
+    q.fa.max_step = max_step;
+    q.fa.err_max = (double)err_max;
+    q.fa.inv_ndip = 1.0 / (double)c->ndip;
+    TRY(step0(q, solver));
+    hipLaunchKernelGGL(k_stop_max, 1, 1, 0, c->stream, c->st, nrhs, c->stop_dev);
+    HIPCHECK(hipMemcpyAsync(&c->stop_host[0], c->stop_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+
+    // Iterations: the device decides when each right-hand side stops; the
+    // host keeps at most two iterations in flight and stops enqueuing once
+    // it has seen every right-hand side stop.
+    int known_stop = max_step + 1;
+    for (int o = 1; o <= max_step; ++o) {
+        if (o >= known_stop) break;
+        TRY(iteration(q, solver, o, tau));
+        hipLaunchKernelGGL(k_stop_max, 1, 1, 0, c->stream, c->st, nrhs, c->stop_dev);
+        HIPCHECK(hipMemcpyAsync(&c->stop_host[o & 3], c->stop_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipEventRecord(c->ev[o & 3], c->stream));
+        // wait for the flag of step o-1 (keeps one iteration queued ahead)
+        HIPCHECK(hipEventSynchronize(c->ev[(o - 1) & 3]));
+        known_stop = std::min(known_stop, c->stop_host[(o - 1) & 3]);
+    }
+    {
+        dim3 grid((int)std::min<long long>((3 * d.n + kBlock - 1) / kBlock, 2048), nrhs);
+        hipLaunchKernelGGL(k_restore, grid, kBlock, 0, c->stream, d, c->st, c->P, c->Pmin);
+        HIPCHECK(hipGetLastError());
+    }
+    TRY(observables(c, nrhs, kinc, einc, ndirs, ddirs, esc, cabs, cext));
+    std::vector<SolveState> sf(nrhs);
+    HIPCHECK(hipMemcpy(sf.data(), c->st, nrhs * sizeof(SolveState), hipMemcpyDeviceToHost));
+    if (err_hist) {
+        std::vector<double> h((size_t)nrhs * (max_step + 1));
+        HIPCHECK(hipMemcpy(h.data(), c->hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int r = 0; r < nrhs; ++r)
+            for (int o = 0; o <= max_step; ++o)
+                err_hist[(size_t)r * (max_step + 1) + o] = o <= sf[r].last ? h[(size_t)r * (max_step + 1) + o] : NAN;
+    }
+    if (info) {
+        for (int r = 0; r < nrhs; ++r) {
+            const int L = sf[r].last;
+            int mvs = 0;
+            switch (solver) {
+                case DDAFFT_SOLVER_CG: mvs = 1 + L; break;
+                case DDAFFT_SOLVER_AA: mvs = 2 + 2 * L; break;
+                case DDAFFT_SOLVER_BICGSTAB: mvs = 3 + 2 * L; break;
+                default: mvs = 1 + L; break;
+            }
+            info[4 * r + 0] = L;
+            info[4 * r + 1] = sf[r].step_min;
+            info[4 * r + 2] = sf[r].restored;
+            info[4 * r + 3] = mvs;
+        }
+    }
+    return DDAFFT_OK;
+}
+
+int ddafft_get_dipoles(ddafft_ctx* c, int rhs, float* p) {
+    if (!c || !p || rhs < 0 || rhs >= c->nrhs_max) return fail(DDAFFT_EINVAL, "bad argument");
+    return field_out(c, c->P + (long long)rhs * 3 * c->d.n, p, c->Y);
+}
+
+int ddafft_cabs_cext(ddafft_ctx* c, const float* alpha, const float* kinc, const float* einc, const float* p,
+                     float* cabs, float* cext) {
+    if (!c || !alpha || !kinc || !einc || !p) return fail(DDAFFT_EINVAL, "NULL argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    TRY(upload(c, c->alpha, alpha, (size_t)c->nmat * sizeof(float2)));
+    TRY(upload(c, c->einc, einc, 3 * sizeof(float2)));
+    TRY(field_in(c, p, c->P, c->Y));
+    return observables(c, 1, kinc, einc, 0, nullptr, nullptr, cabs, cext);
+}
+
+int ddafft_far_field(ddafft_ctx* c, const float* p, int ndirs, const float* dirs, float* esc) {
+    if (!c || !p || !dirs || !esc || ndirs < 1) return fail(DDAFFT_EINVAL, "bad argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    TRY(ddafft_set_directions(c, ndirs, dirs));
+    TRY(field_in(c, p, c->P, c->Y));
+    // cross sections are computed alongside; their values are discarded
+    std::vector<float> zero_alpha((size_t)2 * c->nmat, 0.f);
+    TRY(upload(c, c->alpha, zero_alpha.data(), zero_alpha.size() * sizeof(float)));
+    const float kz[3] = {0.f, 0.f, 1.f};
+    const float e[6] = {1.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    TRY(upload(c, c->einc, e, sizeof(e)));
+    return observables(c, 1, kz, e, ndirs, c->dirs, esc, nullptr, nullptr);
+}
+
+int ddafft_time_matvec(ddafft_ctx* c, int nrhs, int reps, float* kernel_ms, float* total_ms) {
+    if (!c || nrhs < 1 || nrhs > c->nrhs_max || reps < 1) return fail(DDAFFT_EINVAL, "bad argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    hipEvent_t ev[6];
+    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+    ProArgs pa{PRO_COPY, c->S, nullptr, nullptr, nullptr, nullptr};
+    EpiArgs ea{};
+    ea.kind = EK_MV;
+    ea.x = c->S;
+    ea.out = c->W;
+    ea.alpha = c->alpha;
+    ea.nmat = c->nmat;
+    double acc[5] = {0, 0, 0, 0, 0};
+    double tot = 0;
+    // warm-up
+    TRY(launch_matvec(c, nrhs, pa, nullptr, 0, ea, 0));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < reps; ++r) {
+        TRY(launch_matvec(c, nrhs, pa, nullptr, 0, ea, 0, ev));
+        HIPCHECK(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            acc[k] += ms;
+        }
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, ev[0], ev[5]));
+        tot += ms;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    if (kernel_ms)
+        for (int k = 0; k < 5; ++k) kernel_ms[k] = (float)(acc[k] / reps);
+    if (total_ms) *total_ms = (float)(tot / reps);
+    return DDAFFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,249 @@
+// Device building blocks shared by every kernel of the DDA-FFT solve path.
+//
+// Floating-point contract: every float expression below is written in the
+// same association order as the reference's x86-64 build (SSE, no FMA), and
+// the library is compiled with -ffp-contract=off, so each butterfly, twiddle
+// product and 3x3 product rounds exactly as the reference's does.  The FFT
+// stages reproduce the reference's mixed radix-4/8 decimation-in-time
+// algorithm (fft_kernel.hpp:35-322): the same radix plan (choose_radices,
+// fft_kernel.hpp:193-208), the same digit-reversed input order
+// (fft_kernel.hpp:222-236), the same exact float twiddles W_N^q
+// (fft_kernel.hpp:238-243, built on the host) and the same butterflies
+// (bf2c/bf4c/bf8c, fft_kernel.hpp:47-106).  A line therefore transforms to the
+// same bits on the GPU as in the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace ddafft {
+
+constexpr int kBlock = 256;   // threads per block for every pass kernel
+constexpr int kLpad = 1;      // LDS line padding (float2) against bank conflicts
+
+// ---------------------------------------------------------------------------
+// Complex helpers (complex.cpp:61-64, 11-14, 32-35, 67-71)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cneg(float2 a) { return make_float2(-a.x, -a.y); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 rmul(float r, float2 a) { return make_float2(r * a.x, r * a.y); }
+
+// ---------------------------------------------------------------------------
+// Radix plan: log2 N = 3a + b with the radix-8 stages first
+// ---------------------------------------------------------------------------
+
+__host__ __device__ constexpr int ilog2c(int n) {
+    int k = 0;
+    while ((1 << k) < n) ++k;
+    return k;
+}
+
+__host__ __device__ constexpr int plan_nst(int k) {
+    int m = 0;
+    while (k > 6) { ++m; k -= 3; }
+    return m + (k == 0 ? 0 : (k == 4 || k == 5 || k == 6) ? 2 : 1);
+}
+
+__host__ __device__ constexpr int plan_radix(int k, int t) {
+    int m = 0;
+    while (k > 6) {
+        if (m == t) return 8;
+        ++m;
+        k -= 3;
+    }
+    int r0 = 0, r1 = 0;
+    switch (k) {
+        case 1: r0 = 2; break;
+        case 2: r0 = 4; break;
+        case 3: r0 = 8; break;
+        case 4: r0 = 4; r1 = 4; break;
+        case 5: r0 = 8; r1 = 4; break;
+        case 6: r0 = 8; r1 = 8; break;
+        default: break;
+    }
+    return (t == m) ? r0 : r1;
+}
+
+__host__ __device__ constexpr int pow2floor(int v) {
+    int p = 1;
+    while (p * 2 <= v) p *= 2;
+    return p;
+}
+__host__ __device__ constexpr int clamp16(int v) { return v < 1 ? 1 : (v > 16 ? 16 : v); }
+
+template <int N>
+struct Plan {
+    static constexpr int K = ilog2c(N);
+    static constexpr int NST = plan_nst(K);
+    static_assert((1 << K) == N, "power-of-two line length");
+};
+
+// Slot k of the digit-reversed input order that receives input element j
+// (perm[k] == j, with perm as in fft_kernel.hpp:222-236).
+template <int N>
+__device__ __forceinline__ int perm_slot(int j) {
+    constexpr int K = Plan<N>::K;
+    int k = 0, mult = 1, rem = j, L = 1;
+#pragma unroll
+    for (int t = 0; t < Plan<N>::NST; ++t) {
+        const int r = plan_radix(K, t);
+        L *= r;
+        const int w = N / L;
+        const int d = rem / w;
+        rem -= d * w;
+        k += d * mult;
+        mult *= r;
+    }
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// Butterflies (fft_kernel.hpp:47-106)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void bf2(float2& a, float2& b) {
+    float tr = a.x + b.x, ti = a.y + b.y;
+    b.x = a.x - b.x;
+    b.y = a.y - b.y;
+    a.x = tr;
+    a.y = ti;
+}
+
+__device__ __forceinline__ void bf4(float2& x0, float2& x1, float2& x2, float2& x3) {
+    float t0r = x0.x + x2.x, t0i = x0.y + x2.y;
+    float t1r = x0.x - x2.x, t1i = x0.y - x2.y;
+    float t2r = x1.x + x3.x, t2i = x1.y + x3.y;
+    float t3r = x1.x - x3.x, t3i = x1.y - x3.y;
+    x0.x = t0r + t2r; x0.y = t0i + t2i;
+    x2.x = t0r - t2r; x2.y = t0i - t2i;
+    x1.x = t1r + t3i; x1.y = t1i - t3r;
+    x3.x = t1r - t3i; x3.y = t1i + t3r;
+}
+
+__device__ __forceinline__ void bf8(float2* z) {
+    const float s = 0.70710678118654752440f;
+    float2 a0, a1, a2, a3, b0, b1, b2, b3;
+    a0 = make_float2(z[0].x + z[4].x, z[0].y + z[4].y);
+    a1 = make_float2(z[1].x + z[5].x, z[1].y + z[5].y);
+    a2 = make_float2(z[2].x + z[6].x, z[2].y + z[6].y);
+    a3 = make_float2(z[3].x + z[7].x, z[3].y + z[7].y);
+    b0 = make_float2(z[0].x - z[4].x, z[0].y - z[4].y);
+    {
+        float dr = z[1].x - z[5].x, di = z[1].y - z[5].y;
+        b1 = make_float2((dr + di) * s, (di - dr) * s);
+    }
+    {
+        float dr = z[2].x - z[6].x, di = z[2].y - z[6].y;
+        b2 = make_float2(di, -dr);
+    }
+    {
+        float dr = z[3].x - z[7].x, di = z[3].y - z[7].y;
+        b3 = make_float2((di - dr) * s, -(dr + di) * s);
+    }
+    bf4(a0, a1, a2, a3);
+    bf4(b0, b1, b2, b3);
+    z[0] = a0; z[1] = b0; z[2] = a1; z[3] = b1;
+    z[4] = a2; z[5] = b2; z[6] = a3; z[7] = b3;
+}
+
+template <int R>
+__device__ __forceinline__ void butterfly(float2* z);
+template <>
+__device__ __forceinline__ void butterfly<2>(float2* z) { bf2(z[0], z[1]); }
+template <>
+__device__ __forceinline__ void butterfly<4>(float2* z) { bf4(z[0], z[1], z[2], z[3]); }
+template <>
+__device__ __forceinline__ void butterfly<8>(float2* z) { bf8(z); }
+
+// ---------------------------------------------------------------------------
+// In-place line FFTs in LDS.  `nlines` lines of N points at stride `ls`,
+// input already in digit-reversed order; output in natural order.  Stage t
+// combines r_t sub-transforms of length h into transforms of length span
+// (fft_kernel.hpp:270-285, stage kernels :119-185).
+// ---------------------------------------------------------------------------
+
+template <int N, int T>
+struct StageRunner {
+    static __device__ __forceinline__ void run(float2* s, int nlines, int ls, const float2* __restrict__ tw,
+                                               int span_prev) {
+        if constexpr (T < Plan<N>::NST) {
+            constexpr int K = Plan<N>::K;
+            constexpr int R = plan_radix(K, T);
+            const int span = span_prev * R;
+            const int H = span_prev;
+            const int step = N / span;
+            constexpr int NB = N / R;
+            const int total = nlines * NB;
+            for (int f = threadIdx.x; f < total; f += blockDim.x) {
+                const int line = f / NB;
+                const int b = f - line * NB;
+                const int g = b / H;
+                const int i = b - g * H;
+                float2* v = s + line * ls + g * span + i;
+                float2 z[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) z[q] = v[q * H];
+                if (R > 2 && i > 0) {
+#pragma unroll
+                    for (int q = 1; q < R; ++q) {
+                        const float2 w = tw[q * i * step];
+                        const float tr = z[q].x * w.x - z[q].y * w.y;
+                        z[q].y = z[q].x * w.y + z[q].y * w.x;
+                        z[q].x = tr;
+                    }
+                }
+                butterfly<R>(z);
+#pragma unroll
+                for (int q = 0; q < R; ++q) v[q * H] = z[q];
+            }
+            __syncthreads();
+            StageRunner<N, T + 1>::run(s, nlines, ls, tw, span);
+        }
+    }
+};
+
+template <int N>
+__device__ __forceinline__ void fft_lds(float2* s, int nlines, int ls, const float2* __restrict__ tw) {
+    StageRunner<N, 0>::run(s, nlines, ls, tw, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions in FP64 (the only FP64 accumulation on the path)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sums NV values over the block; the result is valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch /* >= NV*16 doubles */) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) scratch[q * 16 + wid] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double t = 0;
+            for (int w = 0; w < nw; ++w) t += scratch[q * 16 + w];
+            v[q] = t;
+        }
+    }
+}
+
+}  // namespace ddafft

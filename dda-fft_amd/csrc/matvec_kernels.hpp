@@ -1,0 +1,469 @@
+// The five passes of one DDA matvec  y = x - alpha * IFFT(Â ⊙ FFT(pad x))
+// (dda_func.cpp:150-187 Convolution, :227-437 Convolution_conj, epilogues
+// e.g. :1335-1356), for `nrhs` right-hand sides sharing one Â.
+//
+// HBM layouts (float2 = complex64, C = 3*nrhs component planes):
+//   field x, out : [C][ni][nj][nl]            (one plane per component)
+//   B1 / B4      : [C][ni][2nl][nj] / [C][ni][nj][2nl]
+//   B2           : [C][2nj][2nl][ni]
+//   B3           : [C][ni][2nl][2nj]
+//   Â            : [2nj][2nl][2ni][6]         (e00 e01 e02 e11 e12 e22)
+// Every pass reads whole contiguous lines and writes its result transposed so
+// that the next pass again reads contiguous lines; the transposed writes go
+// out in runs of a tile of adjacent lines.
+//
+//   ZF  z forward, pruned (fft_func.cpp:442-463)      x  -> B1
+//   YF  y forward, pruned (fft_func.cpp:484-504)      B1 -> B2
+//   X   x forward (:525-545) · Â (dda_func.cpp:157-182) · x inverse
+//       (fft_func.cpp:578-589), keeping i < ni         B2 -> B3
+//   YI  y inverse (fft_func.cpp:605-616), j < nj       B3 -> B4
+//   ZI  z inverse (fft_func.cpp:632-643), l < nl, fused with the solver
+//       epilogue and its FP64 partial sums             B4 -> out
+#pragma once
+
+#include "device_types.hpp"
+#include "fft_common.hpp"
+
+namespace ddafft {
+
+// Tile widths (lines per block) chosen so the LDS footprint stays <= 48 KB
+// (64 KB for the fused X pass, which holds 3 components).
+template <int N> struct TileZF { static constexpr int T = clamp16(pow2floor(6144 / (2 * (N + kLpad)))); };
+template <int N> struct TileX  { static constexpr int T = clamp16(pow2floor(8192 / (6 * (N + kLpad)))); };
+template <int N> struct TileYI { static constexpr int T = clamp16(pow2floor(6144 / (N + kLpad))); };
+template <int N> struct TileZI { static constexpr int T = clamp16(pow2floor(6144 / (3 * (N + kLpad)))); };
+
+// ---------------------------------------------------------------------------
+// Prologue of the z-forward pass: builds the matvec input from solver vectors
+// (fusing the LinearComb that precedes every CopyingBack in DDA_FFT).
+// ---------------------------------------------------------------------------
+
+enum ProMode : int {
+    PRO_COPY = 0,  // v = A
+    PRO_LC1 = 1,   // dipoles: A += c1*B (dda_func.cpp:884-905), v = A
+    PRO_LC4 = 2,   // dipoles: B = c2*D + (A + c1*B), else 0 (:843-882), v = B
+    PRO_LC2 = 3,   // dipoles: B = A + c1*B (:906-924), v = B
+};
+
+struct ProArgs {
+    int mode;
+    float2* A;
+    float2* B;
+    const float2* D;
+    const float2* P;    // snapshot source (P)
+    float2* Pmin;       // snapshot destination (P_min, dda_func.cpp:1305-1313)
+};
+
+__device__ __forceinline__ float2 prologue(const ProArgs& pa, float2 c1, float2 c2, long long ck, bool dip) {
+    switch (pa.mode) {
+        case PRO_LC1: {
+            float2 a = pa.A[ck];
+            if (dip) {
+                a = cadd(a, cmul(c1, pa.B[ck]));
+                pa.A[ck] = a;
+            }
+            return a;
+        }
+        case PRO_LC4: {
+            float2 v = make_float2(0.f, 0.f);
+            if (dip) v = cadd(cmul(c2, pa.D[ck]), cadd(pa.A[ck], cmul(c1, pa.B[ck])));
+            pa.B[ck] = v;
+            return v;
+        }
+        case PRO_LC2: {
+            float2 v = pa.B[ck];
+            if (dip) {
+                v = cadd(pa.A[ck], cmul(c1, v));
+                pa.B[ck] = v;
+            }
+            return v;
+        }
+        default:
+            return pa.A[ck];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ZF: lines (c, i, j) along l; n_l inputs -> 2 n_l outputs.
+// ---------------------------------------------------------------------------
+
+template <int NL>
+__global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveState* __restrict__ st, int iter,
+                                               const int8_t* __restrict__ occ, const int* __restrict__ mv,
+                                               float2* __restrict__ b1, const float2* __restrict__ wpre,
+                                               const float2* __restrict__ tw) {
+    constexpr int TJ = TileZF<NL>::T;
+    constexpr int LS = NL + kLpad;
+    __shared__ float2 s[2 * TJ * LS];
+    const int c = blockIdx.y, rhs = c / 3;
+    float2 c1 = make_float2(0.f, 0.f), c2 = c1;
+    int snap = 0;
+    if (st) {
+        if (iter >= st[rhs].stop_at) return;
+        c1 = st[rhs].pro_c1;
+        c2 = st[rhs].pro_c2;
+        // only the first matvec of an iteration carries the P -> P_min copy
+        snap = (pa.Pmin != nullptr && pa.P != nullptr) ? st[rhs].snapshot : 0;
+    }
+    const int tj = min(TJ, d.nj), ntile = d.nj / tj;
+    const int i = blockIdx.x / ntile, j0 = (blockIdx.x - i * ntile) * tj;
+    const long long cell0 = ((long long)i * d.nj + j0) * NL;
+    const long long cb = (long long)c * d.n;
+    for (int f = threadIdx.x; f < tj * NL; f += blockDim.x) {
+        const long long k = cell0 + f;
+        const bool dip = mv[occ[k]] != 0;
+        const float2 v = prologue(pa, c1, c2, cb + k, dip);
+        if (snap) pa.Pmin[cb + k] = pa.P[cb + k];
+        const int t = f / NL, l = f - t * NL;
+        const int slot = perm_slot<NL>(l);
+        s[t * LS + slot] = v;
+        s[(tj + t) * LS + slot] = cmul(wpre[l], v);
+    }
+    __syncthreads();
+    fft_lds<NL>(s, 2 * tj, LS, tw);
+    float2* dst = b1 + ((long long)c * d.ni + i) * (2 * NL) * d.nj + j0;
+    for (int f = threadIdx.x; f < 2 * NL * tj; f += blockDim.x) {
+        const int L = f / tj, t = f - L * tj;
+        const float2 v = (L & 1) ? s[(tj + t) * LS + (L >> 1)] : s[t * LS + (L >> 1)];
+        dst[(long long)L * d.nj + t] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// YF: lines (c, i, L) along j; n_j -> 2 n_j.
+// ---------------------------------------------------------------------------
+
+template <int NJ>
+__global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restrict__ st, int iter,
+                                               const float2* __restrict__ b1, float2* __restrict__ b2,
+                                               const float2* __restrict__ wpre, const float2* __restrict__ tw) {
+    constexpr int TI = TileZF<NJ>::T;
+    constexpr int LS = NJ + kLpad;
+    __shared__ float2 s[2 * TI * LS];
+    const int c = blockIdx.y, rhs = c / 3;
+    if (st && iter >= st[rhs].stop_at) return;
+    const int NL2 = 2 * d.nl;
+    const int ti = min(TI, d.ni), nit = d.ni / ti;
+    const int L = blockIdx.x / nit, i0 = (blockIdx.x - L * nit) * ti;
+    for (int f = threadIdx.x; f < ti * NJ; f += blockDim.x) {
+        const int t = f / NJ, jj = f - t * NJ;
+        const float2 v = b1[(((long long)c * d.ni + i0 + t) * NL2 + L) * NJ + jj];
+        const int slot = perm_slot<NJ>(jj);
+        s[t * LS + slot] = v;
+        s[(ti + t) * LS + slot] = cmul(wpre[jj], v);
+    }
+    __syncthreads();
+    fft_lds<NJ>(s, 2 * ti, LS, tw);
+    for (int f = threadIdx.x; f < 2 * NJ * ti; f += blockDim.x) {
+        const int J = f / ti, t = f - J * ti;
+        const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
+        b2[(((long long)c * 2 * NJ + J) * NL2 + L) * d.ni + i0 + t] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// X: lines (J, L) along i for the 3 components of one right-hand side.
+// ---------------------------------------------------------------------------
+
+// y = Â x for one padded cell (dda_func.cpp:163-181: c3 = c0 + c1, out = c2 + c3)
+__device__ __forceinline__ void ahat_apply(const float2* a, float2 x, float2 y, float2 z, float2& ox, float2& oy,
+                                           float2& oz) {
+    ox = cadd(cmul(a[2], z), cadd(cmul(a[0], x), cmul(a[1], y)));
+    oy = cadd(cmul(a[4], z), cadd(cmul(a[1], x), cmul(a[3], y)));
+    oz = cadd(cmul(a[5], z), cadd(cmul(a[2], x), cmul(a[4], y)));
+}
+
+template <int NI>
+__global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st, int iter,
+                                              const float2* __restrict__ b2, float2* __restrict__ b3,
+                                              const float2* __restrict__ ahat, int conj,
+                                              const float2* __restrict__ wpre, const float2* __restrict__ tw1,
+                                              const float2* __restrict__ tw2) {
+    constexpr int TJ = TileX<NI>::T;
+    constexpr int NI2 = 2 * NI;
+    constexpr int LS1 = NI + kLpad, LS2 = 2 * LS1;
+    constexpr int ITEMS = (TJ * NI2 + kBlock - 1) / kBlock;
+    __shared__ float2 s[3 * TJ * LS2];
+    const int rhs = blockIdx.y;
+    if (st && iter >= st[rhs].stop_at) return;
+    const int NJ2 = 2 * d.nj, NL2 = 2 * d.nl;
+    const int tj = min(TJ, NJ2), njt = NJ2 / tj;
+    const int L = blockIdx.x / njt, J0 = (blockIdx.x - L * njt) * tj;
+
+    for (int f = threadIdx.x; f < 3 * tj * NI; f += blockDim.x) {
+        const int ii = f % NI, r = f / NI, t = r % tj, comp = r / tj;
+        const int c = rhs * 3 + comp;
+        const float2 v = b2[(((long long)c * NJ2 + J0 + t) * NL2 + L) * NI + ii];
+        const int line = (comp * tj + t) * 2;
+        const int slot = perm_slot<NI>(ii);
+        s[line * LS1 + slot] = v;
+        s[(line + 1) * LS1 + slot] = cmul(wpre[ii], v);
+    }
+    __syncthreads();
+    fft_lds<NI>(s, 6 * tj, LS1, tw1);
+
+    // Â multiply in natural order of I; results land digit-reversed and
+    // conjugated for the inverse transform (ifft_line, fft_kernel.hpp:301-322).
+    float2 y[ITEMS][3];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const int f = threadIdx.x + it * kBlock;
+        if (f < tj * NI2) {
+            const int I = f % NI2, t = f / NI2;
+            float2 xv[3];
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp)
+                xv[comp] = s[((comp * tj + t) * 2 + (I & 1)) * LS1 + (I >> 1)];
+            int Ja = J0 + t, La = L, Ia = I;
+            if (conj) {
+                Ja = (NJ2 - Ja) & (NJ2 - 1);
+                La = (NL2 - La) & (NL2 - 1);
+                Ia = (NI2 - Ia) & (NI2 - 1);
+            }
+            const float4* ap = reinterpret_cast<const float4*>(ahat + (((long long)Ja * NL2 + La) * NI2 + Ia) * 6);
+            const float4 q0 = ap[0], q1 = ap[1], q2 = ap[2];
+            float2 a[6] = {make_float2(q0.x, q0.y), make_float2(q0.z, q0.w), make_float2(q1.x, q1.y),
+                           make_float2(q1.z, q1.w), make_float2(q2.x, q2.y), make_float2(q2.z, q2.w)};
+            if (conj)
+#pragma unroll
+                for (int e = 0; e < 6; ++e) a[e].y = -a[e].y;
+            ahat_apply(a, xv[0], xv[1], xv[2], y[it][0], y[it][1], y[it][2]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const int f = threadIdx.x + it * kBlock;
+        if (f < tj * NI2) {
+            const int I = f % NI2, t = f / NI2;
+            const int slot = perm_slot<NI2>(I);
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp) s[(comp * tj + t) * LS2 + slot] = cconj(y[it][comp]);
+        }
+    }
+    __syncthreads();
+    fft_lds<NI2>(s, 3 * tj, LS2, tw2);
+    for (int f = threadIdx.x; f < 3 * tj * NI; f += blockDim.x) {
+        const int t = f % tj, r = f / tj, i = r % NI, comp = r / NI;
+        const int c = rhs * 3 + comp;
+        b3[(((long long)c * d.ni + i) * NL2 + L) * NJ2 + J0 + t] = cconj(s[(comp * tj + t) * LS2 + i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// YI: lines (c, i, L) along J; 2 n_j -> j < n_j.
+// ---------------------------------------------------------------------------
+
+template <int NJ2>
+__global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restrict__ st, int iter,
+                                               const float2* __restrict__ b3, float2* __restrict__ b4,
+                                               const float2* __restrict__ tw) {
+    constexpr int TL = TileYI<NJ2>::T;
+    constexpr int LS = NJ2 + kLpad;
+    __shared__ float2 s[TL * LS];
+    const int c = blockIdx.y, rhs = c / 3;
+    if (st && iter >= st[rhs].stop_at) return;
+    const int NL2 = 2 * d.nl;
+    const int tl = min(TL, NL2), nlt = NL2 / tl;
+    const int i = blockIdx.x / nlt, L0 = (blockIdx.x - i * nlt) * tl;
+    const float2* src = b3 + (((long long)c * d.ni + i) * NL2 + L0) * NJ2;
+    for (int f = threadIdx.x; f < tl * NJ2; f += blockDim.x) {
+        const int t = f / NJ2, jj = f - t * NJ2;
+        s[t * LS + perm_slot<NJ2>(jj)] = cconj(src[f]);
+    }
+    __syncthreads();
+    fft_lds<NJ2>(s, tl, LS, tw);
+    const int nj = NJ2 / 2;
+    float2* dst = b4 + (((long long)c * d.ni + i) * nj) * NL2 + L0;
+    for (int f = threadIdx.x; f < tl * nj; f += blockDim.x) {
+        const int j = f / tl, t = f - j * tl;
+        dst[(long long)j * NL2 + t] = cconj(s[t * LS + j]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ZI: lines (i, j) along L for the 3 components of one right-hand side;
+// 2 n_l -> l < n_l, then the epilogue of the calling solver step.
+// ---------------------------------------------------------------------------
+
+enum EpiKind : int {
+    EK_CONV = 0,      // out = conv                                  (Convolution)
+    EK_MV = 1,        // out = x - a*conv on dipoles, else 0
+    EK_BICG_R0 = 2,   // R = S = G = -(a*conv); dot1 = (R,R)         (:1114-1135, 1178-1183)
+    EK_BICG_H = 3,    // out = x - a*conv; dot1 = (G,out)            (:1192-1216)
+    EK_BICG_W = 4,    // out = x - a*conv; dot1 = (out,x); dot2 = (out,out)  (:1231-1258)
+    EK_CG_R0 = 5,     // R = S = -(a*conv); err(R)                   (:1114-1138, 1280)
+    EK_CG_W = 6,      // out = x - a*conv; dot1 = (x,out); dot2 = (y,x)       (:1325-1361)
+    EK_AA_R0 = 7,     // R = -(a*conv); real1 = |R|^2               (:1114-1141)
+    EK_AA_S0 = 8,     // conj: out = x - conj(a)*conv; err(x)        (:1143-1176, 1280)
+    EK_AA_W = 9,      // out = x - a*conv; real1 = |x|^2             (:1325-1374)
+    EK_AA_SUPD = 10,  // conj: W = x - conj(a)*conv; S = W + rB*S on dipoles  (:1386-1416)
+    EK_SIM_R0 = 11,   // R = -(a*conv); S = x; P = S - R; err(R)     (:1266-1278)
+    EK_SIM = 12,      // W = x - a*conv; R = W - S; P = x - tau*R; err(R)     (:1471-1521)
+};
+
+constexpr int kSlots = 6;  // partial sums: dot1 (re,im), dot2 (re,im), real1, err
+
+struct EpiArgs {
+    int kind;
+    int conj;
+    const float2* x;     // matvec input
+    float2* out;         // primary output
+    float2* out2;        // secondary output (S, or P)
+    float2* out3;        // tertiary output (G)
+    const float2* y;     // extra input (G or R or S)
+    const float2* alpha; // [nrhs][nmat]
+    int nmat;
+    float tau;
+    double* partials;    // [nrhs][gridDim.x][kSlots]
+};
+
+// ldcScalMul term (dda_func.cpp:734-750): float products, summed wider
+__device__ __forceinline__ void dot_acc(float2 a, float2 b, double& re, double& im) {
+    re += (double)(a.x * b.x - a.y * b.y);
+    im += (double)(a.x * b.y + a.y * b.x);
+}
+// ldScalMulConj / ldError term (dda_func.cpp:767-775, 796-805)
+__device__ __forceinline__ double norm_term(float2 a) { return (double)(a.x * a.x + a.y * a.y); }
+
+template <int NL2>
+__global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restrict__ st, int iter,
+                                               const float2* __restrict__ b4, EpiArgs ea,
+                                               const int8_t* __restrict__ occ, const int* __restrict__ mv,
+                                               const float2* __restrict__ tw) {
+    constexpr int TJ = TileZI<NL2>::T;
+    constexpr int LS = NL2 + kLpad;
+    __shared__ float2 s[3 * TJ * LS];
+    __shared__ double red[kSlots * 16];
+    const int rhs = blockIdx.y;
+    float rB = 0.f;
+    if (st) {
+        if (iter >= st[rhs].stop_at) return;
+        rB = st[rhs].rB;
+    }
+    const int nl = NL2 / 2;
+    const int tj = min(TJ, d.nj), njt = d.nj / tj;
+    const int i = blockIdx.x / njt, j0 = (blockIdx.x - i * njt) * tj;
+    for (int f = threadIdx.x; f < 3 * tj * NL2; f += blockDim.x) {
+        const int LL = f % NL2, r = f / NL2, t = r % tj, comp = r / tj;
+        const int c = rhs * 3 + comp;
+        const float2 v = b4[(((long long)c * d.ni + i) * d.nj + j0 + t) * NL2 + LL];
+        s[(comp * tj + t) * LS + perm_slot<NL2>(LL)] = cconj(v);
+    }
+    __syncthreads();
+    fft_lds<NL2>(s, 3 * tj, LS, tw);
+
+    double acc[kSlots] = {0, 0, 0, 0, 0, 0};
+    const float2* al = ea.alpha + (long long)rhs * ea.nmat;
+    const long long rb = (long long)rhs * 3 * d.n;
+    for (int f = threadIdx.x; f < tj * nl; f += blockDim.x) {
+        const int t = f / nl, l = f - t * nl;
+        const long long k = ((long long)i * d.nj + j0 + t) * nl + l;
+        const int m = occ[k];
+        const bool dip = mv[m] != 0;
+        float2 a = al[m];
+        if (ea.conj) a.y = -a.y;
+        float2 cv[3], o[3];
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp) cv[comp] = cconj(s[(comp * tj + t) * LS + l]);
+        double nrm = 0;
+        switch (ea.kind) {
+            case EK_CONV:
+#pragma unroll
+                for (int q = 0; q < 3; ++q) ea.out[rb + q * d.n + k] = cv[q];
+                break;
+            case EK_MV:
+            case EK_BICG_H:
+            case EK_BICG_W:
+            case EK_CG_W:
+            case EK_AA_S0:
+            case EK_AA_W:
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const long long ck = rb + q * d.n + k;
+                    const float2 x = ea.x[ck];
+                    o[q] = dip ? csub(x, cmul(a, cv[q])) : make_float2(0.f, 0.f);
+                    ea.out[ck] = o[q];
+                    if (!dip) continue;
+                    if (ea.kind == EK_BICG_H) dot_acc(ea.y[ck], o[q], acc[0], acc[1]);
+                    if (ea.kind == EK_BICG_W) {
+                        dot_acc(o[q], x, acc[0], acc[1]);
+                        dot_acc(o[q], o[q], acc[2], acc[3]);
+                    }
+                    if (ea.kind == EK_CG_W) {
+                        dot_acc(x, o[q], acc[0], acc[1]);
+                        dot_acc(ea.y[ck], x, acc[2], acc[3]);
+                    }
+                    if (ea.kind == EK_AA_S0) nrm += norm_term(x);
+                    if (ea.kind == EK_AA_W) acc[4] += norm_term(x);
+                }
+                if (ea.kind == EK_AA_S0 && dip) acc[5] += sqrt(nrm);
+                break;
+            case EK_BICG_R0:
+            case EK_CG_R0:
+            case EK_AA_R0:
+            case EK_SIM_R0:
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const long long ck = rb + q * d.n + k;
+                    o[q] = dip ? cneg(cmul(a, cv[q])) : make_float2(0.f, 0.f);
+                    ea.out[ck] = o[q];
+                    if (ea.kind == EK_BICG_R0) {
+                        ea.out2[ck] = o[q];
+                        ea.out3[ck] = o[q];
+                        if (dip) dot_acc(o[q], o[q], acc[0], acc[1]);
+                    }
+                    if (ea.kind == EK_CG_R0) ea.out2[ck] = o[q];
+                    if (ea.kind == EK_AA_R0 && dip) acc[4] += norm_term(o[q]);
+                    if (ea.kind == EK_SIM_R0) {
+                        const float2 x = ea.x[ck];
+                        ea.out2[ck] = x;                       // S = P0 (Copying)
+                        ea.out3[ck] = csub(x, o[q]);           // P = S - R (all cells)
+                    }
+                    if (dip) nrm += norm_term(o[q]);
+                }
+                if ((ea.kind == EK_CG_R0 || ea.kind == EK_SIM_R0) && dip) acc[5] += sqrt(nrm);
+                break;
+            case EK_AA_SUPD:
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const long long ck = rb + q * d.n + k;
+                    if (dip) {
+                        const float2 w = csub(ea.x[ck], cmul(a, cv[q]));
+                        ea.out2[ck] = cadd(w, rmul(rB, ea.out2[ck]));
+                    }
+                }
+                break;
+            case EK_SIM:
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const long long ck = rb + q * d.n + k;
+                    if (dip) {
+                        const float2 x = ea.x[ck];
+                        const float2 w = csub(x, cmul(a, cv[q]));
+                        const float2 rr = csub(w, ea.y[ck]);
+                        ea.out[ck] = rr;
+                        ea.out2[ck] = csub(x, rmul(ea.tau, rr));
+                        nrm += norm_term(rr);
+                    } else {
+                        ea.out[ck] = make_float2(0.f, 0.f);
+                        ea.out2[ck] = make_float2(0.f, 0.f);
+                    }
+                }
+                if (dip) acc[5] += sqrt(nrm);
+                break;
+            default:
+                break;
+        }
+    }
+    if (ea.partials) {
+        block_sum<kSlots>(acc, red);
+        if (threadIdx.x == 0) {
+            double* p = ea.partials + ((long long)rhs * gridDim.x + blockIdx.x) * kSlots;
+#pragma unroll
+            for (int q = 0; q < kSlots; ++q) p[q] = acc[q];
+        }
+    }
+}
+
+}  // namespace ddafft

@@ -1,0 +1,441 @@
+// Elementwise and scalar kernels of the DDA_FFT solvers (dda_func.cpp:1075-1570)
+// plus the observables computed from the converged dipoles: CabsCext
+// (dda_func.cpp:1679-1727) and FarField (dda_func.cpp:1576-1625).
+#pragma once
+
+#include "device_types.hpp"
+#include "fft_common.hpp"
+#include "matvec_kernels.hpp"
+
+namespace ddafft {
+
+// exp(i x) of a float phase.  The reference rounds cosf/sinf of the float
+// argument (complex.cpp:176-181); evaluating in FP64 and rounding once gives
+// the correctly rounded float, which is what glibc returns for all but a
+// handful of arguments.
+__device__ __forceinline__ float2 exp_i(float x) {
+    double s, c;
+    sincos((double)x, &s, &c);
+    return make_float2((float)c, (float)s);
+}
+
+// ---------------------------------------------------------------------------
+// Geometry passed to the observables and to the first approximation.
+// ---------------------------------------------------------------------------
+struct Geometry {
+    float d0;          // kd
+    float kx, ky, kz;  // incident direction
+    float cx, cy, cz;  // centre used by FirstApproximation / FarField: (n-1) d / 2
+    float ex, ey, ez;  // centre used by CabsCext: n d / 2 (dda_func.cpp:1685)
+};
+
+// P0 = alpha_m exp(i k.(r - c)) E on dipoles (dda_func.cpp:654-689).
+// einc: [nrhs][3] complex; alpha: [nrhs][nmat].
+__global__ __launch_bounds__(kBlock) void k_first_approx(Dims d, Geometry g, const int8_t* __restrict__ occ,
+                                                         const int* __restrict__ mv, const float2* __restrict__ alpha,
+                                                         int nmat, const float2* __restrict__ einc,
+                                                         float2* __restrict__ p) {
+    const int rhs = blockIdx.y;
+    const long long rb = (long long)rhs * 3 * d.n;
+    const float2 E0 = einc[rhs * 3 + 0], E1 = einc[rhs * 3 + 1], E2 = einc[rhs * 3 + 2];
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < d.n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int l = (int)(k % d.nl);
+        const long long ij = k / d.nl;
+        const int j = (int)(ij % d.nj), i = (int)(ij / d.nj);
+        const int m = occ[k];
+        float2 o0 = make_float2(0.f, 0.f), o1 = o0, o2 = o0;
+        if (mv[m] != 0) {
+            const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
+            const float dx = rx - g.cx, dy = ry - g.cy, dz = rz - g.cz;
+            const float ph = g.kx * dx + g.ky * dy + g.kz * dz;
+            const float2 c0 = exp_i(ph);
+            const float2 c1 = cmul(c0, alpha[(long long)rhs * nmat + m]);
+            o0 = cmul(c1, E0);
+            o1 = cmul(c1, E1);
+            o2 = cmul(c1, E2);
+        }
+        p[rb + k] = o0;
+        p[rb + d.n + k] = o1;
+        p[rb + 2 * d.n + k] = o2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise solver updates with fused FP64 partial sums.
+// ---------------------------------------------------------------------------
+
+enum UpdKind : int {
+    UK_BICG = 0,  // P = -cW R + (P - cA S); R += -cW W; dot1 = (R,G); err(R)   (:1461-1466, 1523)
+    UK_CG = 1,    // P += -cA S; R += -cA W; dot1 = (R,W); err(R)              (:1364-1368, 1523)
+    UK_AA = 2,    // R += rA W; P += rA S; real1 = |R|^2; err(R)              (:1375-1380, 1523)
+};
+
+struct UpdArgs {
+    int kind;
+    float2* P;
+    float2* R;
+    const float2* S;
+    const float2* W;
+    const float2* G;
+    double* partials;  // [nrhs][gridDim.x][kSlots]
+};
+
+__global__ __launch_bounds__(kBlock) void k_update(Dims d, const SolveState* __restrict__ st, int iter, UpdArgs ua,
+                                                   const int8_t* __restrict__ occ, const int* __restrict__ mv) {
+    __shared__ double red[kSlots * 16];
+    const int rhs = blockIdx.y;
+    if (iter >= st[rhs].stop_at) return;
+    const float2 ncA = cneg(st[rhs].cA), ncW = cneg(st[rhs].cW);
+    const float rA = st[rhs].rA;
+    const long long rb = (long long)rhs * 3 * d.n;
+    double acc[kSlots] = {0, 0, 0, 0, 0, 0};
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < d.n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const bool dip = mv[occ[k]] != 0;
+        double nrm = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const long long ck = rb + q * d.n + k;
+            float2 p = ua.P[ck], r = ua.R[ck];
+            if (ua.kind == UK_BICG) {
+                if (dip) {
+                    const float2 t2 = cadd(p, cmul(ncA, ua.S[ck]));
+                    p = cadd(cmul(ncW, r), t2);
+                    r = cadd(r, cmul(ncW, ua.W[ck]));
+                } else {
+                    p = make_float2(0.f, 0.f);
+                }
+                if (dip) dot_acc(r, ua.G[ck], acc[0], acc[1]);
+            } else if (ua.kind == UK_CG) {
+                if (dip) {
+                    const float2 w = ua.W[ck];
+                    p = cadd(p, cmul(ncA, ua.S[ck]));
+                    r = cadd(r, cmul(ncA, w));
+                    dot_acc(r, w, acc[0], acc[1]);
+                }
+            } else {
+                if (dip) {
+                    r = cadd(r, rmul(rA, ua.W[ck]));
+                    p = cadd(p, rmul(rA, ua.S[ck]));
+                    acc[4] += norm_term(r);
+                }
+            }
+            ua.P[ck] = p;
+            ua.R[ck] = r;
+            if (dip) nrm += norm_term(r);
+        }
+        if (dip) acc[5] += sqrt(nrm);
+    }
+    block_sum<kSlots>(acc, red);
+    if (threadIdx.x == 0) {
+        double* p = ua.partials + ((long long)rhs * gridDim.x + blockIdx.x) * kSlots;
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) p[q] = acc[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scalar steps: sum the block partials in a fixed order and update the
+// solver state of one right-hand side (one block per right-hand side).
+// ---------------------------------------------------------------------------
+
+enum FinKind : int {
+    FK_BICG_0 = 0,   // rho_prev = rho = (R,G)                       (:1183)
+    FK_BICG_A = 1,   // A = rho/(G,H); pro_c1 = -cA                   (:1218-1222, 1421-1426)
+    FK_BICG_W = 2,   // W = (W,R)/(W,W)                               (:1256-1260, 1461-1464)
+    FK_BICG_E = 3,   // err; next rho; B; pro_c1 = cB, pro_c2 = -(cB cW)   (:1280, 1315-1323)
+    FK_CG_E0 = 4,    // err after step 0                              (:1280)
+    FK_CG_A = 5,     // den = (S,W); A = (R,S)/den                    (:1360-1362)
+    FK_CG_E = 6,     // B = (R,W)/den; pro_c1 = -cB; err              (:1367-1370, 1523)
+    FK_AA_0 = 7,     // RR_1 = |R|^2                                  (:1141)
+    FK_AA_E0 = 8,    // err after step 0
+    FK_AA_A = 9,     // rA = -RR_1/|S|^2                              (:1374-1375)
+    FK_AA_E = 10,    // RR; rB = RR/RR_1; RR_1 = RR; err              (:1382-1383, 1523)
+    FK_SIM_E = 11,   // err                                           (:1523)
+};
+
+struct FinArgs {
+    int kind;
+    const double* partials;
+    int nblk;              // partial blocks per right-hand side
+    double* hist;          // [nrhs][max_step+1]
+    int max_step;
+    double err_max;
+    double inv_ndip;       // 1 / number of dipoles (ldError's n)
+};
+
+__device__ __forceinline__ void ld_div(double ar, double ai, double br, double bi, double& cr, double& ci) {
+    const double n = br * br + bi * bi;
+    cr = (ar * br + ai * bi) / n;
+    ci = (ai * br - ar * bi) / n;
+}
+
+__device__ __forceinline__ float2 to_f(double r, double i) { return make_float2((float)r, (float)i); }
+
+// End-of-step bookkeeping shared by every solver: records the discrepancy of
+// step `o` and decides whether iteration o+1 runs (dda_func.cpp:1280-1313).
+__device__ void end_step(SolveState& s, const FinArgs& fa, int rhs, int o, double err) {
+    s.err = err;
+    s.last = o;
+    fa.hist[(long long)rhs * (fa.max_step + 1) + o] = err;
+    if (o == 0) {
+        s.err_min = err;
+        s.step_min = 0;
+        s.snapshot = 1;
+    } else {
+        s.snapshot = 0;
+    }
+    if (o + 1 > fa.max_step || err <= fa.err_max) {
+        s.stop_at = o + 1;
+    } else if (err < s.err_min) {
+        s.err_min = err;
+        s.step_min = o;
+        s.snapshot = 1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize(SolveState* __restrict__ st, int iter, FinArgs fa) {
+    __shared__ double red[kSlots * 16];
+    const int rhs = blockIdx.x;
+    if (iter >= st[rhs].stop_at) return;
+    double v[kSlots] = {0, 0, 0, 0, 0, 0};
+    const double* p = fa.partials + (long long)rhs * fa.nblk * kSlots;
+    for (int b = threadIdx.x; b < fa.nblk; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) v[q] += p[(long long)b * kSlots + q];
+    block_sum<kSlots>(v, red);
+    if (threadIdx.x != 0) return;
+    SolveState& s = st[rhs];
+    const double err = v[5] * fa.inv_ndip;
+    switch (fa.kind) {
+        case FK_BICG_0:
+            s.rho_prev = s.rho = v[0];
+            s.rho_prev_i = s.rho_i = v[1];
+            break;
+        case FK_BICG_A: {
+            ld_div(s.rho, s.rho_i, v[0], v[1], s.A, s.A_i);
+            s.cA = to_f(s.A, s.A_i);
+            s.pro_c1 = cneg(s.cA);
+            break;
+        }
+        case FK_BICG_W: {
+            ld_div(v[0], v[1], v[2], v[3], s.W, s.W_i);
+            s.cW = to_f(s.W, s.W_i);
+            break;
+        }
+        case FK_BICG_E: {
+            if (iter > 0) {
+                s.rho_prev = s.rho;
+                s.rho_prev_i = s.rho_i;
+            }
+            s.rho = v[0];
+            s.rho_i = v[1];
+            // ldcB = (rho*A)/(rho_prev*W)
+            const double nr = s.rho * s.A - s.rho_i * s.A_i, ni = s.rho * s.A_i + s.rho_i * s.A;
+            const double dr = s.rho_prev * s.W - s.rho_prev_i * s.W_i, di = s.rho_prev * s.W_i + s.rho_prev_i * s.W;
+            double br, bi;
+            ld_div(nr, ni, dr, di, br, bi);
+            s.cB = to_f(br, bi);
+            s.pro_c1 = s.cB;
+            s.pro_c2 = cneg(cmul(s.cB, s.cW));
+            end_step(s, fa, rhs, iter, err);
+            break;
+        }
+        case FK_CG_E0:
+        case FK_AA_E0:
+        case FK_SIM_E:
+            end_step(s, fa, rhs, iter, err);
+            break;
+        case FK_CG_A: {
+            s.den = v[0];
+            s.den_i = v[1];
+            ld_div(v[2], v[3], s.den, s.den_i, s.A, s.A_i);
+            s.cA = to_f(s.A, s.A_i);
+            break;
+        }
+        case FK_CG_E: {
+            double br, bi;
+            ld_div(v[0], v[1], s.den, s.den_i, br, bi);
+            s.cB = to_f(br, bi);
+            s.pro_c1 = cneg(s.cB);
+            end_step(s, fa, rhs, iter, err);
+            break;
+        }
+        case FK_AA_0:
+            s.RR_1 = v[4];
+            break;
+        case FK_AA_A:
+            s.rA = (float)(-s.RR_1 / v[4]);
+            break;
+        case FK_AA_E: {
+            const double rr = v[4];
+            s.rB = (float)(rr / s.RR_1);
+            s.RR_1 = rr;
+            end_step(s, fa, rhs, iter, err);
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+// Best-iterate fallback (dda_func.cpp:1537-1548).
+__global__ __launch_bounds__(kBlock) void k_restore(Dims d, SolveState* __restrict__ st, float2* __restrict__ P,
+                                                    const float2* __restrict__ Pmin) {
+    const int rhs = blockIdx.y;
+    const SolveState s = st[rhs];
+    if (!(s.err > s.err_min)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st[rhs].restored = 1;
+    const long long rb = (long long)rhs * 3 * d.n;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < 3 * d.n;
+         k += (long long)gridDim.x * blockDim.x)
+        P[rb + k] = Pmin[rb + k];
+}
+
+// ---------------------------------------------------------------------------
+// Cross sections (dda_func.cpp:1679-1727): per-dipole float terms, FP64 sums.
+// partials: [nrhs][gridDim.x][2]
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const int8_t* __restrict__ occ,
+                                                      const float2* __restrict__ alpha, int nmat,
+                                                      const float2* __restrict__ einc, const float2* __restrict__ P,
+                                                      double* __restrict__ partials) {
+    __shared__ double red[2 * 16];
+    const int rhs = blockIdx.y;
+    const long long rb = (long long)rhs * 3 * d.n;
+    const float2 e0 = cconj(einc[rhs * 3]), e1 = cconj(einc[rhs * 3 + 1]), e2 = cconj(einc[rhs * 3 + 2]);
+    const float r4 = (float)(2. / 3.);
+    double acc[2] = {0, 0};
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < d.n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int m = occ[k];
+        const float2 a = alpha[(long long)rhs * nmat + m];
+        if (a.x == 0.f && a.y == 0.f) continue;
+        const int l = (int)(k % d.nl);
+        const long long ij = k / d.nl;
+        const int j = (int)(ij % d.nj), i = (int)(ij / d.nj);
+        const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
+        const float r3 = -(g.kx * (rx - g.ex) + g.ky * (ry - g.ey) + g.kz * (rz - g.ez));
+        const float2 c0 = exp_i(r3);
+        const float2 px = P[rb + k], py = P[rb + d.n + k], pz = P[rb + 2 * d.n + k];
+        const float2 c5 = cadd(cmul(pz, e2), cadd(cmul(px, e0), cmul(py, e1)));
+        acc[0] += (double)cmul(c0, c5).y;
+        const float nx = px.x * px.x + px.y * px.y, ny = py.x * py.x + py.y * py.y, nz = pz.x * pz.x + pz.y * pz.y;
+        const float ix = px.y * px.x - px.x * px.y, iy = py.y * py.x - py.x * py.y, iz = pz.y * pz.x - pz.x * pz.y;
+        const float2 q = make_float2(nz + (nx + ny), iz + (ix + iy));
+        const float2 ca = make_float2(a.x, -a.y);
+        const float nn = ca.x * ca.x + ca.y * ca.y;
+        const float qi = (-q.x * ca.y + q.y * ca.x) / nn;
+        acc[1] += (double)(qi - r4 * q.x);
+    }
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+        double* p = partials + ((long long)rhs * gridDim.x + blockIdx.x) * 2;
+        p[0] = acc[0];
+        p[1] = acc[1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Far field (dda_func.cpp:1576-1625) over a compact list of dipole cells.
+// Each block takes a chunk of dipoles and kFD directions; partial sums per
+// (chunk, direction) go to `partials` [nrhs][nchunk][ndirs][6].
+// ---------------------------------------------------------------------------
+
+constexpr int kFD = 4;
+
+__global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const int* __restrict__ cells, int ncells,
+                                                      const float* __restrict__ dirs, int ndirs,
+                                                      const float2* __restrict__ P, double* __restrict__ partials) {
+    __shared__ double red[6 * kFD * 16];
+    const int rhs = blockIdx.z;
+    const int d0 = blockIdx.y * kFD;
+    const int nchunk = gridDim.x;
+    const long long rb = (long long)rhs * 3 * d.n;
+    float kx[kFD], ky[kFD], kz[kFD];
+#pragma unroll
+    for (int q = 0; q < kFD; ++q) {
+        const int dd = min(d0 + q, ndirs - 1);
+        kx[q] = dirs[3 * dd];
+        ky[q] = dirs[3 * dd + 1];
+        kz[q] = dirs[3 * dd + 2];
+    }
+    double acc[6 * kFD];
+#pragma unroll
+    for (int q = 0; q < 6 * kFD; ++q) acc[q] = 0;
+    const int per = (ncells + nchunk - 1) / nchunk;
+    const int lo = blockIdx.x * per, hi = min(ncells, lo + per);
+    for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+        const int k = cells[e];
+        const int l = k % d.nl, ij = k / d.nl;
+        const int j = ij % d.nj, i = ij / d.nj;
+        const float rx = (float)i * g.d0 - g.cx, ry = (float)j * g.d0 - g.cy, rz = (float)l * g.d0 - g.cz;
+        const float2 px = P[rb + k], py = P[rb + d.n + k], pz = P[rb + 2 * d.n + k];
+#pragma unroll
+        for (int q = 0; q < kFD; ++q) {
+            const float r0 = kx[q] * rx + ky[q] * ry + kz[q] * rz;
+            const float2 c0 = exp_i(r0);
+            const float2 t = cadd(rmul(kz[q], pz), cadd(rmul(kx[q], px), rmul(ky[q], py)));
+            const float2 c5 = csub(px, rmul(kx[q], t)), c6 = csub(py, rmul(ky[q], t)), c7 = csub(pz, rmul(kz[q], t));
+            const float2 o0 = cmul(c0, c5), o1 = cmul(c0, c6), o2 = cmul(c0, c7);
+            acc[6 * q + 0] += o0.x;
+            acc[6 * q + 1] += o0.y;
+            acc[6 * q + 2] += o1.x;
+            acc[6 * q + 3] += o1.y;
+            acc[6 * q + 4] += o2.x;
+            acc[6 * q + 5] += o2.y;
+        }
+    }
+    block_sum<6 * kFD>(acc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < kFD; ++q) {
+            if (d0 + q >= ndirs) break;
+            double* p = partials + (((long long)rhs * nchunk + blockIdx.x) * ndirs + d0 + q) * 6;
+#pragma unroll
+            for (int e = 0; e < 6; ++e) p[e] = acc[6 * q + e];
+        }
+    }
+}
+
+// Sums far-field chunks: out[nrhs][ndirs][3] complex float.
+__global__ __launch_bounds__(kBlock) void k_far_field_sum(const double* __restrict__ partials, int nchunk, int ndirs,
+                                                          int nrhs, float2* __restrict__ out) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nrhs * ndirs * 3) return;
+    const int comp = idx % 3, dd = (idx / 3) % ndirs, rhs = idx / (3 * ndirs);
+    double re = 0, im = 0;
+    for (int c = 0; c < nchunk; ++c) {
+        const double* p = partials + (((long long)rhs * nchunk + c) * ndirs + dd) * 6 + 2 * comp;
+        re += p[0];
+        im += p[1];
+    }
+    out[idx] = make_float2((float)re, (float)im);
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion between the reference's cvector AoS ([n][3] complex) and
+// the component planes used on the device ([3][n]).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_aos_to_soa(long long n, const float2* __restrict__ aos,
+                                                       float2* __restrict__ soa) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        soa[k] = aos[3 * k];
+        soa[n + k] = aos[3 * k + 1];
+        soa[2 * n + k] = aos[3 * k + 2];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_soa_to_aos(long long n, const float2* __restrict__ soa,
+                                                       float2* __restrict__ aos) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        aos[3 * k] = soa[k];
+        aos[3 * k + 1] = soa[n + k];
+        aos[3 * k + 2] = soa[2 * n + k];
+    }
+}
+
+}  // namespace ddafft

@@ -1,0 +1,115 @@
+/*
+ * ddafft.h -- C ABI of the MI355X DDA-FFT solve path (libddafft_amd.so).
+ *
+ * The reference (physicist2018/dda-fft v7.2) is one C++ program with no
+ * plugin or FFI layer; its solve path sits behind the C++ functions cited
+ * beside each entry point below.  This header is the drop-in boundary for
+ * that path: plain pointers and sizes, integer status codes, no exit().
+ *
+ * Data layouts (caller-owned host buffers, as the reference stores them):
+ *   occupancy  int8  [ni][nj][nl]         material index per cell, 0 = vacuum
+ *   field      float [ni][nj][nl][3][2]   complex 3-vector per cell (cvector)
+ *   ahat       float [2ni][2nj][2nl][6][2] (e00 e01 e02 e11 e12 e22)
+ *   alpha      float [nmat][2]            polarizability per material index,
+ *                                         alpha[0] is vacuum (0)
+ *   mv         int   [nmat]               1 if material index m is a dipole
+ *   directions float [ndirs][3]           unit scattering directions
+ *   esc        float [ndirs][3][2]        scattered far field per direction
+ *
+ * Threading: a context owns one HIP stream and its device buffers; calls on
+ * one context must not overlap.  Different contexts may be used from
+ * different host threads.
+ */
+#ifndef DDAFFT_H
+#define DDAFFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDAFFT_OK 0
+#define DDAFFT_EINVAL (-1)   /* bad argument (sizes, pointers, not a power of two) */
+#define DDAFFT_ENOMEM (-2)   /* device or host allocation failed */
+#define DDAFFT_EHIP (-3)     /* HIP runtime error */
+#define DDAFFT_ENODEV (-4)   /* no usable GPU */
+#define DDAFFT_ESTATE (-5)   /* call out of order (e.g. solve before set_target) */
+
+#define DDAFFT_SOLVER_CG 0        /* solver codes of DDA_FFT (options.cpp:80-86) */
+#define DDAFFT_SOLVER_AA 1
+#define DDAFFT_SOLVER_BICGSTAB 2
+#define DDAFFT_SOLVER_SIMPLE 3
+
+typedef struct ddafft_ctx ddafft_ctx;
+
+/* Message of the last failing call on this host thread. */
+const char* ddafft_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int ddafft_version(void);
+
+/* Creates a context for an ni x nj x nl lattice (powers of two, 4..256 per
+ * axis) with lattice spacing kd, able to solve up to nrhs_max right-hand
+ * sides at once, and builds Â on the device.
+ * Replaces main.cpp:390-476 (cmA allocation, CreateAMatrix, FourierPhase
+ * tables, FFT_3D(cmatrix)) and dda_workspace_create (dda_func.cpp:1031-1054). */
+int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** out);
+
+/* Frees every device buffer of the context (dda_workspace_free,
+ * dda_func.cpp:1056-1069, and the frees of main.cpp:1030-1077). */
+int ddafft_destroy(ddafft_ctx* ctx);
+
+/* Copies Â back in the reference's layout (cmA of main.cpp:390-409). */
+int ddafft_get_ahat(ddafft_ctx* ctx, float* ahat);
+
+/* Sets the occupancy map and the dipole flags of the materials
+ * (cAlpha / iListMV arguments of DDA_FFT, function.hpp:153-160). */
+int ddafft_set_target(ddafft_ctx* ctx, const signed char* occ, int nmat, const int* mv);
+
+/* Makes `dirs` the resident scattering directions used by ddafft_solve when
+ * it is given dirs == NULL (ksc argument of DDA_FFT). */
+int ddafft_set_directions(ddafft_ctx* ctx, int ndirs, const float* dirs);
+
+/* y = first octant of IFFT(Â ⊙ FFT(pad x)), or with conj(Â(-k)) when conj
+ * is non-zero: CopyingBack + Convolution / Convolution_conj
+ * (dda_func.cpp:702-718, 150-187, 227-437). */
+int ddafft_convolution(ddafft_ctx* ctx, const float* x, float* y, int conj);
+
+/* y = x - alpha_m * (A x) on dipole cells and 0 elsewhere (conj: with
+ * conj(alpha_m) and A^H) -- the matvec epilogue of DDA_FFT
+ * (dda_func.cpp:1325-1357, 1386-1416). */
+int ddafft_matvec(ddafft_ctx* ctx, const float* alpha, const float* x, float* y, int conj);
+
+/* One DDA_FFT solve per right-hand side (dda_func.cpp:1075-1570,
+ * function.hpp:153-160): nrhs systems share the lattice, the occupancy and
+ * kinc; each has its own alpha [nrhs][nmat][2] and incident field
+ * einc [nrhs][3][2].  Outputs per right-hand side: esc [nrhs][ndirs][3][2],
+ * cabs [nrhs], cext [nrhs], err_hist [nrhs][max_step+1] (the discrepancy
+ * printed after each step; optional) and info [nrhs][4] = {last step,
+ * best step, 1 if the best iterate was restored, matvecs} (optional).
+ * dirs == NULL uses the directions of ddafft_set_directions. */
+int ddafft_solve(ddafft_ctx* ctx, int nrhs, int solver, float tau, int max_step, float err_max,
+                 const float* alpha, const float* kinc, const float* einc, int ndirs, const float* dirs,
+                 float* esc, float* cabs, float* cext, double* err_hist, int* info);
+
+/* Dipole moments P of right-hand side `rhs` after the last solve. */
+int ddafft_get_dipoles(ddafft_ctx* ctx, int rhs, float* p);
+
+/* Cabs and Cext of a given dipole field (CabsCext, dda_func.cpp:1679-1727). */
+int ddafft_cabs_cext(ddafft_ctx* ctx, const float* alpha, const float* kinc, const float* einc,
+                     const float* p, float* cabs, float* cext);
+
+/* Far field of a given dipole field in ndirs directions
+ * (FarField, dda_func.cpp:1576-1625). */
+int ddafft_far_field(ddafft_ctx* ctx, const float* p, int ndirs, const float* dirs, float* esc);
+
+/* Times `reps` matvecs of nrhs right-hand sides on device-resident data with
+ * HIP events on the context's stream.  kernel_ms[5] receives the average
+ * duration of each pass (z-fwd, y-fwd, fused x, y-inv, z-inv) and
+ * total_ms the average wall time of one matvec.  Test and benchmark aid. */
+int ddafft_time_matvec(ddafft_ctx* ctx, int nrhs, int reps, float* kernel_ms, float* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDAFFT_H */

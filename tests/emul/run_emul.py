@@ -1,0 +1,95 @@
+"""Drives the CPU-emulated libddafft (tests/emul/libddafft_emul.so) through its
+C ABI and checks it against the oracle.  Run under AddressSanitizer, from
+tests/test_emul.py:
+
+    LD_PRELOAD=<libasan.so> python tests/emul/run_emul.py [quick]
+
+Exit status 0 when every check passes; ASan aborts the process on any
+out-of-bounds or null access made by a kernel or by the host sequencing.
+"""
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "dda-fft_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
+
+import ddafft  # noqa: E402
+import oracle  # noqa: E402
+from conftest import bits, directions  # noqa: E402
+
+
+def rnd(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def check(cond, what):
+    print(("ok   " if cond else "FAIL ") + what, flush=True)
+    return bool(cond)
+
+
+def main():
+    quick = "quick" in sys.argv[1:]
+    ddafft.load(os.path.join(HERE, "libddafft_emul.so"))
+    orc = oracle.Oracle("f32")
+    ok = True
+    rng = np.random.default_rng(3)
+    cases = [(8, 8, 8)] if quick else [(8, 8, 8), (4, 8, 16)]
+    for dims in cases:
+        kd = np.float32(0.6)
+        occ, _ = orc.target_ellipsoid(dims, min(1.0, dims[1] / dims[0]), min(1.0, dims[2] / dims[0]))
+        occ = (occ * (1 + (rng.random(dims) < 0.3))).astype(np.int8)
+        mv = np.array([0, 1, 1], np.int32)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.01j), orc.polarizability(1, kd, 1.7 + 0.1j)],
+                         np.complex64)
+        with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+            A = ctx.ahat()
+            ok &= check(np.array_equal(bits(A), bits(orc.build_ahat(dims, kd))), "ahat %s" % (dims,))
+            ctx.set_target(occ, mv)
+            x = rnd(rng, dims + (3,))
+            for conj in (False, True):
+                y = ctx.convolution(x, conj)
+                ok &= check(np.array_equal(bits(y), bits(orc.convolution(A, x, conj))), "conv %s %d" % (dims, conj))
+                y = ctx.matvec(alpha, x, conj)
+                ok &= check(np.array_equal(bits(y), bits(orc.matvec(A, occ, mv, alpha, x, conj))),
+                            "matvec %s %d" % (dims, conj))
+            dirs = np.concatenate([directions(np.arange(0, 181, 30), p) for p in (0.0, 90.0)])
+            kinc = np.array([0, 0, 1], np.float32)
+            einc = np.eye(3, dtype=np.complex64)[:2]
+            for solver in (0, 1, 2, 3):
+                res = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=4, err_max=1e-6, tau=0.5)
+                for pol in range(2):
+                    ref = orc.dda_fft(solver, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=4,
+                                      err_max=1e-6, tau=0.5, return_p=True)
+                    got = res[pol]
+                    p = ctx.dipoles(pol)
+                    rel = np.max(np.abs(p - ref["p"])) / np.max(np.abs(ref["p"]))
+                    erel = np.max(np.abs(got["esc"] - ref["esc"])) / np.max(np.abs(ref["esc"]))
+                    good = (got["steps"] == ref["steps"] and got["step_min"] == ref["step_min"]
+                            and got["restored"] == ref["restored"]
+                            and np.allclose(got["err_hist"], ref["err_hist"], rtol=1e-9) and rel < 1e-5
+                            and erel < 1e-5 and abs(got["cext"] / ref["cext"] - 1) < 1e-5
+                            and abs(got["cabs"] / ref["cabs"] - 1) < 1e-4)
+                    ok &= check(good, "solve %s solver %d pol %d steps %d/%d P %.2e esc %.2e" % (
+                        dims, solver, pol, got["steps"], ref["steps"], rel, erel))
+            # resident directions, one right-hand side
+            ctx.set_directions(dirs)
+            res1 = ctx.solve(2, alpha, kinc, einc[:1], max_step=3, err_max=1e-6)
+            ok &= check(len(res1) == 1 and res1[0]["esc"].shape == (dirs.shape[0], 3), "solve resident dirs")
+            p = ctx.dipoles(0)
+            esc = ctx.far_field(p, dirs)
+            ok &= check(np.max(np.abs(esc - res1[0]["esc"])) <= 1e-6 * np.max(np.abs(esc)), "far_field")
+            ca, ce = ctx.cabs_cext(alpha, kinc, einc[0], p)
+            ok &= check(abs(ce / res1[0]["cext"] - 1) < 1e-6 and abs(ca / res1[0]["cabs"] - 1) < 1e-6,
+                        "cabs_cext")
+            k, t = ctx.time_matvec(2, 1)
+            ok &= check(t >= 0, "time_matvec")
+    print("ALL OK" if ok else "SOME CHECKS FAILED", flush=True)
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,173 @@
+"""Parity of the HIP path (libddafft_amd.so, through its C ABI) with the oracle
+and with the reference's golden fixtures.
+
+Bit-exact: Â, the pruned transforms inside the convolution, the matvec
+epilogue (same float operation order as the reference, no FMA contraction).
+Solves: same iteration count and printed discrepancies, dipoles within float
+rounding, observables within 1e-5 relative of the oracle (the reference sums
+Cext/Cabs/far field sequentially in float; the GPU sums them in FP64).
+"""
+
+import numpy as np
+import pytest
+
+from conftest import bits, directions, golden
+
+pytestmark = pytest.mark.gpu
+
+ddafft = pytest.importorskip("ddafft")
+
+DIMS = [(4, 4, 4), (8, 4, 16), (16, 16, 16), (16, 32, 8), (32, 32, 32)]
+
+
+def rnd(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def target(orc32, dims, b_a=1.0, c_a=1.0):
+    c_a = min(c_a, dims[2] / dims[0])
+    b_a = min(b_a, dims[1] / dims[0])
+    occ, cnt = orc32.target_ellipsoid(dims, b_a, c_a)
+    return occ
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_ahat_bitexact(orc32, dims):
+    kd = np.float32(0.3125 if dims[0] >= 16 else 0.7)
+    print("ahat", dims, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=1) as ctx:
+        A = ctx.ahat()
+    R = orc32.build_ahat(dims, kd)
+    assert np.array_equal(bits(A), bits(R))
+
+
+def test_ahat_golden():
+    g = golden("lattice.npz")
+    for tag in ("a", "b"):
+        dims = tuple(int(v) for v in g["dims_" + tag])
+        with ddafft.Context(dims, g["kd_" + tag], nrhs_max=1) as ctx:
+            A = ctx.ahat()
+        assert np.array_equal(bits(A), bits(g["ahat_" + tag])), tag
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("conj", [False, True])
+def test_convolution_bitexact(orc32, dims, conj):
+    rng = np.random.default_rng(hash(dims) % 2**32)
+    kd = np.float32(0.5)
+    x = rnd(rng, dims + (3,))
+    print("convolution", dims, conj, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=1) as ctx:
+        ctx.set_target(np.ones(dims, np.int8), [0, 1])
+        y = ctx.convolution(x, conj=conj)
+        A = ctx.ahat()
+    ref = orc32.convolution(A, x, conj=conj)
+    assert np.array_equal(bits(y), bits(ref))
+
+
+def test_convolution_golden():
+    g = golden("lattice.npz")
+    for tag in ("a", "b"):
+        dims = tuple(int(v) for v in g["dims_" + tag])
+        with ddafft.Context(dims, g["kd_" + tag], nrhs_max=1) as ctx:
+            ctx.set_target(np.ones(dims, np.int8), [0, 1])
+            y = ctx.convolution(g["x_" + tag])
+            yc = ctx.convolution(g["x_" + tag], conj=True)
+        assert np.array_equal(bits(y), bits(g["conv_" + tag])), tag
+        assert np.array_equal(bits(yc), bits(g["convc_" + tag])), tag
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8), (16, 16, 32), (32, 32, 32)])
+@pytest.mark.parametrize("conj", [False, True])
+def test_matvec_bitexact(orc32, dims, conj):
+    rng = np.random.default_rng(5)
+    kd = np.float32(0.4)
+    occ = target(orc32, dims)
+    occ = occ * (1 + (rng.random(dims) < 0.3)).astype(np.int8)    # two materials
+    mv = np.array([0, 1, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, 1.5 + 0.01j), orc32.polarizability(1, kd, 1.7 + 0.1j)],
+                     np.complex64)
+    x = rnd(rng, dims + (3,))
+    print("matvec", dims, conj, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=1) as ctx:
+        ctx.set_target(occ, mv)
+        y = ctx.matvec(alpha, x, conj=conj)
+        A = ctx.ahat()
+    ref = orc32.matvec(A, occ, mv, alpha, x, conj=conj)
+    assert np.array_equal(bits(y), bits(ref))
+
+
+SOLVE_CASES = [
+    # dims, x, m, tol, max_step
+    ((16, 16, 16), 3.0, 1.5 + 0.01j, 1e-5, 8),
+    ((8, 8, 16), 2.0, 1.7 + 0.1j, 1e-5, 8),
+    ((32, 32, 32), 5.0, 1.5 + 0.01j, 1e-3, 100),
+]
+
+
+def _relerr(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(np.asarray(b))), 1e-30)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+@pytest.mark.parametrize("case", range(len(SOLVE_CASES)))
+def test_solve_matches_oracle(orc32, solver, case):
+    dims, x, m, tol, steps = SOLVE_CASES[case]
+    kd = np.float32(2 * x / dims[0])
+    occ = target(orc32, dims)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, m)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = directions(np.arange(0, 181, 10))
+    print("solve", solver, dims, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        res = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=steps, err_max=tol, tau=0.5)
+        A = ctx.ahat()
+        P = [ctx.dipoles(r) for r in range(2)]
+    for pol in range(2):
+        ref = orc32.dda_fft(solver, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=steps, err_max=tol,
+                            tau=0.5, return_p=True)
+        got = res[pol]
+        print("pol", pol, "steps", got["steps"], ref["steps"], "P", _relerr(P[pol], ref["p"]),
+              "P bit-equal frac", np.mean(bits(P[pol]) == bits(ref["p"])), "esc", _relerr(got["esc"], ref["esc"]),
+              "cabs", got["cabs"] / ref["cabs"] - 1, "cext", got["cext"] / ref["cext"] - 1, flush=True)
+        assert got["steps"] == ref["steps"]
+        assert got["step_min"] == ref["step_min"]
+        assert got["restored"] == ref["restored"]
+        np.testing.assert_allclose(got["err_hist"], ref["err_hist"], rtol=1e-9)
+        assert _relerr(P[pol], ref["p"]) < 1e-5
+        assert _relerr(got["esc"], ref["esc"]) < 1e-5
+        np.testing.assert_allclose(got["cabs"], ref["cabs"], rtol=1e-4)
+        np.testing.assert_allclose(got["cext"], ref["cext"], rtol=1e-5)
+
+
+@pytest.mark.parametrize("tag,solvers", [("s16", (0, 1, 2, 3)), ("e8", (0, 1, 2, 3)), ("a32", (2,))])
+def test_solve_golden(orc32, tag, solvers):
+    """Against the reference's own DDA_FFT outputs (tests/golden/solves.npz)."""
+    g = golden("solves.npz")
+    dims = {"s16": (16, 16, 16), "e8": (8, 8, 16), "a32": (32, 32, 32)}[tag]
+    kd = g[tag + "_kd"]
+    alpha = g[tag + "_alpha"]
+    occ = target(orc32, dims)
+    assert int(np.count_nonzero(occ)) == int(g[tag + "_occ_count"])
+    dirs = directions(np.arange(0, 181, 10))
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, [0, 1])
+        for solver in solvers:
+            tol, steps = g["%s_sv%d_p0_cfg" % (tag, solver)]
+            res = ctx.solve(solver, alpha, [0, 0, 1], np.eye(3, dtype=np.complex64)[:2], dirs=dirs,
+                            max_step=int(steps), err_max=float(tol), tau=0.5)
+            for pol in range(2):
+                key = "%s_sv%d_p%d" % (tag, solver, pol)
+                hist = g[key + "_hist"]
+                got = res[pol]
+                print(key, "steps", len(got["err_hist"]), len(hist), "esc", _relerr(got["esc"], g[key + "_esc"]),
+                      "cabs", got["cabs"] / g[key + "_cabs"] - 1, "cext", got["cext"] / g[key + "_cext"] - 1,
+                      flush=True)
+                assert len(got["err_hist"]) == len(hist), key
+                np.testing.assert_allclose(np.round(got["err_hist"], 7), hist, atol=1.5e-7, err_msg=key)
+                assert _relerr(got["esc"], g[key + "_esc"]) < 2e-5, key
+                np.testing.assert_allclose(got["cabs"], g[key + "_cabs"], rtol=1e-4, err_msg=key)
+                np.testing.assert_allclose(got["cext"], g[key + "_cext"], rtol=2e-5, err_msg=key)
