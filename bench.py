@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark of the DDA-FFT solve path on one MI355X.
+
+Workload (the reference's single-instance 64^3 configuration): homogeneous
+oblate spheroid (b/a = 1, c/a = 0.5) in a 64x64x64 lattice (128^3 padded FFT),
+x = 10, m = 1.6 + 0.005i, polarizability CM, BiCGStab to a mean per-dipole
+residual of 1e-3 (max 300 steps), 2 incident polarizations, 4 scattering
+planes, theta 0..180 step 1 (724 directions per polarization).
+
+One step = one sample of that workload: both polarizations solved together as
+two right-hand sides sharing Â, including the far field in all 724 directions
+and Cabs/Cext, with every input resident in HBM (Â, occupancy and directions
+are uploaded once before the timed region).  value = BiCGStab solves/s, one
+solve being one (sample, polarization) DDA_FFT solve to the stopping residual.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this path runs on one GPU.
+"""
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dda-fft_amd"))
+
+import ddafft  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+GRID = (64, 64, 64)
+X_SIZE, M_INDEX, ASPECT = 10.0, (1.6, 0.005), (1.0, 0.5)
+NPLANES, THETA = 4, (0.0, 180.0, 1.0)
+TOL, MAX_STEPS = 1e-3, 300
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def build_workload(lib):
+    """Occupancy, polarizability and geometry through the library's host code."""
+    ni, nj, nl = GRID
+    kd = np.float32(2.0 * np.float32(X_SIZE) / np.float32(ni))            # main.cpp:318
+    occ = np.zeros(GRID, np.int8)
+    lib.ddafft_target_ellipsoid.argtypes = [C.c_int] * 3 + [C.c_double, C.c_double, C.c_void_p]
+    ndip = lib.ddafft_target_ellipsoid(ni, nj, nl, ASPECT[0], ASPECT[1], _p(occ))
+    lib.ddafft_polarizability.argtypes = [C.c_int, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    a = np.zeros(2, np.float32)
+    m = np.array(M_INDEX, np.float32)
+    lib.ddafft_polarizability(1, C.c_float(kd), _p(m), None, None, _p(a))
+    alpha = np.array([0, a[0] + 1j * a[1]], np.complex64)
+    nang = int(round((THETA[1] - THETA[0]) / THETA[2] + 1))
+    dirs = np.zeros((NPLANES, nang, 3), np.float32)
+    normals = np.zeros((NPLANES, 3), np.float32)
+    kinc = np.zeros(3, np.float32)
+    einc = np.zeros((2, 3), np.complex64)
+    lib.ddafft_geometry_fixed.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5 + [C.c_void_p] * 4
+    lib.ddafft_geometry_fixed(NPLANES, nang, THETA[0], THETA[2], 0.0, 0.0, 0.0, _p(dirs), _p(normals), _p(kinc),
+                              _p(einc))
+    return dict(kd=kd, occ=occ, ndip=ndip, alpha=alpha, dirs=dirs.reshape(-1, 3), kinc=kinc, einc=einc)
+
+
+def matvec_bytes(n, nrhs):
+    """Algorithmic HBM bytes of one matvec pass set (complex64 3-vectors, full
+    Â of 6 complex64 per padded cell, int8 occupancy), per pass."""
+    return {
+        "zf": nrhs * (24 + 48) * n,
+        "yf": nrhs * (48 + 96) * n,
+        "x": nrhs * (96 + 96) * n + 384 * n,
+        "yi": nrhs * (96 + 48) * n,
+        "zi": nrhs * (48 + 24 + 24) * n + n,
+    }
+
+
+def cpu_baseline(w, budget_s=10.0):
+    """The reference's own DDA_FFT (oracle/_ref, compiled from its sources) on
+    this host for the same sample; falls back to the oracle restatement."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    kind = "reference"
+    try:
+        impl = oracle.Reference()
+    except (FileNotFoundError, OSError):
+        impl = oracle.Oracle("f32")
+        kind = "port"
+    ahat = impl.build_ahat(GRID, w["kd"])
+    mv = np.array([0, 1], np.int32)
+    solves, t_total, reps = 0, 0.0, 0
+    while reps < 3 and (t_total < budget_s or reps == 0):
+        for pol in range(2):
+            kw = {"log_path": os.devnull} if kind == "reference" else {}
+            t0 = time.perf_counter()
+            impl.dda_fft(2, w["occ"], mv, w["alpha"], w["kd"], w["kinc"], w["einc"][pol], ahat, w["dirs"],
+                         max_step=MAX_STEPS, err_max=TOL, **kw)
+            t_total += time.perf_counter() - t0
+            solves += 1
+        reps += 1
+    return {
+        "value": round(solves / t_total, 4),
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": "%d BiCGStab solve(s) of the same 64^3 workload (DDA_FFT incl. far field, A-hat prebuilt), "
+                  "single worker as the reference runs one sample" % solves,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (torch), the reference kernel "
+                         "against which rocprofv3 FETCH_SIZE / WRITE_SIZE counters are scaled")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this DDA-FFT path runs on one GPU; multi-GPU runs are not part of this build "
+              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        return 2
+
+    if args.calib_copy_mib > 0:
+        import torch
+
+        src = torch.ones(args.calib_copy_mib * 2**20 // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    lib = ddafft.load()
+    w = build_workload(lib)
+    ctx = ddafft.Context(GRID, w["kd"], nrhs_max=2)
+    ctx.set_target(w["occ"], [0, 1])
+    ctx.set_directions(w["dirs"])
+    alpha2 = np.stack([w["alpha"], w["alpha"]])
+
+    def step():
+        return ctx.solve(2, alpha2, w["kinc"], w["einc"], max_step=MAX_STEPS, err_max=TOL)
+
+    for _ in range(args.warmup):
+        res = step()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    t1 = time.perf_counter()
+    pass_ms, nmv = ctx.profile(False)
+    dt = t1 - t0
+
+    solves = 2 * args.steps
+    matvecs = sum(r["matvecs"] for r in res) // 2 * args.steps     # batched matvecs (2 RHS each)
+    n = int(np.prod(GRID))
+    by = matvec_bytes(n, 2)
+    avg_ms = pass_ms / max(nmv, 1)
+    x_gbs = by["x"] / (avg_ms[2] * 1e-3) / 1e9
+    mv_ms = float(np.sum(avg_ms))
+    mv_gbs = sum(by.values()) / (mv_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "BiCGStab solves/s & FFT-matvec GB/s (%HBM peak) at 64³, resid 1e-3",
+        "value": round(solves / dt, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "64^3 oblate spheroid (b/a=1, c/a=0.5), x=10, m=1.6+0.005i, CM, BiCGStab resid 1e-3 "
+                        "(max 300), 2 polarizations batched as 2 RHS, 4 planes x 181 angles",
+            "lattice": list(GRID),
+            "padded_fft": [2 * g for g in GRID],
+            "dipoles": int(w["ndip"]),
+            "rhs_per_step": 2,
+            "iterations": [int(r["steps"]) for r in res],
+            "discrepancy": [[round(float(v), 7) for v in r["err_hist"]] for r in res],
+        },
+        "matvecs_per_s": round(matvecs * 2 / dt, 1),
+        "matvec_ms": round(mv_ms, 5),
+        "matvec_gbps": round(mv_gbs, 1),
+        "matvec_hbm_frac": round(mv_gbs / HBM_PEAK_GBS, 4),
+        "pass_ms": {k: round(float(v), 5) for k, v in zip(("zf", "yf", "x", "yi", "zi"), avg_ms)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_x (fused x-forward FFT, A-hat product, x-inverse FFT), 2 RHS",
+            "achieved": round(x_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(x_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(),
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the dominant kernel from the committed rocprof
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
+    path = os.path.join(ROOT, "profiles", "k_x_spheroid64_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -153,6 +153,14 @@ struct ddafft_ctx {
     int* stop_dev = nullptr;
     int* stop_host = nullptr;          // pinned ring
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+    // opt-in per-pass timing of every matvec (ddafft_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;   // 6 events per profiled matvec
+    std::vector<int> prof_iter;        // solver iteration of each recorded matvec
+    int prof_used = 0;                 // matvecs recorded since the last collect
+    double prof_ms[5] = {0, 0, 0, 0, 0};
+    long long prof_n = 0;
 };
 
 namespace {
@@ -278,6 +286,18 @@ int check_args(const ProArgs& pa, const EpiArgs& ea, bool solver) {
 // One matvec of nrhs right-hand sides: ZF -> YF -> X -> YI -> ZI.
 int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, const SolveState* st, int iter, const EpiArgs& ea,
                   int conj, hipEvent_t* ev = nullptr) {
+    if (!ev && c->prof) {
+        const size_t need = (size_t)6 * (c->prof_used + 1);
+        while (c->prof_ev.size() < need) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            c->prof_ev.push_back(e);
+        }
+        ev = &c->prof_ev[(size_t)6 * c->prof_used];
+        if ((int)c->prof_iter.size() <= c->prof_used) c->prof_iter.resize(c->prof_used + 1);
+        c->prof_iter[c->prof_used] = st ? iter : -1;
+        c->prof_used++;
+    }
     {
         const int rc = check_args(pa, ea, st != nullptr);
         if (rc) return rc;
@@ -383,6 +403,24 @@ int build_ahat(ddafft_ctx* c) {
     return DDAFFT_OK;
 }
 
+// Adds the pass durations of every matvec recorded since the last call (the
+// stream must be idle).  Matvecs of solver iterations past `last_iter` were
+// queued ahead of the stopping decision and exited at once; they are dropped.
+int prof_collect(ddafft_ctx* c, int last_iter = 1 << 30) {
+    for (int m = 0; m < c->prof_used; ++m) {
+        hipEvent_t* e = &c->prof_ev[(size_t)6 * m];
+        if (c->prof_iter[m] > last_iter) continue;
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+            c->prof_ms[k] += ms;
+        }
+        c->prof_n++;
+    }
+    c->prof_used = 0;
+    return DDAFFT_OK;
+}
+
 int upload(ddafft_ctx* c, void* dst, const void* src, size_t bytes) {
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     return DDAFFT_OK;
@@ -460,6 +498,7 @@ int ddafft_destroy(ddafft_ctx* c) {
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->stop_host) hipHostFree(c->stop_host);
+    for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
     for (hipEvent_t& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -923,6 +962,11 @@ int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, f
     TRY(observables(c, nrhs, kinc, einc, ndirs, ddirs, esc, cabs, cext));
     std::vector<SolveState> sf(nrhs);
     HIPCHECK(hipMemcpy(sf.data(), c->st, nrhs * sizeof(SolveState), hipMemcpyDeviceToHost));
+    if (c->prof) {
+        int last = 0;
+        for (int r = 0; r < nrhs; ++r) last = std::max(last, sf[r].last);
+        TRY(prof_collect(c, last));
+    }
     if (err_hist) {
         std::vector<double> h((size_t)nrhs * (max_step + 1));
         HIPCHECK(hipMemcpy(h.data(), c->hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -976,6 +1020,21 @@ int ddafft_far_field(ddafft_ctx* c, const float* p, int ndirs, const float* dirs
     const float e[6] = {1.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     TRY(upload(c, c->einc, e, sizeof(e)));
     return observables(c, 1, kz, e, ndirs, c->dirs, esc, nullptr, nullptr);
+}
+
+int ddafft_profile(ddafft_ctx* c, int enable, double* pass_ms, long long* matvecs) {
+    if (!c) return fail(DDAFFT_EINVAL, "NULL context");
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    TRY(prof_collect(c));
+    if (pass_ms)
+        for (int k = 0; k < 5; ++k) pass_ms[k] = c->prof_ms[k];
+    if (matvecs) *matvecs = c->prof_n;
+    if (enable) {
+        for (double& v : c->prof_ms) v = 0;
+        c->prof_n = 0;
+    }
+    c->prof = enable != 0;
+    return DDAFFT_OK;
 }
 
 int ddafft_time_matvec(ddafft_ctx* c, int nrhs, int reps, float* kernel_ms, float* total_ms) {

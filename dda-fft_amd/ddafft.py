@@ -21,7 +21,7 @@ SOLVERS = {"cg": 0, "aa": 1, "bicgstab": 2, "simple": 3}
 EXPORTS = (
     "ddafft_last_error", "ddafft_version", "ddafft_create", "ddafft_destroy", "ddafft_get_ahat",
     "ddafft_set_target", "ddafft_set_directions", "ddafft_convolution", "ddafft_matvec", "ddafft_solve",
-    "ddafft_get_dipoles", "ddafft_cabs_cext", "ddafft_far_field", "ddafft_time_matvec",
+    "ddafft_get_dipoles", "ddafft_cabs_cext", "ddafft_far_field", "ddafft_profile", "ddafft_time_matvec",
 )
 
 _lib = None
@@ -53,10 +53,11 @@ def load(path=LIB_PATH):
     lib.ddafft_get_dipoles.argtypes = [P, I, P]
     lib.ddafft_cabs_cext.argtypes = [P, P, P, P, P, P, P]
     lib.ddafft_far_field.argtypes = [P, P, I, P, P]
+    lib.ddafft_profile.argtypes = [P, I, P, P]
     lib.ddafft_time_matvec.argtypes = [P, I, I, P, P]
     for name in EXPORTS:
-        if name not in ("ddafft_last_error",):
-            getattr(lib, name).restype = I if name != "ddafft_last_error" else C.c_char_p
+        if name != "ddafft_last_error":
+            getattr(lib, name).restype = I
     _lib = lib
     return lib
 
@@ -183,6 +184,14 @@ class Context:
         _check(_lib.ddafft_far_field(self._h, _p(np.ascontiguousarray(p, np.complex64)), dirs.shape[0], _p(dirs),
                                      _p(esc)))
         return esc
+
+    def profile(self, enable):
+        """Returns (summed pass durations [5] in ms, matvecs timed) since the
+        last call, then enables or disables per-pass event timing."""
+        ms = np.zeros(5, np.float64)
+        n = C.c_longlong(0)
+        _check(_lib.ddafft_profile(self._h, int(enable), _p(ms), C.byref(n)))
+        return ms, int(n.value)
 
     def time_matvec(self, nrhs=1, reps=20):
         k = np.zeros(5, np.float32)

@@ -102,11 +102,55 @@ int ddafft_cabs_cext(ddafft_ctx* ctx, const float* alpha, const float* kinc, con
  * (FarField, dda_func.cpp:1576-1625). */
 int ddafft_far_field(ddafft_ctx* ctx, const float* p, int ndirs, const float* dirs, float* esc);
 
+/* Opt-in timing of every matvec pass with HIP events on the context's
+ * stream.  Returns, in pass_ms[5] and *matvecs, the summed durations of the
+ * five passes (z-fwd, y-fwd, fused x, y-inv, z-inv) and the number of matvecs
+ * timed since profiling was last enabled; then enables (enable != 0, with
+ * the sums reset) or disables profiling.  Benchmark aid; no counterpart in
+ * the reference. */
+int ddafft_profile(ddafft_ctx* ctx, int enable, double* pass_ms, long long* matvecs);
+
 /* Times `reps` matvecs of nrhs right-hand sides on device-resident data with
  * HIP events on the context's stream.  kernel_ms[5] receives the average
  * duration of each pass (z-fwd, y-fwd, fused x, y-inv, z-inv) and
  * total_ms the average wall time of one matvec.  Test and benchmark aid. */
 int ddafft_time_matvec(ddafft_ctx* ctx, int nrhs, int reps, float* kernel_ms, float* total_ms);
+
+/* ------------------------------------------------------------------------
+ * Host-side pieces (no GPU needed), bit-exact with the reference's float
+ * arithmetic.
+ * ------------------------------------------------------------------------ */
+
+/* Occupancy of the deterministic targets; return the number of occupied
+ * cells or -1 if the body does not fit: iGenerateEllipsoid (particle.cpp:408-444,
+ * targets 1 and 2), iGenerateParallelepiped (:501-527, target 3),
+ * iGenerateBisphere (:448-497, target 12). */
+int ddafft_target_ellipsoid(int ni, int nj, int nl, double b_a, double c_a, signed char* occ);
+int ddafft_target_parallelepiped(int ni, int nj, int nl, double b_a, double c_a, signed char* occ);
+int ddafft_target_bisphere(int ni, int nj, int nl, double r1_r2, signed char* occ);
+
+/* Polarizability model 1..5 (CM, CMRR, CMNR, ILDR, LDR) of refractive index
+ * m[2] at lattice spacing kd (Polarizability_*, dda_func.cpp:1631-1677, as
+ * selected by main.cpp:606-631 and 926-931); kinc[3] and einc[3][2] are used
+ * by model 5 only. */
+int ddafft_polarizability(int model, float kd, const float* m, const float* kinc, const float* einc, float* alpha);
+
+/* Scattering planes, directions and incident field for a fixed orientation
+ * (main.cpp:354-387 and 516-542): dirs [nplanes][nangles][3],
+ * normals [nplanes][3], kinc [3], einc [2][3][2].  Angles in degrees. */
+int ddafft_geometry_fixed(int nplanes, int nangles, double theta_min, double theta_step, double psi,
+                          double iota, double kappa, float* dirs, float* normals, float* kinc, float* einc);
+
+/* Mueller matrix [4][4] of one direction from the far fields of the two
+ * incident polarizations (BuildMatrixJones + JonesToMuller, mmuller.cpp:175-314). */
+int ddafft_muller(const float* kinc, const float* ksc, const float* normal, const float* einc0,
+                  const float* einc1, const float* esc0, const float* esc1, float* mm);
+
+/* Plane-averaged Mueller matrices of one sample (main.cpp:943-954):
+ * mm [nangles][4][4]; esc0/esc1 [nplanes][nangles][3][2]. */
+int ddafft_muller_average(int nplanes, int nangles, const float* kinc, const float* dirs,
+                          const float* normals, const float* einc, const float* esc0, const float* esc1,
+                          float* mm);
 
 #ifdef __cplusplus
 }
