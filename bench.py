@@ -121,8 +121,8 @@ def main():
                          "against which rocprofv3 FETCH_SIZE / WRITE_SIZE counters are scaled")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this DDA-FFT path runs on one GPU; multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py: this DDA-FFT path runs on one GPU; a multi-GPU run is not part of it "
+              "(independent solves would shard by instance, one lattice by slabs along i).", file=sys.stderr)
         return 2
 
     if args.calib_copy_mib > 0:
