@@ -765,10 +765,38 @@ void orc_far_field(int ni, int nj, int nl, const int *mv, const signed char *occ
    info[0] = last step executed, info[1] = step_min, info[2] = 1 if the best
    iterate was restored, info[3] = number of matvecs. p_out (optional) gets the
    final dipole moments. */
+static int dda_core(int solver, real tau, int ni, int nj, int nl, const real *alpha, const int *mv,
+                    const signed char *occ, int max_step, real err_max, real d0, const real *kinc,
+                    const real *einc, const real *ahat, int ndirs, const real *dirs, real *esc, real *cabs,
+                    real *cext, double *err_hist, int *info, real *p_out, const real *b);
+
 int orc_dda_fft(int solver, real tau, int ni, int nj, int nl, const real *alpha, const int *mv,
                 const signed char *occ, int max_step, real err_max, real d0, const real *kinc,
                 const real *einc, const real *ahat, int ndirs, const real *dirs, real *esc, real *cabs,
                 real *cext, double *err_hist, int *info, real *p_out) {
+    return dda_core(solver, tau, ni, nj, nl, alpha, mv, occ, max_step, err_max, d0, kinc, einc, ahat, ndirs,
+                    dirs, esc, cabs, cext, err_hist, info, p_out, NULL);
+}
+
+/* The same iteration for a caller-given source b in place of the incident
+   field's first approximation (P_0 = b, dda_func.cpp:1097).  Not a reference
+   entry point: the adjoint solve of the gradients uses it, because the DDA
+   matrix K = diag(alpha)^-1 - A is complex symmetric, so K^T lambda = g is the
+   forward system with the source diag(alpha) g.  No observables. */
+int orc_solve_b(int solver, real tau, int ni, int nj, int nl, const real *alpha, const int *mv,
+                const signed char *occ, int max_step, real err_max, const real *ahat, const real *b,
+                double *err_hist, int *info, real *p_out) {
+    const real kinc[3] = {0, 0, 1};
+    const real einc[6] = {1, 0, 0, 0, 0, 0};
+    real cabs, cext;
+    return dda_core(solver, tau, ni, nj, nl, alpha, mv, occ, max_step, err_max, (real)1, kinc, einc, ahat, 0,
+                    NULL, NULL, &cabs, &cext, err_hist, info, p_out, b);
+}
+
+static int dda_core(int solver, real tau, int ni, int nj, int nl, const real *alpha, const int *mv,
+                    const signed char *occ, int max_step, real err_max, real d0, const real *kinc,
+                    const real *einc, const real *ahat, int ndirs, const real *dirs, real *esc, real *cabs,
+                    real *cext, double *err_hist, int *info, real *p_out, const real *b) {
     orc_sys s = {ni, nj, nl, mv, (const cplx *)alpha, occ};
     size_t n = ncell(&s);
     const cmat *A = (const cmat *)ahat;
@@ -784,7 +812,12 @@ int orc_dda_fft(int solver, real tau, int ni, int nj, int nl, const real *alpha,
     cplx cA, cB, cW;
     int step_min = 0, o, last = 0;
 
-    orc_first_approx(ni, nj, nl, mv, alpha, occ, d0, kinc, einc, (real *)P);
+    if (b) {
+        const cvec *B = (const cvec *)b;
+        for (size_t k = 0; k < n; k++) P[k] = occupied(&s, k) ? B[k] : CV0;
+    } else {
+        orc_first_approx(ni, nj, nl, mv, alpha, occ, d0, kinc, einc, (real *)P);
+    }
     /* :1105-1135 R = -α·(A P) */
     convolve(&f, A, P, w, NULL, 0); nmv++;
     for (int i = 0; i < ni; i++)

@@ -165,8 +165,8 @@ class Oracle(_Base):
             self.real, self.cplx = np.float64, np.complex128
         for name in ("orc_fourier_phase", "orc_amatrix", "orc_build_ahat", "orc_fft3d", "orc_ifft3d",
                      "orc_convolution", "orc_matvec", "orc_first_approx", "orc_cabs_cext",
-                     "orc_far_field", "orc_polarizability", "orc_muller", "orc_dda_fft"):
-            getattr(self.lib, name).restype = None if name != "orc_dda_fft" else C.c_int
+                     "orc_far_field", "orc_polarizability", "orc_muller", "orc_dda_fft", "orc_solve_b"):
+            getattr(self.lib, name).restype = C.c_int if name in ("orc_dda_fft", "orc_solve_b") else None
 
     def _real_arg(self, x):
         return C.c_float(x) if self.real is np.float32 else C.c_double(x)
@@ -181,6 +181,23 @@ class Oracle(_Base):
                             _ptr(np.ascontiguousarray(alpha, self.cplx)), _ptr(occ), _ptr(x), _ptr(y),
                             _I(int(conj)))
         return y
+
+    def solve_b(self, solver, occ, mv, alpha, ahat, b, max_step=20, err_max=1e-3, tau=0.0):
+        """The solver iteration from the source b (P_0 = b); returns (P, info, err_hist)."""
+        occ = np.ascontiguousarray(occ, np.int8)
+        ni, nj, nl = occ.shape
+        b = np.ascontiguousarray(b, self.cplx).reshape(ni, nj, nl, 3)
+        p = np.zeros_like(b)
+        hist = np.full(max_step + 1, np.nan)
+        info = np.zeros(4, np.int32)
+        rc = self.lib.orc_solve_b(
+            _I(solver), self._real_arg(tau), _I(ni), _I(nj), _I(nl),
+            _ptr(np.ascontiguousarray(alpha, self.cplx)), _ptr(np.ascontiguousarray(mv, np.int32)), _ptr(occ),
+            _I(max_step), self._real_arg(err_max), _ptr(np.ascontiguousarray(ahat, self.cplx)), _ptr(b),
+            _ptr(hist), _ptr(info), _ptr(p))
+        if rc != 0:
+            raise RuntimeError("orc_solve_b failed")
+        return p, info, hist[: int(info[0]) + 1]
 
     def target_ellipsoid(self, n, b_a, c_a):
         occ = np.zeros(n, np.int8)
