@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -14,7 +15,9 @@
 #include "../../include/ddafft.h"
 #include "ahat_kernels.hpp"
 #include "device_types.hpp"
+#include "../host/host_internal.hpp"
 #include "fft_common.hpp"
+#include "gradient_kernels.hpp"
 #include "matvec_kernels.hpp"
 #include "solver_kernels.hpp"
 
@@ -132,6 +135,7 @@ struct ddafft_ctx {
     int* cells = nullptr;   // dipole cell indices (far field)
     int ncells = 0;
     long long ndip = 0;
+    std::vector<long long> mat_count;  // dipoles per material index
     bool have_target = false;
 
     // matvec scratch
@@ -148,6 +152,14 @@ struct ddafft_ctx {
     size_t partials_cap = 0;
     float* dirs = nullptr;
     int ndirs = 0, dirs_cap = 0;
+    // gradients: forward dipoles kept for the adjoint, per-axis phase table,
+    // small per-call arrays
+    float2* Pf = nullptr;
+    size_t Pf_cap = 0;
+    float2* phase = nullptr;
+    size_t phase_cap = 0;
+    char* gaux = nullptr;
+    size_t gaux_cap = 0;
     float2* esc = nullptr;
     int esc_cap = 0;
     int* stop_dev = nullptr;
@@ -190,15 +202,22 @@ int tile_zf(int n) {
     }
     return 1;
 }
-int tile_x(int n) {
-    switch (n) {
-        case 4: return TileX<4>::T;
-        case 8: return TileX<8>::T;
-        case 16: return TileX<16>::T;
-        case 32: return TileX<32>::T;
-        case 64: return TileX<64>::T;
-        case 128: return TileX<128>::T;
-        case 256: return TileX<256>::T;
+int tile_x(int n, int rb) {
+    switch (n * 4 + rb) {
+        case 4 * 4 + 1: return TileX<4, 1>::T;
+        case 8 * 4 + 1: return TileX<8, 1>::T;
+        case 16 * 4 + 1: return TileX<16, 1>::T;
+        case 32 * 4 + 1: return TileX<32, 1>::T;
+        case 64 * 4 + 1: return TileX<64, 1>::T;
+        case 128 * 4 + 1: return TileX<128, 1>::T;
+        case 256 * 4 + 1: return TileX<256, 1>::T;
+        case 4 * 4 + 2: return TileX<4, 2>::T;
+        case 8 * 4 + 2: return TileX<8, 2>::T;
+        case 16 * 4 + 2: return TileX<16, 2>::T;
+        case 32 * 4 + 2: return TileX<32, 2>::T;
+        case 64 * 4 + 2: return TileX<64, 2>::T;
+        case 128 * 4 + 2: return TileX<128, 2>::T;
+        case 256 * 4 + 2: return TileX<256, 2>::T;
     }
     return 1;
 }
@@ -323,10 +342,17 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, const SolveState* 
     }
     if (ev) HIPCHECK(hipEventRecord(ev[2], s));
     {
-        const int tj = std::min(tile_x(d.ni), 2 * d.nj);
-        dim3 grid(2 * d.nl * (2 * d.nj / tj), nrhs);
-        DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN>), grid, kBlock, 0, s, d, st, iter, c->b2, c->b3, c->ahat, conj, c->wpre_i,
-                                                                 c->tw_i, c->tw_2i));
+        // right-hand sides go through the X pass in pairs that share each Â read
+        const int rb = nrhs >= 2 ? 2 : 1;
+        const int tj = std::min(tile_x(d.ni, rb), 2 * d.nj);
+        dim3 grid(2 * d.nl * (2 * d.nj / tj), (nrhs + rb - 1) / rb);
+        if (rb == 2) {
+            DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN, 2>), grid, kBlock, 0, s, d, st, iter, c->b2, c->b3, c->ahat, conj,
+                                                      c->wpre_i, c->tw_i, c->tw_2i));
+        } else {
+            DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN, 1>), grid, kBlock, 0, s, d, st, iter, c->b2, c->b3, c->ahat, conj,
+                                                      c->wpre_i, c->tw_i, c->tw_2i));
+        }
     }
     if (ev) HIPCHECK(hipEventRecord(ev[3], s));
     {
@@ -494,7 +520,7 @@ int ddafft_destroy(ddafft_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->ahat, c->tables, c->occ, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
                     c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
-                    c->dirs, c->esc, c->stop_dev};
+                    c->dirs, c->esc, c->stop_dev, c->Pf, c->phase, c->gaux};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->stop_host) hipHostFree(c->stop_host);
@@ -632,6 +658,8 @@ int ddafft_set_target(ddafft_ctx* c, const signed char* occ, int nmat, const int
     if (!rc) rc = dev_alloc(&c->alpha, (size_t)nmat * c->nrhs_max);
     if (rc) return rc;
     c->nmat = nmat;
+    c->mat_count.assign(nmat, 0);
+    for (long long k = 0; k < n; ++k) c->mat_count[occ[k]] += mv[occ[k]] ? 1 : 0;
     c->ncells = (int)cells.size();
     c->ndip = (long long)cells.size();
     if ((rc = upload(c, c->occ, occ, (size_t)n))) return rc;
@@ -845,7 +873,10 @@ int observables(ddafft_ctx* c, int nrhs, const float* kinc, const float* einc, i
     const Geometry g = make_geometry(c, kinc);
     // cross sections
     const int cb = (int)std::min<long long>((d.n + kBlock - 1) / kBlock, 512);
-    const int fchunk = ndirs > 0 ? (int)std::max(1, std::min(64, (int)((c->ncells + 8191) / 8192))) : 0;
+    // dipole chunks per direction tile: enough blocks to fill the chip
+    const int dtiles = (ndirs + kFD - 1) / kFD;
+    const int fchunk = ndirs > 0 ? (int)std::max(1, std::min({64, (c->ncells + 4095) / 4096,
+                                                              (2048 + dtiles - 1) / std::max(dtiles, 1)})) : 0;
     TRY(ensure_partials(c, std::max<size_t>((size_t)nrhs * cb * 2, (size_t)nrhs * fchunk * std::max(ndirs, 1) * 6)));
     dim3 gc(cb, nrhs);
     hipLaunchKernelGGL(k_cabs_cext, gc, kBlock, 0, c->stream, d, g, c->occ, c->alpha, c->nmat, c->einc, c->P, c->partials);
@@ -862,12 +893,20 @@ int observables(ddafft_ctx* c, int nrhs, const float* kinc, const float* einc, i
             c->esc_cap = nrhs * ndirs * 3;
         }
         // partials for cabs were copied above (stream-ordered), so reuse is safe
-        dim3 gf(fchunk, (ndirs + kFD - 1) / kFD, nrhs);
-        hipLaunchKernelGGL(k_far_field, gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, ddirs, ndirs, c->P, c->partials);
-        HIPCHECK(hipGetLastError());
         const int tot = nrhs * ndirs * 3;
-        hipLaunchKernelGGL(k_far_field_sum, (tot + kBlock - 1) / kBlock, kBlock, 0, c->stream, c->partials, fchunk, ndirs, nrhs,
-                                                                                 c->esc);
+        for (int r0 = 0; r0 < nrhs; r0 += 2) {
+            const int rr = std::min(2, nrhs - r0);
+            dim3 gf(fchunk, (ndirs + kFD - 1) / kFD);
+            if (rr == 2)
+                hipLaunchKernelGGL((k_far_field<2>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, ddirs, ndirs,
+                                   r0, c->P, c->partials);
+            else
+                hipLaunchKernelGGL((k_far_field<1>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, ddirs, ndirs,
+                                   r0, c->P, c->partials);
+            HIPCHECK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_far_field_sum, (nrhs * ndirs + kBlock - 1) / kBlock, kBlock, 0, c->stream, c->partials,
+                           fchunk, ndirs, nrhs, ddirs, c->esc);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(esc, c->esc, (size_t)tot * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
     }
@@ -887,28 +926,16 @@ int observables(ddafft_ctx* c, int nrhs, const float* kinc, const float* einc, i
 
 }  // namespace
 
-int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, const float* alpha,
-                 const float* kinc, const float* einc, int ndirs, const float* dirs, float* esc, float* cabs,
-                 float* cext, double* err_hist, int* info) {
-    if (!c || !alpha || !kinc || !einc) return fail(DDAFFT_EINVAL, "NULL argument");
-    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
-    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDAFFT_EINVAL, "nrhs outside [1, nrhs_max]");
-    if (solver < 0 || solver > 3 || max_step < 0) return fail(DDAFFT_EINVAL, "bad solver or max_step");
-    if (ndirs < 0 || (ndirs > 0 && !esc)) return fail(DDAFFT_EINVAL, "bad direction arguments");
-    const float* ddirs = nullptr;
-    if (ndirs > 0) {
-        if (dirs) {
-            TRY(ddafft_set_directions(c, ndirs, dirs));
-        } else if (ndirs != c->ndirs) {
-            return fail(DDAFFT_EINVAL, "dirs == NULL but ndirs differs from the resident directions");
-        }
-        ddirs = c->dirs;
-    }
+namespace {
+
+// The solver iteration for nrhs right-hand sides whose sources (P_0) are
+// already in c->P and whose polarizabilities are in c->alpha: step 0, the
+// pipelined iterations, and the best-iterate restore.  Fills the final solver
+// states, and err_hist / info when given.
+int solve_core(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, double* err_hist,
+               int* info, std::vector<SolveState>& sf) {
     Dims d = c->d;
     d.nrhs = nrhs;
-    // inputs
-    TRY(upload(c, c->alpha, alpha, (size_t)nrhs * c->nmat * sizeof(float2)));
-    TRY(upload(c, c->einc, einc, (size_t)nrhs * 3 * sizeof(float2)));
     if (max_step + 1 > c->hist_cap || c->hist == nullptr) {
         HIPCHECK(hipStreamSynchronize(c->stream));
         if (c->hist) hipFree(c->hist);
@@ -924,12 +951,6 @@ int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, f
     TRY(upload(c, c->st, s0.data(), nrhs * sizeof(SolveState)));
     TRY(ensure_partials(c, (size_t)nrhs * std::max(zi_blocks(d), update_blocks(d)) * kSlots));
 
-    const Geometry g = make_geometry(c, kinc);
-    {
-        dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), nrhs);
-        hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->occ, c->mv, c->alpha, c->nmat, c->einc, c->P);
-        HIPCHECK(hipGetLastError());
-    }
     Seq q{c, nrhs, FinArgs{}};
     q.fa.hist = c->hist;
     q.fa.max_step = max_step;
@@ -959,9 +980,9 @@ int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, f
         hipLaunchKernelGGL(k_restore, grid, kBlock, 0, c->stream, d, c->st, c->P, c->Pmin);
         HIPCHECK(hipGetLastError());
     }
-    TRY(observables(c, nrhs, kinc, einc, ndirs, ddirs, esc, cabs, cext));
-    std::vector<SolveState> sf(nrhs);
-    HIPCHECK(hipMemcpy(sf.data(), c->st, nrhs * sizeof(SolveState), hipMemcpyDeviceToHost));
+    sf.resize(nrhs);
+    HIPCHECK(hipMemcpyAsync(sf.data(), c->st, nrhs * sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
     if (c->prof) {
         int last = 0;
         for (int r = 0; r < nrhs; ++r) last = std::max(last, sf[r].last);
@@ -989,6 +1010,292 @@ int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, f
             info[4 * r + 2] = sf[r].restored;
             info[4 * r + 3] = mvs;
         }
+    }
+    return DDAFFT_OK;
+}
+
+}  // namespace
+
+int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, const float* alpha,
+                 const float* kinc, const float* einc, int ndirs, const float* dirs, float* esc, float* cabs,
+                 float* cext, double* err_hist, int* info) {
+    if (!c || !alpha || !kinc || !einc) return fail(DDAFFT_EINVAL, "NULL argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDAFFT_EINVAL, "nrhs outside [1, nrhs_max]");
+    if (solver < 0 || solver > 3 || max_step < 0) return fail(DDAFFT_EINVAL, "bad solver or max_step");
+    if (ndirs < 0 || (ndirs > 0 && !esc)) return fail(DDAFFT_EINVAL, "bad direction arguments");
+    const float* ddirs = nullptr;
+    if (ndirs > 0) {
+        if (dirs) {
+            TRY(ddafft_set_directions(c, ndirs, dirs));
+        } else if (ndirs != c->ndirs) {
+            return fail(DDAFFT_EINVAL, "dirs == NULL but ndirs differs from the resident directions");
+        }
+        ddirs = c->dirs;
+    }
+    Dims d = c->d;
+    d.nrhs = nrhs;
+    TRY(upload(c, c->alpha, alpha, (size_t)nrhs * c->nmat * sizeof(float2)));
+    TRY(upload(c, c->einc, einc, (size_t)nrhs * 3 * sizeof(float2)));
+    const Geometry g = make_geometry(c, kinc);
+    {
+        dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), nrhs);
+        hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->occ, c->mv, c->alpha, c->nmat, c->einc, c->P);
+        HIPCHECK(hipGetLastError());
+    }
+    std::vector<SolveState> sf;
+    TRY(solve_core(c, nrhs, solver, tau, max_step, err_max, err_hist, info, sf));
+    TRY(observables(c, nrhs, kinc, einc, ndirs, ddirs, esc, cabs, cext));
+    return DDAFFT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <class T>
+int ensure_buf(ddafft_ctx* c, T** p, size_t& cap, size_t count) {
+    if (count <= cap && *p) return DDAFFT_OK;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (*p) HIPCHECK(hipFree(*p));
+    *p = nullptr;
+    cap = 0;
+    TRY(dev_alloc(p, count));
+    cap = count;
+    return DDAFFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, int max_step, float err_max,
+                    const float* alpha, const float* dalpha_dm, const float* m, const float* kinc, const float* einc,
+                    int ndirs, const float* dirs, const float* normals, const float* weights, float* f, float* esc,
+                    float* cabs, float* cext, float* s11, float* grad_m, float* grad_chi, int* info) {
+    if (!c || !alpha || !dalpha_dm || !m || !kinc || !einc || !weights) return fail(DDAFFT_EINVAL, "NULL argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    if (npol < 1 || npol > 2 || nobj < 1) return fail(DDAFFT_EINVAL, "npol must be 1 or 2 and nobj >= 1");
+    const int nadj = npol * nobj;
+    if (nadj > c->nrhs_max) return fail(DDAFFT_EINVAL, "npol * nobj exceeds nrhs_max");
+    if (ndirs < 0 || (ndirs > 0 && !dirs && ndirs != c->ndirs)) return fail(DDAFFT_EINVAL, "bad directions");
+    const int nmat = c->nmat;
+    const int nw = 2 + ndirs;
+    bool any_s11 = false;
+    for (int o = 0; o < nobj; ++o)
+        for (int a = 0; a < ndirs; ++a) any_s11 = any_s11 || weights[o * nw + 2 + a] != 0.f;
+    if (any_s11 && npol != 2) return fail(DDAFFT_EINVAL, "S11 weights need both polarizations (npol = 2)");
+    if ((any_s11 || s11) && ndirs > 0 && (!normals || npol != 2))
+        return fail(DDAFFT_EINVAL, "S11 needs npol = 2 and the plane normal of every direction");
+    if ((size_t)nadj * nmat * 2 > 4096) return fail(DDAFFT_EINVAL, "too many objectives x materials");
+    Dims d = c->d;
+    const long long n = d.n;
+
+    // forward solve of the npol polarizations; its dipoles are kept
+    std::vector<float> esc_h((size_t)npol * std::max(ndirs, 1) * 6), cabs_h(npol), cext_h(npol);
+    std::vector<int> info_f(4 * npol);
+    TRY(ddafft_solve(c, npol, solver, tau, max_step, err_max, alpha, kinc, einc, ndirs, dirs,
+                     ndirs > 0 ? esc_h.data() : nullptr, cabs_h.data(), cext_h.data(), nullptr, info_f.data()));
+    std::vector<float> hdirs((size_t)ndirs * 3);
+    if (ndirs > 0) {
+        if (dirs)
+            std::memcpy(hdirs.data(), dirs, hdirs.size() * sizeof(float));
+        else
+            HIPCHECK(hipMemcpy(hdirs.data(), c->dirs, hdirs.size() * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    TRY(ensure_buf(c, &c->Pf, c->Pf_cap, (size_t)npol * 3 * n));
+    HIPCHECK(hipMemcpyAsync(c->Pf, c->P, (size_t)npol * 3 * n * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
+
+    // objective pieces on the host (double)
+    using Cd = std::complex<double>;
+    std::vector<double> m00(ndirs, 0.);
+    std::vector<Cd> v((size_t)2 * ndirs * 3, Cd(0., 0.));   // [p][a][3] dM00/dEsc_p
+    if (npol == 2 && ndirs > 0 && normals)
+        for (int a = 0; a < ndirs; ++a)
+            m00[a] = ddafft_host::m00_grad(kinc, &hdirs[3 * a], normals + 3 * a, einc, einc + 6,
+                                           &esc_h[(size_t)a * 6], &esc_h[((size_t)ndirs + a) * 6], &v[(size_t)a * 3],
+                                           &v[((size_t)ndirs + a) * 3]);
+    std::vector<float> r0(npol);
+    for (int p = 0; p < npol; ++p) r0[p] = cabs_prefactor(einc + 6 * p);
+    // aux layout (16-byte aligned pieces)
+    auto al16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+    const size_t off_u = 0;
+    const size_t off_ce = al16(off_u + (size_t)nadj * std::max(ndirs, 1) * 3 * sizeof(float2));
+    const size_t off_ca = al16(off_ce + nadj * sizeof(float2));
+    const size_t off_ex = al16(off_ca + nadj * sizeof(float));
+    const size_t off_s = al16(off_ex + nadj * sizeof(float2));
+    const size_t off_is = al16(off_s + nadj * sizeof(float));
+    const size_t off_dc = al16(off_is + nadj * sizeof(float));
+    const size_t aux_bytes = al16(off_dc + (size_t)npol * nmat * sizeof(float2));
+    std::vector<char> aux(aux_bytes, 0);
+    float2* hu = reinterpret_cast<float2*>(aux.data() + off_u);
+    float2* hce = reinterpret_cast<float2*>(aux.data() + off_ce);
+    float* hca = reinterpret_cast<float*>(aux.data() + off_ca);
+    float2* hex = reinterpret_cast<float2*>(aux.data() + off_ex);
+    float* hs = reinterpret_cast<float*>(aux.data() + off_s);
+    float* his = reinterpret_cast<float*>(aux.data() + off_is);
+    float2* hdc = reinterpret_cast<float2*>(aux.data() + off_dc);
+    std::vector<float2> alpha_adj((size_t)nadj * nmat);
+    for (int o = 0; o < nobj; ++o) {
+        const float* w = weights + (size_t)o * nw;
+        double fo = 0., ce = 0., ca = 0.;
+        for (int p = 0; p < npol; ++p) {
+            ce += cext_h[p] / npol;
+            ca += cabs_h[p] / npol;
+        }
+        fo = w[0] * ce + w[1] * ca;
+        for (int a = 0; a < ndirs; ++a) fo += w[2 + a] * m00[a];
+        if (f) f[o] = (float)fo;
+        for (int p = 0; p < npol; ++p) {
+            const int r = o * npol + p;
+            hce[r] = make_float2(0.f, (float)(-(double)w[0] / npol * r0[p] / 2.));
+            hca[r] = (float)((double)w[1] / npol * r0[p]);
+            hex[r] = make_float2(0.f, (float)(-(double)w[1] / npol * r0[p] / 2.));
+            for (int a = 0; a < ndirs; ++a) {
+                const double* nn = nullptr;
+                const double nd[3] = {hdirs[3 * a], hdirs[3 * a + 1], hdirs[3 * a + 2]};
+                nn = nd;
+                const Cd* vv = &v[((size_t)p * ndirs + a) * 3];
+                const Cd nv = nn[0] * vv[0] + nn[1] * vv[1] + nn[2] * vv[2];
+                for (int q = 0; q < 3; ++q) {
+                    const Cd t = (double)w[2 + a] * (vv[q] - nn[q] * nv);
+                    hu[((size_t)r * ndirs + a) * 3 + q] = make_float2((float)t.real(), (float)t.imag());
+                }
+            }
+            for (int q = 0; q < nmat; ++q)
+                alpha_adj[(size_t)r * nmat + q] = make_float2(alpha[((size_t)p * nmat + q) * 2],
+                                                              alpha[((size_t)p * nmat + q) * 2 + 1]);
+        }
+    }
+    for (int p = 0; p < npol; ++p)
+        for (int q = 0; q < nmat; ++q) {
+            const Cd mq(m[2 * q], m[2 * q + 1]);
+            const Cd da(dalpha_dm[((size_t)p * nmat + q) * 2], dalpha_dm[((size_t)p * nmat + q) * 2 + 1]);
+            const Cd dc = std::abs(mq) > 0. ? da / (2. * mq) : Cd(0., 0.);
+            hdc[(size_t)p * nmat + q] = make_float2((float)dc.real(), (float)dc.imag());
+        }
+    TRY(ensure_buf(c, &c->gaux, c->gaux_cap, aux_bytes));
+    TRY(upload(c, c->alpha, alpha_adj.data(), alpha_adj.size() * sizeof(float2)));
+    const int naxis = d.ni + d.nj + d.nl;
+    if (ndirs > 0) {
+        TRY(ensure_buf(c, &c->phase, c->phase_cap, (size_t)ndirs * naxis));
+        const Geometry g = make_geometry(c, kinc);
+        hipLaunchKernelGGL(k_axis_phase, (int)std::min<long long>(((long long)ndirs * naxis + kBlock - 1) / kBlock, 1024),
+                           kBlock, 0, c->stream, d, g, c->dirs, ndirs, c->phase);
+        HIPCHECK(hipGetLastError());
+    }
+    // the weights and coefficients (the scales are filled in below)
+    HIPCHECK(hipMemcpyAsync(c->gaux, aux.data(), off_s, hipMemcpyHostToDevice, c->stream));
+
+    // adjoint sources b = diag(alpha) g and their mean norms
+    d.nrhs = nadj;
+    const int nblk = (int)std::min<long long>((n + kBlock - 1) / kBlock, 1024);
+    TRY(ensure_partials(c, (size_t)nadj * nblk));
+    {
+        AdjArgs aa{};
+        aa.Pf = c->Pf;
+        aa.alpha = c->alpha;
+        aa.einc = c->einc;
+        aa.cext_c = reinterpret_cast<const float2*>(c->gaux + off_ce);
+        aa.cabs_c = reinterpret_cast<const float*>(c->gaux + off_ca);
+        aa.u = reinterpret_cast<const float2*>(c->gaux + off_u);
+        aa.tab = c->phase;
+        aa.ndirs = ndirs;
+        aa.npol = npol;
+        aa.nmat = nmat;
+        aa.tdirs = ndirs > 0 ? std::max(1, std::min(ndirs, kAdjLds / (naxis + 3))) : 1;
+        aa.b = c->P;
+        aa.partials = c->partials;
+        if (ndirs > 0 && !aa.tab) return fail(DDAFFT_ESTATE, "internal: phase table missing");
+        const Geometry g = make_geometry(c, kinc);
+        hipLaunchKernelGGL(k_adj_source, dim3(nblk, nadj), kBlock, 0, c->stream, d, g, c->occ, c->mv, aa);
+        HIPCHECK(hipGetLastError());
+    }
+    std::vector<double> nb((size_t)nadj * nblk);
+    HIPCHECK(hipMemcpyAsync(nb.data(), c->partials, nb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    // Scale each source to the size of the forward source alpha_j E_j so that
+    // err_max means the same relative residual for both solves.
+    for (int r = 0; r < nadj; ++r) {
+        const int p = r % npol;
+        double sb = 0., sa = 0.;
+        for (int b = 0; b < nblk; ++b) sb += nb[(size_t)r * nblk + b];
+        for (int q = 0; q < nmat; ++q)
+            sa += (double)c->mat_count[q] *
+                  std::hypot((double)alpha[((size_t)p * nmat + q) * 2], (double)alpha[((size_t)p * nmat + q) * 2 + 1]);
+        double e2 = 0.;
+        for (int q = 0; q < 6; ++q) e2 += (double)einc[6 * p + q] * einc[6 * p + q];
+        const double na = sa * std::sqrt(e2);
+        const double sc = (sb > 0. && na > 0.) ? sb / na : 1.;
+        hs[r] = (float)sc;
+        his[r] = (float)(1. / sc);
+    }
+    HIPCHECK(hipMemcpyAsync(c->gaux + off_s, aux.data() + off_s, aux_bytes - off_s, hipMemcpyHostToDevice, c->stream));
+    {
+        dim3 grid((int)std::min<long long>((3 * n + kBlock - 1) / kBlock, 2048), nadj);
+        hipLaunchKernelGGL(k_scale_rhs, grid, kBlock, 0, c->stream, d, reinterpret_cast<const float*>(c->gaux + off_is),
+                           c->P);
+        HIPCHECK(hipGetLastError());
+    }
+
+    // adjoint solve: the same iteration from the sources in c->P
+    std::vector<SolveState> sfa;
+    std::vector<int> info_a(4 * nadj);
+    TRY(solve_core(c, nadj, solver, tau, max_step, err_max, nullptr, info_a.data(), sfa));
+
+    // df/dalpha per dipole, the chi map, and the per-material sums
+    {
+        GradArgs ga{};
+        ga.lam = c->P;
+        ga.s = reinterpret_cast<const float*>(c->gaux + off_s);
+        ga.Pf = c->Pf;
+        ga.alpha = c->alpha;
+        ga.dadchi = reinterpret_cast<const float2*>(c->gaux + off_dc);
+        ga.expl = reinterpret_cast<const float2*>(c->gaux + off_ex);
+        ga.npol = npol;
+        ga.nmat = nmat;
+        ga.H = c->R;
+        ga.gchi = c->S;
+        dim3 grid((int)std::min<long long>((n + kBlock - 1) / kBlock, 2048), nobj);
+        hipLaunchKernelGGL(k_grad, grid, kBlock, 0, c->stream, d, c->occ, c->mv, ga);
+        HIPCHECK(hipGetLastError());
+    }
+    const int nb2 = (int)std::min<long long>((n + kBlock - 1) / kBlock, 256);
+    TRY(ensure_partials(c, (size_t)nadj * nb2 * nmat * 2));
+    hipLaunchKernelGGL(k_mat_sum, dim3(nb2, nadj), kBlock, 0, c->stream, d, c->occ, c->R, nmat, c->partials);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> ms((size_t)nadj * nb2 * nmat * 2);
+    HIPCHECK(hipMemcpyAsync(ms.data(), c->partials, ms.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (grad_chi)
+        HIPCHECK(hipMemcpyAsync(grad_chi, c->S, (size_t)nobj * n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (grad_m) {
+        for (int o = 0; o < nobj; ++o)
+            for (int q = 0; q < nmat; ++q) {
+                Cd t(0., 0.);
+                for (int p = 0; p < npol && q > 0; ++p) {
+                    const int r = o * npol + p;
+                    Cd h(0., 0.);
+                    for (int b = 0; b < nb2; ++b) {
+                        const double* e = &ms[(((size_t)r * nb2 + b) * nmat + q) * 2];
+                        h += Cd(e[0], e[1]);
+                    }
+                    t += h * Cd(dalpha_dm[((size_t)p * nmat + q) * 2], dalpha_dm[((size_t)p * nmat + q) * 2 + 1]);
+                }
+                grad_m[((size_t)o * nmat + q) * 2] = (float)(2. * t.real());
+                grad_m[((size_t)o * nmat + q) * 2 + 1] = (float)(-2. * t.imag());
+            }
+    }
+    if (esc && ndirs > 0) std::memcpy(esc, esc_h.data(), (size_t)npol * ndirs * 6 * sizeof(float));
+    for (int p = 0; p < npol; ++p) {
+        if (cabs) cabs[p] = cabs_h[p];
+        if (cext) cext[p] = cext_h[p];
+    }
+    if (s11)
+        for (int a = 0; a < ndirs; ++a) s11[a] = (float)m00[a];
+    if (info) {
+        std::memcpy(info, info_f.data(), info_f.size() * sizeof(int));
+        std::memcpy(info + 4 * npol, info_a.data(), info_a.size() * sizeof(int));
     }
     return DDAFFT_OK;
 }

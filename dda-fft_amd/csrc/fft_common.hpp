@@ -182,9 +182,12 @@ struct StageRunner {
             const int step = N / span;
             constexpr int NB = N / R;
             const int total = nlines * NB;
+            // consecutive lanes take the same butterfly of consecutive lines:
+            // with the padded line stride their LDS addresses fall in
+            // different banks (lanes walking one line would hit one bank)
             for (int f = threadIdx.x; f < total; f += blockDim.x) {
-                const int line = f / NB;
-                const int b = f - line * NB;
+                const int b = f / nlines;
+                const int line = f - b * nlines;
                 const int g = b / H;
                 const int i = b - g * H;
                 float2* v = s + line * ls + g * span + i;
@@ -213,6 +216,20 @@ struct StageRunner {
 template <int N>
 __device__ __forceinline__ void fft_lds(float2* s, int nlines, int ls, const float2* __restrict__ tw) {
     StageRunner<N, 0>::run(s, nlines, ls, tw, 1);
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware tile order.  The dispatcher hands consecutive workgroups to the
+// chip's 8 XCDs in turn, each with its own L2.  Renumbering them so that XCD x
+// works through the x-th contiguous range of tiles keeps the neighbouring
+// tiles whose short transposed runs share 128-byte lines in one L2.
+// ---------------------------------------------------------------------------
+
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    if (nb % kXcds != 0) return b;
+    return (b % kXcds) * (nb / kXcds) + b / kXcds;
 }
 
 // ---------------------------------------------------------------------------

@@ -27,9 +27,10 @@
 namespace ddafft {
 
 // Tile widths (lines per block) chosen so the LDS footprint stays <= 48 KB
-// (64 KB for the fused X pass, which holds 3 components).
+// (64 KB for the fused X pass, which holds 3 components of RB right-hand
+// sides).
 template <int N> struct TileZF { static constexpr int T = clamp16(pow2floor(6144 / (2 * (N + kLpad)))); };
-template <int N> struct TileX  { static constexpr int T = clamp16(pow2floor(8192 / (6 * (N + kLpad)))); };
+template <int N, int RB> struct TileX { static constexpr int T = clamp16(pow2floor(8192 / (6 * RB * (N + kLpad)))); };
 template <int N> struct TileYI { static constexpr int T = clamp16(pow2floor(6144 / (N + kLpad))); };
 template <int N> struct TileZI { static constexpr int T = clamp16(pow2floor(6144 / (3 * (N + kLpad)))); };
 
@@ -162,7 +163,8 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
 }
 
 // ---------------------------------------------------------------------------
-// X: lines (J, L) along i for the 3 components of one right-hand side.
+// X: lines (J, L) along i for the 3 components of RB right-hand sides, which
+// share each Â value read from HBM.
 // ---------------------------------------------------------------------------
 
 // y = Â x for one padded cell (dda_func.cpp:163-181: c3 = c0 + c1, out = c2 + c3)
@@ -173,47 +175,54 @@ __device__ __forceinline__ void ahat_apply(const float2* a, float2 x, float2 y, 
     oz = cadd(cmul(a[5], z), cadd(cmul(a[2], x), cmul(a[4], y)));
 }
 
-template <int NI>
+template <int NI, int RB>
 __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st, int iter,
                                               const float2* __restrict__ b2, float2* __restrict__ b3,
                                               const float2* __restrict__ ahat, int conj,
                                               const float2* __restrict__ wpre, const float2* __restrict__ tw1,
                                               const float2* __restrict__ tw2) {
-    constexpr int TJ = TileX<NI>::T;
+    constexpr int CB = 3 * RB;                       // component lines per (J, L)
+    constexpr int TJ = TileX<NI, RB>::T;
     constexpr int NI2 = 2 * NI;
     constexpr int LS1 = NI + kLpad, LS2 = 2 * LS1;
     constexpr int ITEMS = (TJ * NI2 + kBlock - 1) / kBlock;
-    __shared__ float2 s[3 * TJ * LS2];
-    const int rhs = blockIdx.y;
-    if (st && iter >= st[rhs].stop_at) return;
+    __shared__ float2 s[CB * TJ * LS2];
+    const int rhs0 = blockIdx.y * RB;
+    bool act[RB];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        act[r] = rhs0 + r < d.nrhs && !(st && iter >= st[rhs0 + r].stop_at);
+        any = any || act[r];
+    }
+    if (!any) return;
     const int NJ2 = 2 * d.nj, NL2 = 2 * d.nl;
     const int tj = min(TJ, NJ2), njt = NJ2 / tj;
-    const int L = blockIdx.x / njt, J0 = (blockIdx.x - L * njt) * tj;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int L = tile / njt, J0 = (tile - L * njt) * tj;
 
-    for (int f = threadIdx.x; f < 3 * tj * NI; f += blockDim.x) {
-        const int ii = f % NI, r = f / NI, t = r % tj, comp = r / tj;
-        const int c = rhs * 3 + comp;
-        const float2 v = b2[(((long long)c * NJ2 + J0 + t) * NL2 + L) * NI + ii];
-        const int line = (comp * tj + t) * 2;
+    for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
+        const int ii = f % NI, r = f / NI, t = r % tj, cc = r / tj;
+        const int c = rhs0 * 3 + cc;
+        float2 v = make_float2(0.f, 0.f);
+        if (act[cc / 3]) v = b2[(((long long)c * NJ2 + J0 + t) * NL2 + L) * NI + ii];
+        const int line = (cc * tj + t) * 2;
         const int slot = perm_slot<NI>(ii);
         s[line * LS1 + slot] = v;
         s[(line + 1) * LS1 + slot] = cmul(wpre[ii], v);
     }
     __syncthreads();
-    fft_lds<NI>(s, 6 * tj, LS1, tw1);
+    fft_lds<NI>(s, 2 * CB * tj, LS1, tw1);
 
-    // Â multiply in natural order of I; results land digit-reversed and
-    // conjugated for the inverse transform (ifft_line, fft_kernel.hpp:301-322).
-    float2 y[ITEMS][3];
+    // Â multiply in natural order of I, one Â read for the RB right-hand
+    // sides; results land digit-reversed and conjugated for the inverse
+    // transform (ifft_line, fft_kernel.hpp:301-322).
+    float2 y[ITEMS][CB];
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
         const int f = threadIdx.x + it * kBlock;
         if (f < tj * NI2) {
             const int I = f % NI2, t = f / NI2;
-            float2 xv[3];
-#pragma unroll
-            for (int comp = 0; comp < 3; ++comp)
-                xv[comp] = s[((comp * tj + t) * 2 + (I & 1)) * LS1 + (I >> 1)];
             int Ja = J0 + t, La = L, Ia = I;
             if (conj) {
                 Ja = (NJ2 - Ja) & (NJ2 - 1);
@@ -227,7 +236,14 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
             if (conj)
 #pragma unroll
                 for (int e = 0; e < 6; ++e) a[e].y = -a[e].y;
-            ahat_apply(a, xv[0], xv[1], xv[2], y[it][0], y[it][1], y[it][2]);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                float2 xv[3];
+#pragma unroll
+                for (int comp = 0; comp < 3; ++comp)
+                    xv[comp] = s[(((r * 3 + comp) * tj + t) * 2 + (I & 1)) * LS1 + (I >> 1)];
+                ahat_apply(a, xv[0], xv[1], xv[2], y[it][r * 3], y[it][r * 3 + 1], y[it][r * 3 + 2]);
+            }
         }
     }
     __syncthreads();
@@ -238,15 +254,16 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
             const int I = f % NI2, t = f / NI2;
             const int slot = perm_slot<NI2>(I);
 #pragma unroll
-            for (int comp = 0; comp < 3; ++comp) s[(comp * tj + t) * LS2 + slot] = cconj(y[it][comp]);
+            for (int cc = 0; cc < CB; ++cc) s[(cc * tj + t) * LS2 + slot] = cconj(y[it][cc]);
         }
     }
     __syncthreads();
-    fft_lds<NI2>(s, 3 * tj, LS2, tw2);
-    for (int f = threadIdx.x; f < 3 * tj * NI; f += blockDim.x) {
-        const int t = f % tj, r = f / tj, i = r % NI, comp = r / NI;
-        const int c = rhs * 3 + comp;
-        b3[(((long long)c * d.ni + i) * NL2 + L) * NJ2 + J0 + t] = cconj(s[(comp * tj + t) * LS2 + i]);
+    fft_lds<NI2>(s, CB * tj, LS2, tw2);
+    for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
+        const int t = f % tj, r = f / tj, i = r % NI, cc = r / NI;
+        if (!act[cc / 3]) continue;
+        const int c = rhs0 * 3 + cc;
+        b3[(((long long)c * d.ni + i) * NL2 + L) * NJ2 + J0 + t] = cconj(s[(cc * tj + t) * LS2 + i]);
     }
 }
 

@@ -339,80 +339,117 @@ __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const 
 }
 
 // ---------------------------------------------------------------------------
-// Far field (dda_func.cpp:1576-1625) over a compact list of dipole cells.
-// Each block takes a chunk of dipoles and kFD directions; partial sums per
-// (chunk, direction) go to `partials` [nrhs][nchunk][ndirs][6].
+// Far field (dda_func.cpp:1576-1625) over a compact list of dipole cells:
+//   Esc(n) = S(n) - n (n . S(n)),   S(n) = sum_j exp(i n.(r_j - c)) P_j.
+// The transverse projection is linear, so it is applied once per direction
+// after the sum (k_far_field_sum).  The phase factorizes per axis,
+// exp(i n_x (i d - c_x)) exp(i n_y (j d - c_y)) exp(i n_z (l d - c_z)); each
+// block tabulates the three factors of its kFD directions in LDS (FP64
+// sincos rounded to float, arguments formed in float as the reference forms
+// its coordinates), so no transcendental is evaluated per dipole, and one
+// phase serves all R right-hand sides.
+// partials: [nrhs][nchunk][ndirs][6] (x, y, z as re, im)
 // ---------------------------------------------------------------------------
 
-constexpr int kFD = 4;
+constexpr int kFD = 4;            // directions per block
+constexpr int kFFAxis = 3 * 256;  // largest ni + nj + nl
 
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const int* __restrict__ cells, int ncells,
-                                                      const float* __restrict__ dirs, int ndirs,
+                                                      const float* __restrict__ dirs, int ndirs, int rhs0,
                                                       const float2* __restrict__ P, double* __restrict__ partials) {
-    __shared__ double red[6 * kFD * 16];
-    const int rhs = blockIdx.z;
-    const int d0 = blockIdx.y * kFD;
+    constexpr int NV = kFD * R * 6;
+    __shared__ float2 tab[kFD * kFFAxis];
+    __shared__ double red[NV * 16];
     const int nchunk = gridDim.x;
-    const long long rb = (long long)rhs * 3 * d.n;
-    float kx[kFD], ky[kFD], kz[kFD];
-#pragma unroll
-    for (int q = 0; q < kFD; ++q) {
+    const int d0 = blockIdx.y * kFD;
+    const int naxis = d.ni + d.nj + d.nl;
+    for (int f = threadIdx.x; f < kFD * naxis; f += blockDim.x) {
+        const int q = f / naxis, a = f - q * naxis;
         const int dd = min(d0 + q, ndirs - 1);
-        kx[q] = dirs[3 * dd];
-        ky[q] = dirs[3 * dd + 1];
-        kz[q] = dirs[3 * dd + 2];
+        float arg;
+        if (a < d.ni)
+            arg = dirs[3 * dd] * ((float)a * g.d0 - g.cx);
+        else if (a < d.ni + d.nj)
+            arg = dirs[3 * dd + 1] * ((float)(a - d.ni) * g.d0 - g.cy);
+        else
+            arg = dirs[3 * dd + 2] * ((float)(a - d.ni - d.nj) * g.d0 - g.cz);
+        tab[q * kFFAxis + a] = exp_i(arg);
     }
-    double acc[6 * kFD];
+    __syncthreads();
+    float2 acc[kFD][R][3];
 #pragma unroll
-    for (int q = 0; q < 6 * kFD; ++q) acc[q] = 0;
+    for (int q = 0; q < kFD; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[q][r][c] = make_float2(0.f, 0.f);
     const int per = (ncells + nchunk - 1) / nchunk;
     const int lo = blockIdx.x * per, hi = min(ncells, lo + per);
     for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
         const int k = cells[e];
         const int l = k % d.nl, ij = k / d.nl;
         const int j = ij % d.nj, i = ij / d.nj;
-        const float rx = (float)i * g.d0 - g.cx, ry = (float)j * g.d0 - g.cy, rz = (float)l * g.d0 - g.cz;
-        const float2 px = P[rb + k], py = P[rb + d.n + k], pz = P[rb + 2 * d.n + k];
+        float2 p[R][3];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p[r][c] = P[(long long)(rhs0 + r) * 3 * d.n + c * d.n + k];
 #pragma unroll
         for (int q = 0; q < kFD; ++q) {
-            const float r0 = kx[q] * rx + ky[q] * ry + kz[q] * rz;
-            const float2 c0 = exp_i(r0);
-            const float2 t = cadd(rmul(kz[q], pz), cadd(rmul(kx[q], px), rmul(ky[q], py)));
-            const float2 c5 = csub(px, rmul(kx[q], t)), c6 = csub(py, rmul(ky[q], t)), c7 = csub(pz, rmul(kz[q], t));
-            const float2 o0 = cmul(c0, c5), o1 = cmul(c0, c6), o2 = cmul(c0, c7);
-            acc[6 * q + 0] += o0.x;
-            acc[6 * q + 1] += o0.y;
-            acc[6 * q + 2] += o1.x;
-            acc[6 * q + 3] += o1.y;
-            acc[6 * q + 4] += o2.x;
-            acc[6 * q + 5] += o2.y;
+            const float2* t = tab + q * kFFAxis;
+            const float2 ph = cmul(cmul(t[i], t[d.ni + j]), t[d.ni + d.nj + l]);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[q][r][c] = cadd(acc[q][r][c], cmul(ph, p[r][c]));
         }
     }
-    block_sum<6 * kFD>(acc, red);
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < kFD; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                v[((q * R + r) * 3 + c) * 2] = acc[q][r][c].x;
+                v[((q * R + r) * 3 + c) * 2 + 1] = acc[q][r][c].y;
+            }
+    block_sum<NV>(v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < kFD; ++q) {
             if (d0 + q >= ndirs) break;
-            double* p = partials + (((long long)rhs * nchunk + blockIdx.x) * ndirs + d0 + q) * 6;
 #pragma unroll
-            for (int e = 0; e < 6; ++e) p[e] = acc[6 * q + e];
+            for (int r = 0; r < R; ++r) {
+                double* o = partials + (((long long)(rhs0 + r) * nchunk + blockIdx.x) * ndirs + d0 + q) * 6;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) o[e] = v[(q * R + r) * 6 + e];
+            }
         }
     }
 }
 
-// Sums far-field chunks: out[nrhs][ndirs][3] complex float.
+// Sums the chunks and applies the transverse projection:
+// out[nrhs][ndirs][3] complex float.
 __global__ __launch_bounds__(kBlock) void k_far_field_sum(const double* __restrict__ partials, int nchunk, int ndirs,
-                                                          int nrhs, float2* __restrict__ out) {
+                                                          int nrhs, const float* __restrict__ dirs,
+                                                          float2* __restrict__ out) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nrhs * ndirs * 3) return;
-    const int comp = idx % 3, dd = (idx / 3) % ndirs, rhs = idx / (3 * ndirs);
-    double re = 0, im = 0;
+    if (idx >= nrhs * ndirs) return;
+    const int dd = idx % ndirs, rhs = idx / ndirs;
+    double s[6] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < nchunk; ++c) {
-        const double* p = partials + (((long long)rhs * nchunk + c) * ndirs + dd) * 6 + 2 * comp;
-        re += p[0];
-        im += p[1];
+        const double* p = partials + (((long long)rhs * nchunk + c) * ndirs + dd) * 6;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) s[e] += p[e];
     }
-    out[idx] = make_float2((float)re, (float)im);
+    const double n[3] = {dirs[3 * dd], dirs[3 * dd + 1], dirs[3 * dd + 2]};
+    const double dr = n[0] * s[0] + n[1] * s[2] + n[2] * s[4];
+    const double di = n[0] * s[1] + n[1] * s[3] + n[2] * s[5];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        out[(long long)idx * 3 + c] = make_float2((float)(s[2 * c] - n[c] * dr), (float)(s[2 * c + 1] - n[c] * di));
 }
 
 // ---------------------------------------------------------------------------

@@ -22,6 +22,9 @@ EXPORTS = (
     "ddafft_last_error", "ddafft_version", "ddafft_create", "ddafft_destroy", "ddafft_get_ahat",
     "ddafft_set_target", "ddafft_set_directions", "ddafft_convolution", "ddafft_matvec", "ddafft_solve",
     "ddafft_get_dipoles", "ddafft_cabs_cext", "ddafft_far_field", "ddafft_profile", "ddafft_time_matvec",
+    "ddafft_gradient", "ddafft_polarizability", "ddafft_polarizability_deriv", "ddafft_target_ellipsoid",
+    "ddafft_target_parallelepiped", "ddafft_target_bisphere", "ddafft_geometry_fixed", "ddafft_muller",
+    "ddafft_muller_average",
 )
 
 _lib = None
@@ -55,6 +58,16 @@ def load(path=LIB_PATH):
     lib.ddafft_far_field.argtypes = [P, P, I, P, P]
     lib.ddafft_profile.argtypes = [P, I, P, P]
     lib.ddafft_time_matvec.argtypes = [P, I, I, P, P]
+    lib.ddafft_gradient.argtypes = [P, I, I, I, F, I, F, P, P, P, P, P, I, P, P, P, P, P, P, P, P, P, P, P]
+    lib.ddafft_polarizability.argtypes = [I, F, P, P, P, P]
+    lib.ddafft_polarizability_deriv.argtypes = [I, F, P, P, P, P]
+    D = C.c_double
+    lib.ddafft_target_ellipsoid.argtypes = [I, I, I, D, D, P]
+    lib.ddafft_target_parallelepiped.argtypes = [I, I, I, D, D, P]
+    lib.ddafft_target_bisphere.argtypes = [I, I, I, D, P]
+    lib.ddafft_geometry_fixed.argtypes = [I, I, D, D, D, D, D, P, P, P, P]
+    lib.ddafft_muller.argtypes = [P] * 8
+    lib.ddafft_muller_average.argtypes = [I, I, P, P, P, P, P, P, P]
     for name in EXPORTS:
         if name != "ddafft_last_error":
             getattr(lib, name).restype = I
@@ -164,6 +177,48 @@ class Context:
                             matvecs=int(info[r, 3])))
         return out
 
+    def gradient(self, solver, alpha, dalpha_dm, m, kinc, einc, weights, dirs=None, normals=None, max_step=20,
+                 err_max=1e-3, tau=0.0, ndirs=None):
+        """Forward + adjoint gradients (ddafft_gradient).
+
+        alpha, dalpha_dm: [npol, nmat] complex; m: [nmat] complex; einc: [npol, 3];
+        weights: [nobj, 2 + ndirs] rows (w_ext, w_abs, w_s11...); normals: [ndirs, 3].
+        Returns a dict: f [nobj], grad_m [nobj, nmat, 2], grad_chi [nobj, ni, nj, nl, 2],
+        esc [npol, ndirs, 3], cabs/cext [npol], s11 [ndirs], info [npol*(1+nobj), 4].
+        """
+        solver = SOLVERS.get(solver, solver)
+        alpha = np.ascontiguousarray(np.atleast_2d(alpha), np.complex64)
+        dadm = np.ascontiguousarray(np.atleast_2d(dalpha_dm), np.complex64)
+        einc = np.ascontiguousarray(np.atleast_2d(einc), np.complex64)
+        npol = einc.shape[0]
+        mm = np.ascontiguousarray(m, np.complex64)
+        kinc = np.ascontiguousarray(kinc, np.float32)
+        if dirs is not None:
+            dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+            nd = dirs.shape[0]
+        else:
+            nd = int(ndirs if ndirs is not None else getattr(self, "ndirs", 0))
+        if normals is not None:
+            normals = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(np.atleast_2d(weights), np.float32)
+        nobj = w.shape[0]
+        assert w.shape[1] == 2 + nd
+        f = np.zeros(nobj, np.float32)
+        esc = np.zeros((npol, max(nd, 1), 3), np.complex64)
+        cabs = np.zeros(npol, np.float32)
+        cext = np.zeros(npol, np.float32)
+        s11 = np.zeros(max(nd, 1), np.float32)
+        gm = np.zeros((nobj, self.nmat, 2), np.float32)
+        gchi = np.zeros((nobj,) + self.dims + (2,), np.float32)
+        info = np.zeros((npol * (1 + nobj), 4), np.int32)
+        want_s11 = normals is not None and npol == 2 and nd > 0
+        _check(_lib.ddafft_gradient(self._h, npol, nobj, int(solver), C.c_float(tau), int(max_step),
+                                    C.c_float(err_max), _p(alpha), _p(dadm), _p(mm), _p(kinc), _p(einc), nd,
+                                    _p(dirs), _p(normals), _p(w), _p(f), _p(esc), _p(cabs), _p(cext),
+                                    _p(s11) if want_s11 else None, _p(gm), _p(gchi), _p(info)))
+        return dict(f=f, grad_m=gm, grad_chi=gchi, esc=esc[:, :nd], cabs=cabs, cext=cext, s11=s11[:nd],
+                    info=info)
+
     def dipoles(self, rhs=0):
         p = np.zeros(self.dims + (3,), np.complex64)
         _check(_lib.ddafft_get_dipoles(self._h, int(rhs), _p(p)))
@@ -198,3 +253,59 @@ class Context:
         t = np.zeros(1, np.float32)
         _check(_lib.ddafft_time_matvec(self._h, int(nrhs), int(reps), _p(k), _p(t)))
         return k, float(t[0])
+
+
+# ----------------------------------------------------------------------
+# Host-side functions of the library (no GPU needed)
+# ----------------------------------------------------------------------
+
+def _einc_arg(einc):
+    if einc is None:
+        return None
+    e = np.asarray(einc, np.complex64).reshape(3)
+    return np.ascontiguousarray(np.stack([e.real, e.imag], 1).reshape(-1), np.float32)
+
+
+def polarizability(model, kd, m, kinc=None, einc=None):
+    """alpha of model 1..5 (float, as the reference rounds it)."""
+    load()
+    mm = np.asarray([np.real(m), np.imag(m)], np.float32)
+    k = None if kinc is None else np.ascontiguousarray(kinc, np.float32)
+    e = _einc_arg(einc)
+    out = np.zeros(2, np.float32)
+    _check(_lib.ddafft_polarizability(int(model), C.c_float(kd), _p(mm), _p(k), _p(e), _p(out)))
+    return np.complex64(out[0] + 1j * out[1])
+
+
+def polarizability_deriv(model, kd, m, kinc=None, einc=None):
+    """d alpha / d m of model 1..5."""
+    load()
+    mm = np.asarray([np.real(m), np.imag(m)], np.float32)
+    k = None if kinc is None else np.ascontiguousarray(kinc, np.float32)
+    e = _einc_arg(einc)
+    out = np.zeros(2, np.float32)
+    _check(_lib.ddafft_polarizability_deriv(int(model), C.c_float(kd), _p(mm), _p(k), _p(e), _p(out)))
+    return np.complex64(out[0] + 1j * out[1])
+
+
+def target_ellipsoid(dims, b_a=1.0, c_a=1.0):
+    load()
+    occ = np.zeros(tuple(dims), np.int8)
+    cnt = _lib.ddafft_target_ellipsoid(*[int(v) for v in dims], float(b_a), float(c_a), _p(occ))
+    if cnt < 0:
+        raise DdafftError("ellipsoid does not fit the lattice")
+    return occ, cnt
+
+
+def geometry_fixed(nplanes, nangles, theta_min, theta_step, psi=0.0, iota=0.0, kappa=0.0):
+    """Directions [nplanes, nangles, 3], plane normals [nplanes, 3], kinc [3],
+    einc [2, 3] complex of a fixed orientation."""
+    load()
+    dirs = np.zeros((nplanes, nangles, 3), np.float32)
+    normals = np.zeros((nplanes, 3), np.float32)
+    kinc = np.zeros(3, np.float32)
+    einc = np.zeros((2, 3, 2), np.float32)
+    _check(_lib.ddafft_geometry_fixed(int(nplanes), int(nangles), float(theta_min), float(theta_step), float(psi),
+                                      float(iota), float(kappa), _p(dirs), _p(normals), _p(kinc), _p(einc)))
+    return dirs, normals, kinc, (einc[..., 0] + 1j * einc[..., 1]).astype(np.complex64)
+

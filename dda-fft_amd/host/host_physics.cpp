@@ -5,7 +5,10 @@
 // alpha, directions and Mueller elements round as the reference's do.
 // Compiled with -ffp-contract=off.
 #include <cmath>
+#include <complex>
 #include <cstring>
+
+#include "host_internal.hpp"
 
 #include "../../include/ddafft.h"
 
@@ -180,6 +183,47 @@ int ddafft_polarizability(int model, float kd, const float* m, const float* kinc
     return DDAFFT_OK;
 }
 
+// Holomorphic derivative d alpha / d m of the five models, in double
+// (the formulas of dda_func.cpp:1631-1677 differentiated by hand).
+int ddafft_polarizability_deriv(int model, float kd, const float* m, const float* kinc, const float* einc,
+                                float* dalpha_dm) {
+    if (!m || !dalpha_dm || model < 1 || model > 5) return DDAFFT_EINVAL;
+    if (model == 5 && (!kinc || !einc)) return DDAFFT_EINVAL;
+    using C = std::complex<double>;
+    const double d = kd;
+    const C n(m[0], m[1]), e = n * n;
+    const double f = d * d * d * (0.75 / kPi);
+    C a = f * (e - 1.) / (e + 2.), da = f * 6. * n / ((e + 2.) * (e + 2.));
+    if (model == 3) {
+        const double b = -std::sqrt(4. * kPi / 3.) / d;
+        const C u = 1. + b * a;
+        da = da / (u * u);
+        a = a / u;
+    }
+    if (model == 2 || model == 3) {
+        const C u = 1. - C(0., 2. / 3.) * a;
+        da = da / (u * u);
+        a = a / u;
+    }
+    if (model == 4 || model == 5) {
+        double S = .2;
+        if (model == 5) {
+            const double rx = (double)kinc[0] * einc[0], ry = (double)kinc[1] * einc[2],
+                         rz = (double)kinc[2] * einc[4];
+            S = rx * rx + ry * ry + rz * rz;
+        }
+        const double b1 = -1.8915316, b2 = 0.1648469, b3 = -1.7700004;
+        const C t = b1 + b2 * e + S * b3 * e - C(0., (2. / 3.) * d);
+        const C dt = (b2 + S * b3) * 2. * n;
+        const C u = 1. + (a / d) * t;
+        const C du = (da / d) * t + (a / d) * dt;
+        da = (da * u - a * du) / (u * u);
+    }
+    dalpha_dm[0] = (float)da.real();
+    dalpha_dm[1] = (float)da.imag();
+    return DDAFFT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Scattering geometry for a fixed orientation (main.cpp:354-387, 516-542):
 // planes rotated about z by m*180/nplanes, phase angles theta_min + m*step,
@@ -338,3 +382,50 @@ int ddafft_muller_average(int nplanes, int nangles, const float* kinc, const flo
 }
 
 }  // extern "C"
+
+namespace ddafft_host {
+
+// M00 of one direction (mmuller.cpp:175-314, evaluated in double) and its
+// Wirtinger derivatives g0 = dM00/dEsc_0, g1 = dM00/dEsc_1 (3 complex each).
+// M00 = (|J00|^2 + |J01|^2 + |J10|^2 + |J11|^2) / 2 with every J linear in the
+// projections of Esc_0 and Esc_1 on the plane normal and on psc.
+double m00_grad(const float* kinc, const float* ksc, const float* normal, const float* einc0, const float* einc1,
+                const float* esc0, const float* esc1, std::complex<double>* g0, std::complex<double>* g1) {
+    using C = std::complex<double>;
+    const double k[3] = {kinc[0], kinc[1], kinc[2]}, s[3] = {ksc[0], ksc[1], ksc[2]},
+                 nv[3] = {normal[0], normal[1], normal[2]};
+    auto cr = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double pin[3], psc[3];
+    cr(k, nv, pin);
+    cr(s, nv, psc);
+    const double n0 = std::sqrt(pin[0] * pin[0] + pin[1] * pin[1] + pin[2] * pin[2]);
+    const double n1 = std::sqrt(psc[0] * psc[0] + psc[1] * psc[1] + psc[2] * psc[2]);
+    for (int q = 0; q < 3; ++q) {
+        pin[q] /= n0;
+        psc[q] = -psc[q] / n1;
+    }
+    auto dot = [](const double* v, const float* e) {
+        return v[0] * C(e[0], e[1]) + v[1] * C(e[2], e[3]) + v[2] * C(e[4], e[5]);
+    };
+    const C cc0 = dot(nv, einc0), cc1 = dot(nv, einc1), cc2 = dot(pin, einc0), cc3 = dot(pin, einc1);
+    const C den = C(0., 1.) * (cc0 * cc3 - cc1 * cc2);
+    const C cc4 = dot(nv, esc0), cc5 = dot(nv, esc1), cc6 = dot(psc, esc0), cc7 = dot(psc, esc1);
+    const C J00 = (cc7 * cc0 - cc6 * cc1) / den, J01 = (cc6 * cc3 - cc7 * cc2) / den;
+    const C J10 = (cc5 * cc0 - cc4 * cc1) / den, J11 = (cc4 * cc3 - cc5 * cc2) / den;
+    const double m00 = .5 * (std::norm(J00) + std::norm(J01) + std::norm(J10) + std::norm(J11));
+    const C a0 = .5 * (std::conj(J00) * -cc1 + std::conj(J01) * cc3) / den;   // psc part of dM00/dEsc_0
+    const C b0 = .5 * (std::conj(J10) * -cc1 + std::conj(J11) * cc3) / den;   // normal part
+    const C a1 = .5 * (std::conj(J00) * cc0 - std::conj(J01) * cc2) / den;
+    const C b1 = .5 * (std::conj(J10) * cc0 - std::conj(J11) * cc2) / den;
+    for (int q = 0; q < 3; ++q) {
+        g0[q] = a0 * psc[q] + b0 * nv[q];
+        g1[q] = a1 * psc[q] + b1 * nv[q];
+    }
+    return m00;
+}
+
+}  // namespace ddafft_host

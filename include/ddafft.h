@@ -102,6 +102,31 @@ int ddafft_cabs_cext(ddafft_ctx* ctx, const float* alpha, const float* kinc, con
  * (FarField, dda_func.cpp:1576-1625). */
 int ddafft_far_field(ddafft_ctx* ctx, const float* p, int ndirs, const float* dirs, float* esc);
 
+/* Forward + adjoint gradients (no counterpart in the reference, whose forward
+ * model DDA_FFT, dda_func.cpp:1075-1570, CabsCext :1679-1727, FarField
+ * :1576-1625 and BuildMatrixJones/JonesToMuller, mmuller.cpp:175-314, they
+ * differentiate).  Solves the npol (1 or 2) polarizations as ddafft_solve
+ * does, keeps their dipoles, then solves the adjoint systems of nobj
+ * objectives as npol*nobj more right-hand sides of the same operator
+ * (K = diag(alpha)^-1 - A is complex symmetric) and assembles
+ *   f[o] = w_ext * mean_p Cext_p + w_abs * mean_p Cabs_p + sum_a w_a * S11(a)
+ * with weights[o] = {w_ext, w_abs, w_s11[ndirs]} and S11(a) = M00 of
+ * direction a (before plane averaging).  Outputs:
+ *   grad_m   [nobj][nmat][2]        df/dRe m, df/dIm m per material
+ *   grad_chi [nobj][ni][nj][nl][2]  df/dRe chi, df/dIm chi per dipole,
+ *                                    chi = m^2 - 1 (0 on empty cells)
+ *   f [nobj], s11 [ndirs], esc/cabs/cext of the forward solve,
+ *   info [npol + npol*nobj][4] (forward, then adjoint right-hand sides).
+ * alpha and dalpha_dm are [npol][nmat][2] (ddafft_polarizability and
+ * ddafft_polarizability_deriv per polarization), m is [nmat][2];
+ * normals [ndirs][3] is the scattering-plane normal of each direction
+ * (needed for S11).  npol * nobj <= nrhs_max.  Any output may be NULL. */
+int ddafft_gradient(ddafft_ctx* ctx, int npol, int nobj, int solver, float tau, int max_step, float err_max,
+                    const float* alpha, const float* dalpha_dm, const float* m, const float* kinc,
+                    const float* einc, int ndirs, const float* dirs, const float* normals, const float* weights,
+                    float* f, float* esc, float* cabs, float* cext, float* s11, float* grad_m, float* grad_chi,
+                    int* info);
+
 /* Opt-in timing of every matvec pass with HIP events on the context's
  * stream.  Returns, in pass_ms[5] and *matvecs, the summed durations of the
  * five passes (z-fwd, y-fwd, fused x, y-inv, z-inv) and the number of matvecs
@@ -134,6 +159,12 @@ int ddafft_target_bisphere(int ni, int nj, int nl, double r1_r2, signed char* oc
  * selected by main.cpp:606-631 and 926-931); kinc[3] and einc[3][2] are used
  * by model 5 only. */
 int ddafft_polarizability(int model, float kd, const float* m, const float* kinc, const float* einc, float* alpha);
+
+/* Holomorphic derivative d alpha / d m [2] of the same models, evaluated in
+ * double (the formulas of dda_func.cpp:1631-1677 differentiated); for the
+ * gradient entry point. */
+int ddafft_polarizability_deriv(int model, float kd, const float* m, const float* kinc, const float* einc,
+                                float* dalpha_dm);
 
 /* Scattering planes, directions and incident field for a fixed orientation
  * (main.cpp:354-387 and 516-542): dirs [nplanes][nangles][3],

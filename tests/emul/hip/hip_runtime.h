@@ -8,12 +8,15 @@
 // (dda-fft_amd/ddafft.py) never loads this library.
 //
 // Execution model: a kernel launch runs its blocks one after another; the
-// threads of one block run as real host threads that meet at __syncthreads()
-// through a std::barrier, so `__shared__` arrays can simply be static.
+// threads of one block run as fibers (ucontext) on the calling host thread and
+// meet at __syncthreads(), so `__shared__` arrays can simply be static.
 #pragma once
 
+#include <setjmp.h>
+#include <ucontext.h>
+
 #include <algorithm>
-#include <barrier>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -25,6 +28,11 @@
 #include <mutex>
 #include <thread>
 #include <vector>
+
+#if defined(__SANITIZE_ADDRESS__)
+#include <sanitizer/common_interface_defs.h>
+#define EMUL_ASAN 1
+#endif
 
 #define __global__
 #define __device__
@@ -49,7 +57,7 @@ struct uint3e {
     unsigned x, y, z;
 };
 
-inline thread_local uint3e threadIdx;
+inline uint3e threadIdx;
 inline uint3e blockIdx;
 inline dim3 blockDim;
 inline dim3 gridDim;
@@ -59,45 +67,73 @@ using std::min;
 
 namespace emul {
 
-inline std::barrier<>* g_bar = nullptr;
-inline double g_shfl[1024];
+// Each block runs its threads as fibers on the calling host thread: a fiber
+// runs until it reaches __syncthreads() (or returns), then the next one runs;
+// when every fiber waits at the barrier they are all released.  A block whose
+// threads disagree about a barrier (some wait, some returned) aborts.
+constexpr size_t kStack = 512 * 1024;
+constexpr int kMaxThreads = 1024;
 
-struct Pool {
-    int n;
-    std::vector<std::thread> th;
-    std::barrier<> start, done, block;
-    std::function<void(int)> task;
-    bool quit = false;
-    explicit Pool(int n_) : n(n_), start(n_ + 1), done(n_ + 1), block(n_) {
-        for (int i = 0; i < n; ++i)
-            th.emplace_back([this, i] {
-                for (;;) {
-                    start.arrive_and_wait();
-                    if (quit) return;
-                    task(i);
-                    done.arrive_and_wait();
-                }
-            });
-    }
-    void run(std::function<void(int)> f) {
-        task = std::move(f);
-        start.arrive_and_wait();
-        done.arrive_and_wait();
-    }
-    ~Pool() {
-        quit = true;
-        start.arrive_and_wait();
-        for (auto& t : th) t.join();
-    }
+struct Fiber {
+    ucontext_t ctx;     // used once, to enter the fiber
+    jmp_buf jb;         // later switches: _setjmp/_longjmp (no signal-mask syscall)
+    char* stack = nullptr;
+    int state = 0;      // 0 runnable, 1 waiting at the barrier, 2 returned
+    bool started = false;
 };
 
-inline Pool* pool_for(int n) {
-    static std::mutex m;
-    static std::map<int, std::unique_ptr<Pool>> pools;
-    std::lock_guard<std::mutex> g(m);
-    auto& p = pools[n];
-    if (!p) p = std::make_unique<Pool>(n);
-    return p.get();
+inline ucontext_t g_main;
+inline jmp_buf g_main_jb;
+inline Fiber g_fib[kMaxThreads];
+inline int g_cur = -1;
+inline std::function<void()>* g_body = nullptr;
+inline const void* g_main_bottom = nullptr;
+inline size_t g_main_size = 0;
+inline double g_shfl[kMaxThreads];
+
+inline void to_main() {
+    Fiber& f = g_fib[g_cur];
+    if (!_setjmp(f.jb)) {
+#ifdef EMUL_ASAN
+        __sanitizer_start_switch_fiber(nullptr, g_main_bottom, g_main_size);
+#endif
+        _longjmp(g_main_jb, 1);
+    }
+#ifdef EMUL_ASAN
+    __sanitizer_finish_switch_fiber(nullptr, nullptr, nullptr);
+#endif
+}
+
+inline void trampoline(int t) {
+#ifdef EMUL_ASAN
+    __sanitizer_finish_switch_fiber(nullptr, &g_main_bottom, &g_main_size);
+#endif
+    (*g_body)();
+    g_fib[t].state = 2;
+#ifdef EMUL_ASAN
+    __sanitizer_start_switch_fiber(nullptr, g_main_bottom, g_main_size);
+#endif
+    _longjmp(g_main_jb, 1);
+}
+
+inline void resume(int t) {
+    g_cur = t;
+    Fiber& f = g_fib[t];
+#ifdef EMUL_ASAN
+    void* fake = nullptr;
+    __sanitizer_start_switch_fiber(&fake, f.stack, kStack);
+#endif
+    if (!_setjmp(g_main_jb)) {
+        if (!f.started) {
+            f.started = true;
+            swapcontext(&g_main, &f.ctx);
+        } else {
+            _longjmp(f.jb, 1);
+        }
+    }
+#ifdef EMUL_ASAN
+    __sanitizer_finish_switch_fiber(fake, nullptr, nullptr);
+#endif
 }
 
 template <class F, class... A>
@@ -105,24 +141,54 @@ void launch(F f, dim3 g, dim3 b, A... args) {
     static std::mutex serial;   // one kernel at a time, blocks in order
     std::lock_guard<std::mutex> lk(serial);
     const int nt = (int)(b.x * b.y * b.z);
-    Pool* p = pool_for(nt);
+    if (nt > kMaxThreads) std::abort();
     gridDim = g;
     blockDim = b;
+    std::function<void()> body = [&]() { f(args...); };
+    g_body = &body;
     for (unsigned bz = 0; bz < g.z; ++bz)
         for (unsigned by = 0; by < g.y; ++by)
             for (unsigned bx = 0; bx < g.x; ++bx) {
                 blockIdx = uint3e{bx, by, bz};
-                g_bar = &p->block;
-                p->run([&](int t) {
-                    threadIdx = uint3e{(unsigned)(t % b.x), (unsigned)((t / b.x) % b.y), (unsigned)(t / (b.x * b.y))};
-                    f(args...);
-                });
+                for (int t = 0; t < nt; ++t) {
+                    Fiber& fb = g_fib[t];
+                    if (!fb.stack) fb.stack = static_cast<char*>(std::malloc(kStack));
+                    getcontext(&fb.ctx);
+                    fb.ctx.uc_stack.ss_sp = fb.stack;
+                    fb.ctx.uc_stack.ss_size = kStack;
+                    fb.ctx.uc_link = nullptr;
+                    makecontext(&fb.ctx, (void (*)())trampoline, 1, t);
+                    fb.state = 0;
+                    fb.started = false;
+                }
+                for (;;) {
+                    for (int t = 0; t < nt; ++t)
+                        if (g_fib[t].state == 0) {
+                            threadIdx = uint3e{(unsigned)(t % b.x), (unsigned)((t / b.x) % b.y),
+                                               (unsigned)(t / (b.x * b.y))};
+                            resume(t);
+                        }
+                    int waiting = 0, done = 0;
+                    for (int t = 0; t < nt; ++t) {
+                        waiting += g_fib[t].state == 1;
+                        done += g_fib[t].state == 2;
+                    }
+                    if (done == nt) break;
+                    if (waiting != nt) {
+                        std::fprintf(stderr, "emul: divergent __syncthreads in block (%u,%u,%u)\n", bx, by, bz);
+                        std::abort();
+                    }
+                    for (int t = 0; t < nt; ++t) g_fib[t].state = 0;
+                }
             }
 }
 
 }  // namespace emul
 
-inline void __syncthreads() { emul::g_bar->arrive_and_wait(); }
+inline void __syncthreads() {
+    emul::g_fib[emul::g_cur].state = 1;
+    emul::to_main();
+}
 
 inline double __shfl_xor(double v, int off, int width = 64) {
     (void)width;

@@ -87,8 +87,41 @@ def main():
                         "cabs_cext")
             k, t = ctx.time_matvec(2, 1)
             ok &= check(t >= 0, "time_matvec")
+    if not quick:
+        ok &= check_gradient()
     print("ALL OK" if ok else "SOME CHECKS FAILED", flush=True)
     return 0 if ok else 1
+
+
+def check_gradient():
+    """ddafft_gradient (forward, adjoint sources, adjoint solve, assembly) on a
+    small two-material target against the FP64 CPU reference gradient."""
+    import gradient as G
+    o64 = oracle.Oracle("f64")
+    dims, kd = (4, 4, 8), 0.5
+    occ, _ = o64.target_ellipsoid(dims, 1.0, 2.0)
+    occ = occ.copy()
+    occ[0, 1, 3] = occ[2, 2, 4] = 2
+    mv, m = [0, 1, 1], [1, 1.5 + 0.001j, 1.7 + 0.1j]
+    th = np.arange(0, 181, 45.0) * np.pi / 180
+    dirs = np.stack([np.sin(th), 0 * th, np.cos(th)], 1).astype(np.float32)
+    normals = np.tile([0, 1.0, 0], (len(th), 1)).astype(np.float32)
+    W = np.zeros((2, 2 + len(th)))
+    W[0, 0] = 1.0
+    W[1] = np.linspace(-1, 1, 2 + len(th))
+    kinc = np.array([0, 0, 1.0], np.float32)
+    einc = np.array([[1, 0, 0], [0, 1, 0]], np.complex64)
+    alpha = np.array([[0] + [ddafft.polarizability(1, kd, mm) for mm in m[1:]]] * 2)
+    dadm = np.array([[0] + [ddafft.polarizability_deriv(1, kd, mm) for mm in m[1:]]] * 2)
+    with ddafft.Context(dims, kd, nrhs_max=4) as ctx:
+        ctx.set_target(occ, mv)
+        res = ctx.gradient("bicgstab", alpha, dadm, m, kinc, einc, W, dirs=dirs, normals=normals, max_step=100,
+                           err_max=1e-8)
+    ref = G.gradient(o64, 2, occ, mv, m, 1, kd, kinc.astype(np.float64), einc.astype(complex),
+                     o64.build_ahat(dims, kd), dirs.astype(np.float64), normals, W, max_step=300, err_max=1e-14)
+    em = np.abs(res["grad_m"] - ref["grad_m"]).max() / np.abs(ref["grad_m"]).max()
+    ec = np.abs(res["grad_chi"] - ref["grad_chi"]).max() / np.abs(ref["grad_chi"]).max()
+    return check(em < 1e-4 and ec < 1e-4, "gradient grad_m %.1e grad_chi %.1e" % (em, ec))
 
 
 if __name__ == "__main__":
