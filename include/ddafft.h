@@ -154,6 +154,51 @@ int ddafft_target_ellipsoid(int ni, int nj, int nl, double b_a, double c_a, sign
 int ddafft_target_parallelepiped(int ni, int nj, int nl, double b_a, double c_a, signed char* occ);
 int ddafft_target_bisphere(int ni, int nj, int nl, double r1_r2, signed char* occ);
 
+/* The reference's random stream (xorshift64*, randomize.cpp:44-58): a state
+ * of 0 is replaced by 0x9E3779B97F4A7C15 as dda_srand does.  A worker's seed
+ * is base + 0x9E3779B97F4A7C15 * (worker + 1) (main.cpp:803-804). */
+unsigned long long ddafft_rng_worker_seed(unsigned long long base_seed, int worker);
+int ddafft_rng_next(unsigned long long* state);
+
+/* Random and parametric targets with the reference's semantics and float
+ * rounding; *state is the random stream, advanced as the reference advances
+ * it.  Each returns what the reference's generator returns (its dipole count
+ * convention, -1 if the body does not fit), or -2 on bad arguments.
+ *   random ellipsoid      iGenerateRandomEllipsoid (particle.cpp:531-613), targets 4 and 5
+ *   random parallelepiped iGenerateRandomParallelepiped (:616-694), target 9
+ *   nibbled ellipsoid     iGenerateNibbleEllipsoid (:696-822), target 10
+ *   hierarchical          iGenerateHierarchicalEllipsoid (:824-962), target 13
+ *   partial sphere        iGenerateSphereSpec (:965-994), target 14 (radius = ni / 2)
+ *   worms, kind 6/7/8     iGenerateAnthrax / Hedgehog / DeadAnthrax (:21-231); with
+ *                         max_attempts > 0 the placement gives up (-3) after that
+ *                         many draws, where the reference can loop forever. */
+int ddafft_target_random_ellipsoid(int ni, int nj, int nl, double b_a, double c_a, int nsorts, const int* nseeds,
+                                   unsigned long long* state, signed char* occ);
+int ddafft_target_random_parallelepiped(int ni, int nj, int nl, double b_a, double c_a, int nsorts,
+                                        const int* nseeds, unsigned long long* state, signed char* occ);
+int ddafft_target_nibble_ellipsoid(int ni, int nj, int nl, double b_a, double c_a, int nsorts, const int* nseeds,
+                                   int bites, double depth, const int* mv, unsigned long long* state,
+                                   signed char* occ);
+int ddafft_target_hierarchical_ellipsoid(int levels, int ni, int nj, int nl, double b_a, double c_a, int nsorts,
+                                         const int* nseeds, unsigned long long* state, signed char* occ);
+int ddafft_target_sphere_spec(int ni, int nj, int nl, double radius, double rho, signed char* occ);
+int ddafft_worm_count(int ni, int nj, int nl, double rho, double radius_worm, double length_ratio);
+int ddafft_target_worms(int kind, int ni, int nj, int nl, double radius_worm, double length_worm, int nworms,
+                        long long max_attempts, unsigned long long* state, signed char* occ);
+
+/* Target 11 shape file (main.cpp:645-677): the count N, then N lines
+ * "X Y Z T".  read returns N, -3 for a cell outside the lattice (the
+ * reference's exit code 3), -4 for a short file; write lists the nonzero cells. */
+int ddafft_target_file_read(const char* path, int ni, int nj, int nl, signed char* occ);
+int ddafft_target_file_write(const char* path, int ni, int nj, int nl, const signed char* occ);
+
+/* Monte-Carlo orientation (main.cpp:875-893): draws the Euler angles from
+ * *state and returns the 3x3 matrix rot[9] (row-major) that main.cpp applies
+ * to kinc, Einc, the directions and the plane normals; ddafft_rotate applies
+ * it to n vectors as main.cpp:895-921 does. */
+int ddafft_orientation_mc(unsigned long long* state, float* rot);
+int ddafft_rotate(const float* rot, int n, const float* in, float* out);
+
 /* Polarizability model 1..5 (CM, CMRR, CMNR, ILDR, LDR) of refractive index
  * m[2] at lattice spacing kd (Polarizability_*, dda_func.cpp:1631-1677, as
  * selected by main.cpp:606-631 and 926-931); kinc[3] and einc[3][2] are used

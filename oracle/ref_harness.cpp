@@ -286,4 +286,42 @@ int ref_target(int kind, int ni, int nj, int nl, float p1, float p2, signed char
     return -1;
 }
 
+// Random and parametric targets (particle.cpp:21-995) after dda_srand(*state);
+// *state is left where the reference's stream stands afterwards, as the
+// next dda_rand() would continue from it.  kind: 5 random ellipsoid,
+// 9 random parallelepiped, 10 nibble, 13 hierarchical, 14 sphere spec,
+// 6/7/8 worms.
+int ref_target_rand(int kind, int ni, int nj, int nl, float b_a, float c_a, int nsorts, int* nseeds, int bites,
+                    float depth, int* mv, int levels, float rho, float radius_worm, float length_worm, int nworms,
+                    unsigned long long seed, signed char* occ) {
+    view3<char> o((char*)occ, ni, nj, nl);
+    dda_srand(seed);
+    switch (kind) {
+        case 5: return iGenerateRandomEllipsoid(ni, nj, nl, b_a, c_a, nsorts, nseeds, o.p);
+        case 9: return iGenerateRandomParallelepiped(ni, nj, nl, b_a, c_a, nsorts, nseeds, o.p);
+        case 10: return iGenerateNibbleEllipsoid(ni, nj, nl, b_a, c_a, nsorts, nseeds, bites, depth, mv, o.p);
+        case 13: return iGenerateHierarchicalEllipsoid(levels, ni, nj, nl, b_a, c_a, nsorts, nseeds, o.p);
+        case 14: return iGenerateSphereSpec(ni, nj, nl, ni / 2., rho, o.p);
+        case 6:
+        case 7:
+        case 8: {
+            std::vector<cylinder> w((size_t)nworms);
+            if (kind == 6) return iGenerateAnthrax(ni, nj, nl, radius_worm, length_worm, nworms, w.data(), o.p);
+            if (kind == 7) return iGenerateHedgehog(ni, nj, nl, radius_worm, length_worm, nworms, w.data(), o.p);
+            return iGenerateDeadAnthrax(ni, nj, nl, radius_worm, length_worm, nworms, w.data(), o.p);
+        }
+    }
+    return -100;
+}
+
+// The reference's next draw from its current stream (after ref_target_rand).
+int ref_rand_next(void) { return dda_rand(); }
+
+// The next value of the reference's stream after dda_srand(seed) and `skip` draws.
+int ref_rand_after(unsigned long long seed, int skip) {
+    dda_srand(seed);
+    for (int k = 0; k < skip; ++k) dda_rand();
+    return dda_rand();
+}
+
 }  // extern "C"
