@@ -145,13 +145,15 @@ struct ddafft_ctx {
     float2 *X = nullptr, *Y = nullptr;
     float2* alpha = nullptr;           // [nrhs][nmat]
     float2* einc = nullptr;            // [nrhs][3]
+    float* kinc = nullptr;             // [nrhs][3] incident direction per right-hand side
     SolveState* st = nullptr;          // [nrhs]
     double* hist = nullptr;
     int hist_cap = 0;
     double* partials = nullptr;
     size_t partials_cap = 0;
     float* dirs = nullptr;
-    int ndirs = 0, dirs_cap = 0;
+    int ndirs = 0, dirs_cap = 0;       // resident directions (dirs_cap in directions)
+    std::vector<char> dir_pairs;       // per-RHS directions: do RHS 2q and 2q+1 share them?
     // gradients: forward dipoles kept for the adjoint, per-axis phase table,
     // small per-call arrays
     float2* Pf = nullptr;
@@ -520,7 +522,7 @@ int ddafft_destroy(ddafft_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->ahat, c->tables, c->occ, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
                     c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
-                    c->dirs, c->esc, c->stop_dev, c->Pf, c->phase, c->gaux};
+                    c->dirs, c->esc, c->stop_dev, c->Pf, c->phase, c->gaux, c->kinc};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->stop_host) hipHostFree(c->stop_host);
@@ -572,6 +574,7 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
     for (float2** f : fields)
         if (!rc) rc = dev_alloc(f, (size_t)(n * C));
     if (!rc) rc = dev_alloc(&c->einc, (size_t)3 * nrhs_max);
+    if (!rc) rc = dev_alloc(&c->kinc, (size_t)3 * nrhs_max);
     if (!rc) rc = dev_alloc(&c->st, (size_t)nrhs_max);
     if (!rc) rc = dev_alloc(&c->occ, (size_t)n);
     if (!rc) rc = dev_alloc(&c->stop_dev, 1);
@@ -866,11 +869,14 @@ __global__ void k_stop_max(const SolveState* st, int nrhs, int* out) {
     *out = m;
 }
 
-int observables(ddafft_ctx* c, int nrhs, const float* kinc, const float* einc, int ndirs, const float* ddirs,
+// Cross sections and far fields of the nrhs solutions in c->P; c->kinc and
+// c->einc hold each right-hand side's incidence.  Right-hand side r uses the
+// directions ddirs + r * dir_stride.
+int observables(ddafft_ctx* c, int nrhs, const float* einc, int ndirs, const float* ddirs, long long dir_stride,
                 float* esc, float* cabs, float* cext) {
     Dims d = c->d;
     d.nrhs = nrhs;
-    const Geometry g = make_geometry(c, kinc);
+    const Geometry g = make_geometry(c, nullptr);
     // cross sections
     const int cb = (int)std::min<long long>((d.n + kBlock - 1) / kBlock, 512);
     // dipole chunks per direction tile: enough blocks to fill the chip
@@ -879,7 +885,8 @@ int observables(ddafft_ctx* c, int nrhs, const float* kinc, const float* einc, i
                                                               (2048 + dtiles - 1) / std::max(dtiles, 1)})) : 0;
     TRY(ensure_partials(c, std::max<size_t>((size_t)nrhs * cb * 2, (size_t)nrhs * fchunk * std::max(ndirs, 1) * 6)));
     dim3 gc(cb, nrhs);
-    hipLaunchKernelGGL(k_cabs_cext, gc, kBlock, 0, c->stream, d, g, c->occ, c->alpha, c->nmat, c->einc, c->P, c->partials);
+    hipLaunchKernelGGL(k_cabs_cext, gc, kBlock, 0, c->stream, d, g, c->kinc, c->occ, c->alpha, c->nmat, c->einc, c->P,
+                       c->partials);
     HIPCHECK(hipGetLastError());
     std::vector<double> part((size_t)nrhs * cb * 2);
     HIPCHECK(hipMemcpyAsync(part.data(), c->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -894,19 +901,22 @@ int observables(ddafft_ctx* c, int nrhs, const float* kinc, const float* einc, i
         }
         // partials for cabs were copied above (stream-ordered), so reuse is safe
         const int tot = nrhs * ndirs * 3;
-        for (int r0 = 0; r0 < nrhs; r0 += 2) {
-            const int rr = std::min(2, nrhs - r0);
+        // right-hand sides in pairs that share one direction set share the phases
+        for (int r0 = 0; r0 < nrhs;) {
+            const bool pair = r0 + 1 < nrhs && (dir_stride == 0 || c->dir_pairs[r0 / 2]);
             dim3 gf(fchunk, (ndirs + kFD - 1) / kFD);
-            if (rr == 2)
-                hipLaunchKernelGGL((k_far_field<2>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, ddirs, ndirs,
+            const float* dd = ddirs + r0 * dir_stride;
+            if (pair)
+                hipLaunchKernelGGL((k_far_field<2>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, dd, ndirs,
                                    r0, c->P, c->partials);
             else
-                hipLaunchKernelGGL((k_far_field<1>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, ddirs, ndirs,
+                hipLaunchKernelGGL((k_far_field<1>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, dd, ndirs,
                                    r0, c->P, c->partials);
             HIPCHECK(hipGetLastError());
+            r0 += pair ? 2 : 1;
         }
         hipLaunchKernelGGL(k_far_field_sum, (nrhs * ndirs + kBlock - 1) / kBlock, kBlock, 0, c->stream, c->partials,
-                           fchunk, ndirs, nrhs, ddirs, c->esc);
+                           fchunk, ndirs, nrhs, ddirs, dir_stride, c->esc);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(esc, c->esc, (size_t)tot * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
     }
@@ -928,53 +938,80 @@ int observables(ddafft_ctx* c, int nrhs, const float* kinc, const float* einc, i
 
 namespace {
 
-// The solver iteration for nrhs right-hand sides whose sources (P_0) are
-// already in c->P and whose polarizabilities are in c->alpha: step 0, the
-// pipelined iterations, and the best-iterate restore.  Fills the final solver
-// states, and err_hist / info when given.
-int solve_core(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, double* err_hist,
-               int* info, std::vector<SolveState>& sf) {
+// One context's solve in flight.  The sources (P_0) are in c->P and the
+// polarizabilities in c->alpha when it begins; then the host enqueues one
+// iteration at a time, keeping one iteration queued ahead of the stopping
+// decision the device makes for each right-hand side.
+struct Run {
+    ddafft_ctx* c = nullptr;
+    int nrhs = 0, solver = 0, max_step = 0;
+    float tau = 0.f, err_max = 0.f;
+    Seq q{};
+    int o = 1;            // next iteration to enqueue
+    int known_stop = 0;   // first iteration known not to be needed
+};
+
+int run_begin(Run& r) {
+    ddafft_ctx* c = r.c;
     Dims d = c->d;
-    d.nrhs = nrhs;
-    if (max_step + 1 > c->hist_cap || c->hist == nullptr) {
+    d.nrhs = r.nrhs;
+    if (r.max_step + 1 > c->hist_cap || c->hist == nullptr) {
         HIPCHECK(hipStreamSynchronize(c->stream));
         if (c->hist) hipFree(c->hist);
         c->hist = nullptr;
-        TRY(dev_alloc(&c->hist, (size_t)c->nrhs_max * (max_step + 1)));
-        c->hist_cap = max_step + 1;
+        TRY(dev_alloc(&c->hist, (size_t)c->nrhs_max * (r.max_step + 1)));
+        c->hist_cap = r.max_step + 1;
     }
-    std::vector<SolveState> s0(nrhs);
+    std::vector<SolveState> s0(r.nrhs);
     for (auto& s : s0) {
         memset(&s, 0, sizeof(s));
-        s.stop_at = max_step + 1;
+        s.stop_at = r.max_step + 1;
     }
-    TRY(upload(c, c->st, s0.data(), nrhs * sizeof(SolveState)));
-    TRY(ensure_partials(c, (size_t)nrhs * std::max(zi_blocks(d), update_blocks(d)) * kSlots));
-
-    Seq q{c, nrhs, FinArgs{}};
-    q.fa.hist = c->hist;
-    q.fa.max_step = max_step;
-    q.fa.err_max = (double)err_max;
-    q.fa.inv_ndip = 1.0 / (double)c->ndip;
-    TRY(step0(q, solver));
-    hipLaunchKernelGGL(k_stop_max, 1, 1, 0, c->stream, c->st, nrhs, c->stop_dev);
+    TRY(upload(c, c->st, s0.data(), r.nrhs * sizeof(SolveState)));
+    TRY(ensure_partials(c, (size_t)r.nrhs * std::max(zi_blocks(d), update_blocks(d)) * kSlots));
+    r.q = Seq{c, r.nrhs, FinArgs{}};
+    r.q.fa.hist = c->hist;
+    r.q.fa.max_step = r.max_step;
+    r.q.fa.err_max = (double)r.err_max;
+    r.q.fa.inv_ndip = 1.0 / (double)c->ndip;
+    TRY(step0(r.q, r.solver));
+    hipLaunchKernelGGL(k_stop_max, 1, 1, 0, c->stream, c->st, r.nrhs, c->stop_dev);
     HIPCHECK(hipMemcpyAsync(&c->stop_host[0], c->stop_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+    r.o = 1;
+    r.known_stop = r.max_step + 1;
+    return DDAFFT_OK;
+}
 
-    // Iterations: the device decides when each right-hand side stops; the
-    // host keeps at most two iterations in flight and stops enqueuing once
-    // it has seen every right-hand side stop.
-    int known_stop = max_step + 1;
-    for (int o = 1; o <= max_step; ++o) {
-        if (o >= known_stop) break;
-        TRY(iteration(q, solver, o, tau));
-        hipLaunchKernelGGL(k_stop_max, 1, 1, 0, c->stream, c->st, nrhs, c->stop_dev);
-        HIPCHECK(hipMemcpyAsync(&c->stop_host[o & 3], c->stop_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipEventRecord(c->ev[o & 3], c->stream));
-        // wait for the flag of step o-1 (keeps one iteration queued ahead)
-        HIPCHECK(hipEventSynchronize(c->ev[(o - 1) & 3]));
-        known_stop = std::min(known_stop, c->stop_host[(o - 1) & 3]);
-    }
+bool run_more(const Run& r) { return r.o <= r.max_step && r.o < r.known_stop; }
+
+// enqueues iteration r.o and its stop flag
+int run_enqueue(Run& r) {
+    ddafft_ctx* c = r.c;
+    const int o = r.o;
+    TRY(iteration(r.q, r.solver, o, r.tau));
+    hipLaunchKernelGGL(k_stop_max, 1, 1, 0, c->stream, c->st, r.nrhs, c->stop_dev);
+    HIPCHECK(hipMemcpyAsync(&c->stop_host[o & 3], c->stop_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipEventRecord(c->ev[o & 3], c->stream));
+    r.o = o + 1;
+    return DDAFFT_OK;
+}
+
+// waits for the stop flag of the iteration before the last one enqueued
+int run_wait(Run& r) {
+    const int w = r.o - 2;
+    if (w < 0) return DDAFFT_OK;
+    HIPCHECK(hipEventSynchronize(r.c->ev[w & 3]));
+    r.known_stop = std::min(r.known_stop, r.c->stop_host[w & 3]);
+    return DDAFFT_OK;
+}
+
+// best-iterate restore, final states, history and counters
+int run_finish(Run& r, double* err_hist, int* info, std::vector<SolveState>& sf) {
+    ddafft_ctx* c = r.c;
+    const int nrhs = r.nrhs, max_step = r.max_step;
+    Dims d = c->d;
+    d.nrhs = nrhs;
     {
         dim3 grid((int)std::min<long long>((3 * d.n + kBlock - 1) / kBlock, 2048), nrhs);
         hipLaunchKernelGGL(k_restore, grid, kBlock, 0, c->stream, d, c->st, c->P, c->Pmin);
@@ -985,67 +1022,188 @@ int solve_core(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, flo
     HIPCHECK(hipStreamSynchronize(c->stream));
     if (c->prof) {
         int last = 0;
-        for (int r = 0; r < nrhs; ++r) last = std::max(last, sf[r].last);
+        for (int q = 0; q < nrhs; ++q) last = std::max(last, sf[q].last);
         TRY(prof_collect(c, last));
     }
     if (err_hist) {
         std::vector<double> h((size_t)nrhs * (max_step + 1));
         HIPCHECK(hipMemcpy(h.data(), c->hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (int r = 0; r < nrhs; ++r)
+        for (int q = 0; q < nrhs; ++q)
             for (int o = 0; o <= max_step; ++o)
-                err_hist[(size_t)r * (max_step + 1) + o] = o <= sf[r].last ? h[(size_t)r * (max_step + 1) + o] : NAN;
+                err_hist[(size_t)q * (max_step + 1) + o] = o <= sf[q].last ? h[(size_t)q * (max_step + 1) + o] : NAN;
     }
     if (info) {
-        for (int r = 0; r < nrhs; ++r) {
-            const int L = sf[r].last;
+        for (int q = 0; q < nrhs; ++q) {
+            const int L = sf[q].last;
             int mvs = 0;
-            switch (solver) {
+            switch (r.solver) {
                 case DDAFFT_SOLVER_CG: mvs = 1 + L; break;
                 case DDAFFT_SOLVER_AA: mvs = 2 + 2 * L; break;
                 case DDAFFT_SOLVER_BICGSTAB: mvs = 3 + 2 * L; break;
                 default: mvs = 1 + L; break;
             }
-            info[4 * r + 0] = L;
-            info[4 * r + 1] = sf[r].step_min;
-            info[4 * r + 2] = sf[r].restored;
-            info[4 * r + 3] = mvs;
+            info[4 * q + 0] = L;
+            info[4 * q + 1] = sf[q].step_min;
+            info[4 * q + 2] = sf[q].restored;
+            info[4 * q + 3] = mvs;
         }
     }
     return DDAFFT_OK;
 }
 
+int solve_core(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, double* err_hist,
+               int* info, std::vector<SolveState>& sf) {
+    Run r;
+    r.c = c;
+    r.nrhs = nrhs;
+    r.solver = solver;
+    r.tau = tau;
+    r.max_step = max_step;
+    r.err_max = err_max;
+    TRY(run_begin(r));
+    while (run_more(r)) {
+        TRY(run_enqueue(r));
+        TRY(run_wait(r));
+    }
+    return run_finish(r, err_hist, info, sf);
+}
+
+// Checks one job against its context and stages its inputs: polarizabilities,
+// incidence, directions, and the first approximation in c->P.
+int job_prepare(ddafft_ctx* c, const ddafft_job* j, int solver, int max_step, const float** ddirs,
+                long long* dir_stride) {
+    if (!c || !j || !j->alpha || !j->kinc || !j->einc) return fail(DDAFFT_EINVAL, "NULL argument");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    const int nrhs = j->nrhs;
+    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDAFFT_EINVAL, "nrhs outside [1, nrhs_max]");
+    if (solver < 0 || solver > 3 || max_step < 0) return fail(DDAFFT_EINVAL, "bad solver or max_step");
+    const int ndirs = j->ndirs;
+    if (ndirs < 0 || (ndirs > 0 && !j->esc)) return fail(DDAFFT_EINVAL, "bad direction arguments");
+    *ddirs = nullptr;
+    *dir_stride = 0;
+    if (ndirs > 0) {
+        if (j->dirs) {
+            const size_t per = (size_t)ndirs * 3;
+            const int sets = j->dirs_per_rhs ? nrhs : 1;
+            if ((long long)ndirs * sets > c->dirs_cap) {
+                HIPCHECK(hipStreamSynchronize(c->stream));
+                if (c->dirs) hipFree(c->dirs);
+                c->dirs = nullptr;
+                c->dirs_cap = 0;
+                TRY(dev_alloc(&c->dirs, per * sets));
+                c->dirs_cap = ndirs * sets;
+            }
+            TRY(upload(c, c->dirs, j->dirs, per * sets * sizeof(float)));
+            c->ndirs = j->dirs_per_rhs ? 0 : ndirs;   // per-RHS sets are not resident for later calls
+            if (j->dirs_per_rhs) {
+                *dir_stride = (long long)per;
+                c->dir_pairs.assign((size_t)(nrhs + 1) / 2, 0);
+                for (int q = 0; q + 1 < nrhs; q += 2)
+                    c->dir_pairs[q / 2] =
+                        std::memcmp(j->dirs + q * per, j->dirs + (q + 1) * per, per * sizeof(float)) == 0;
+            }
+        } else if (ndirs != c->ndirs) {
+            return fail(DDAFFT_EINVAL, "dirs == NULL but ndirs differs from the resident directions");
+        }
+        *ddirs = c->dirs;
+    }
+    Dims d = c->d;
+    d.nrhs = nrhs;
+    TRY(upload(c, c->alpha, j->alpha, (size_t)nrhs * c->nmat * sizeof(float2)));
+    TRY(upload(c, c->einc, j->einc, (size_t)nrhs * 3 * sizeof(float2)));
+    TRY(upload(c, c->kinc, j->kinc, (size_t)nrhs * 3 * sizeof(float)));
+    const Geometry g = make_geometry(c, nullptr);
+    dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), nrhs);
+    hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->kinc, c->occ, c->mv, c->alpha, c->nmat,
+                       c->einc, c->P);
+    HIPCHECK(hipGetLastError());
+    return DDAFFT_OK;
+}
+
 }  // namespace
+
+int ddafft_solve_ex(ddafft_ctx* c, const ddafft_job* job, int solver, float tau, int max_step, float err_max,
+                    double* err_hist) {
+    const float* ddirs = nullptr;
+    long long stride = 0;
+    TRY(job_prepare(c, job, solver, max_step, &ddirs, &stride));
+    std::vector<SolveState> sf;
+    TRY(solve_core(c, job->nrhs, solver, tau, max_step, err_max, err_hist, job->info, sf));
+    return observables(c, job->nrhs, job->einc, job->ndirs, ddirs, stride, job->esc, job->cabs, job->cext);
+}
 
 int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, const float* alpha,
                  const float* kinc, const float* einc, int ndirs, const float* dirs, float* esc, float* cabs,
                  float* cext, double* err_hist, int* info) {
-    if (!c || !alpha || !kinc || !einc) return fail(DDAFFT_EINVAL, "NULL argument");
-    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
-    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDAFFT_EINVAL, "nrhs outside [1, nrhs_max]");
-    if (solver < 0 || solver > 3 || max_step < 0) return fail(DDAFFT_EINVAL, "bad solver or max_step");
-    if (ndirs < 0 || (ndirs > 0 && !esc)) return fail(DDAFFT_EINVAL, "bad direction arguments");
-    const float* ddirs = nullptr;
-    if (ndirs > 0) {
-        if (dirs) {
-            TRY(ddafft_set_directions(c, ndirs, dirs));
-        } else if (ndirs != c->ndirs) {
-            return fail(DDAFFT_EINVAL, "dirs == NULL but ndirs differs from the resident directions");
+    if (!kinc) return fail(DDAFFT_EINVAL, "NULL argument");
+    if (nrhs < 1 || nrhs > 1 << 20) return fail(DDAFFT_EINVAL, "nrhs outside [1, nrhs_max]");
+    std::vector<float> k3((size_t)3 * nrhs);
+    for (int q = 0; q < nrhs; ++q) std::memcpy(&k3[3 * (size_t)q], kinc, 3 * sizeof(float));
+    ddafft_job j{};
+    j.nrhs = nrhs;
+    j.alpha = alpha;
+    j.kinc = k3.data();
+    j.einc = einc;
+    j.ndirs = ndirs;
+    j.dirs = dirs;
+    j.dirs_per_rhs = 0;
+    j.esc = esc;
+    j.cabs = cabs;
+    j.cext = cext;
+    j.info = info;
+    return ddafft_solve_ex(c, &j, solver, tau, max_step, err_max, err_hist);
+}
+
+int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, int solver, float tau, int max_step,
+                       float err_max, int width) {
+    if (n < 1 || !ctxs || !jobs) return fail(DDAFFT_EINVAL, "bad batch");
+    for (int k = 0; k < n; ++k)
+        for (int m = 0; m < k; ++m)
+            if (ctxs[k] == ctxs[m]) return fail(DDAFFT_EINVAL, "a context appears twice in one batch");
+    if (width < 1) width = 4;
+    width = std::min(width, n);
+    struct Slot {
+        Run r;
+        int job = -1;
+        const float* ddirs = nullptr;
+        long long stride = 0;
+    };
+    std::vector<Slot> act;
+    int next = 0;
+    auto start = [&](int k) -> int {
+        Slot s;
+        s.job = k;
+        TRY(job_prepare(ctxs[k], &jobs[k], solver, max_step, &s.ddirs, &s.stride));
+        s.r.c = ctxs[k];
+        s.r.nrhs = jobs[k].nrhs;
+        s.r.solver = solver;
+        s.r.tau = tau;
+        s.r.max_step = max_step;
+        s.r.err_max = err_max;
+        TRY(run_begin(s.r));
+        act.push_back(s);
+        return DDAFFT_OK;
+    };
+    while (next < n && (int)act.size() < width) TRY(start(next++));
+    while (!act.empty()) {
+        // one iteration for every instance still running, then the stop flags
+        for (Slot& s : act)
+            if (run_more(s.r)) TRY(run_enqueue(s.r));
+        for (Slot& s : act) TRY(run_wait(s.r));
+        for (size_t k = 0; k < act.size();) {
+            if (run_more(act[k].r)) {
+                ++k;
+                continue;
+            }
+            Slot s = act[k];
+            act.erase(act.begin() + (long)k);
+            const ddafft_job& j = jobs[s.job];
+            std::vector<SolveState> sf;
+            TRY(run_finish(s.r, nullptr, j.info, sf));
+            TRY(observables(s.r.c, j.nrhs, j.einc, j.ndirs, s.ddirs, s.stride, j.esc, j.cabs, j.cext));
+            if (next < n) TRY(start(next++));
         }
-        ddirs = c->dirs;
     }
-    Dims d = c->d;
-    d.nrhs = nrhs;
-    TRY(upload(c, c->alpha, alpha, (size_t)nrhs * c->nmat * sizeof(float2)));
-    TRY(upload(c, c->einc, einc, (size_t)nrhs * 3 * sizeof(float2)));
-    const Geometry g = make_geometry(c, kinc);
-    {
-        dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), nrhs);
-        hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->occ, c->mv, c->alpha, c->nmat, c->einc, c->P);
-        HIPCHECK(hipGetLastError());
-    }
-    std::vector<SolveState> sf;
-    TRY(solve_core(c, nrhs, solver, tau, max_step, err_max, err_hist, info, sf));
-    TRY(observables(c, nrhs, kinc, einc, ndirs, ddirs, esc, cabs, cext));
     return DDAFFT_OK;
 }
 
@@ -1311,8 +1469,9 @@ int ddafft_cabs_cext(ddafft_ctx* c, const float* alpha, const float* kinc, const
     if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
     TRY(upload(c, c->alpha, alpha, (size_t)c->nmat * sizeof(float2)));
     TRY(upload(c, c->einc, einc, 3 * sizeof(float2)));
+    TRY(upload(c, c->kinc, kinc, 3 * sizeof(float)));
     TRY(field_in(c, p, c->P, c->Y));
-    return observables(c, 1, kinc, einc, 0, nullptr, nullptr, cabs, cext);
+    return observables(c, 1, einc, 0, nullptr, 0, nullptr, cabs, cext);
 }
 
 int ddafft_far_field(ddafft_ctx* c, const float* p, int ndirs, const float* dirs, float* esc) {
@@ -1326,7 +1485,8 @@ int ddafft_far_field(ddafft_ctx* c, const float* p, int ndirs, const float* dirs
     const float kz[3] = {0.f, 0.f, 1.f};
     const float e[6] = {1.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     TRY(upload(c, c->einc, e, sizeof(e)));
-    return observables(c, 1, kz, e, ndirs, c->dirs, esc, nullptr, nullptr);
+    TRY(upload(c, c->kinc, kz, sizeof(kz)));
+    return observables(c, 1, e, ndirs, c->dirs, 0, esc, nullptr, nullptr);
 }
 
 int ddafft_profile(ddafft_ctx* c, int enable, double* pass_ms, long long* matvecs) {

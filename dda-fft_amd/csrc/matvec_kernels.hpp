@@ -96,6 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
     constexpr int TJ = TileZF<NL>::T;
     constexpr int LS = NL + kLpad;
     __shared__ float2 s[2 * TJ * LS];
+    __shared__ float2 stw[NL];
     const int c = blockIdx.y, rhs = c / 3;
     float2 c1 = make_float2(0.f, 0.f), c2 = c1;
     int snap = 0;
@@ -120,8 +121,9 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
         s[t * LS + slot] = v;
         s[(tj + t) * LS + slot] = cmul(wpre[l], v);
     }
+    for (int f = threadIdx.x; f < NL; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NL>(s, 2 * tj, LS, tw);
+    fft_lds<NL>(s, 2 * tj, LS, stw);
     float2* dst = b1 + ((long long)c * d.ni + i) * (2 * NL) * d.nj + j0;
     for (int f = threadIdx.x; f < 2 * NL * tj; f += blockDim.x) {
         const int L = f / tj, t = f - L * tj;
@@ -141,6 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
     constexpr int TI = TileZF<NJ>::T;
     constexpr int LS = NJ + kLpad;
     __shared__ float2 s[2 * TI * LS];
+    __shared__ float2 stw[NJ];
     const int c = blockIdx.y, rhs = c / 3;
     if (st && iter >= st[rhs].stop_at) return;
     const int NL2 = 2 * d.nl;
@@ -153,8 +156,9 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
         s[t * LS + slot] = v;
         s[(ti + t) * LS + slot] = cmul(wpre[jj], v);
     }
+    for (int f = threadIdx.x; f < NJ; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NJ>(s, 2 * ti, LS, tw);
+    fft_lds<NJ>(s, 2 * ti, LS, stw);
     for (int f = threadIdx.x; f < 2 * NJ * ti; f += blockDim.x) {
         const int J = f / ti, t = f - J * ti;
         const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
@@ -187,6 +191,8 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     constexpr int LS1 = NI + kLpad, LS2 = 2 * LS1;
     constexpr int ITEMS = (TJ * NI2 + kBlock - 1) / kBlock;
     __shared__ float2 s[CB * TJ * LS2];
+    __shared__ float2 stw1[NI];
+    __shared__ float2 stw2[NI2];
     const int rhs0 = blockIdx.y * RB;
     bool act[RB];
     bool any = false;
@@ -201,23 +207,9 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int L = tile / njt, J0 = (tile - L * njt) * tj;
 
-    for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
-        const int ii = f % NI, r = f / NI, t = r % tj, cc = r / tj;
-        const int c = rhs0 * 3 + cc;
-        float2 v = make_float2(0.f, 0.f);
-        if (act[cc / 3]) v = b2[(((long long)c * NJ2 + J0 + t) * NL2 + L) * NI + ii];
-        const int line = (cc * tj + t) * 2;
-        const int slot = perm_slot<NI>(ii);
-        s[line * LS1 + slot] = v;
-        s[(line + 1) * LS1 + slot] = cmul(wpre[ii], v);
-    }
-    __syncthreads();
-    fft_lds<NI>(s, 2 * CB * tj, LS1, tw1);
-
-    // Â multiply in natural order of I, one Â read for the RB right-hand
-    // sides; results land digit-reversed and conjugated for the inverse
-    // transform (ifft_line, fft_kernel.hpp:301-322).
-    float2 y[ITEMS][CB];
+    // Â of this thread's points, requested first so that its latency overlaps
+    // the field loads and the forward transform
+    float2 a[ITEMS][6];
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
         const int f = threadIdx.x + it * kBlock;
@@ -231,18 +223,49 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
             }
             const float4* ap = reinterpret_cast<const float4*>(ahat + (((long long)Ja * NL2 + La) * NI2 + Ia) * 6);
             const float4 q0 = ap[0], q1 = ap[1], q2 = ap[2];
-            float2 a[6] = {make_float2(q0.x, q0.y), make_float2(q0.z, q0.w), make_float2(q1.x, q1.y),
-                           make_float2(q1.z, q1.w), make_float2(q2.x, q2.y), make_float2(q2.z, q2.w)};
+            a[it][0] = make_float2(q0.x, q0.y);
+            a[it][1] = make_float2(q0.z, q0.w);
+            a[it][2] = make_float2(q1.x, q1.y);
+            a[it][3] = make_float2(q1.z, q1.w);
+            a[it][4] = make_float2(q2.x, q2.y);
+            a[it][5] = make_float2(q2.z, q2.w);
             if (conj)
 #pragma unroll
-                for (int e = 0; e < 6; ++e) a[e].y = -a[e].y;
+                for (int e = 0; e < 6; ++e) a[it][e].y = -a[it][e].y;
+        }
+    }
+
+    for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
+        const int ii = f % NI, r = f / NI, t = r % tj, cc = r / tj;
+        const int c = rhs0 * 3 + cc;
+        float2 v = make_float2(0.f, 0.f);
+        if (act[cc / 3]) v = b2[(((long long)c * NJ2 + J0 + t) * NL2 + L) * NI + ii];
+        const int line = (cc * tj + t) * 2;
+        const int slot = perm_slot<NI>(ii);
+        s[line * LS1 + slot] = v;
+        s[(line + 1) * LS1 + slot] = cmul(wpre[ii], v);
+    }
+    for (int f = threadIdx.x; f < NI; f += blockDim.x) stw1[f] = tw1[f];
+    for (int f = threadIdx.x; f < NI2; f += blockDim.x) stw2[f] = tw2[f];
+    __syncthreads();
+    fft_lds<NI>(s, 2 * CB * tj, LS1, stw1);
+
+    // Â multiply in natural order of I, one Â read for the RB right-hand
+    // sides; results land digit-reversed and conjugated for the inverse
+    // transform (ifft_line, fft_kernel.hpp:301-322).
+    float2 y[ITEMS][CB];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const int f = threadIdx.x + it * kBlock;
+        if (f < tj * NI2) {
+            const int I = f % NI2, t = f / NI2;
 #pragma unroll
             for (int r = 0; r < RB; ++r) {
                 float2 xv[3];
 #pragma unroll
                 for (int comp = 0; comp < 3; ++comp)
                     xv[comp] = s[(((r * 3 + comp) * tj + t) * 2 + (I & 1)) * LS1 + (I >> 1)];
-                ahat_apply(a, xv[0], xv[1], xv[2], y[it][r * 3], y[it][r * 3 + 1], y[it][r * 3 + 2]);
+                ahat_apply(a[it], xv[0], xv[1], xv[2], y[it][r * 3], y[it][r * 3 + 1], y[it][r * 3 + 2]);
             }
         }
     }
@@ -258,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
         }
     }
     __syncthreads();
-    fft_lds<NI2>(s, CB * tj, LS2, tw2);
+    fft_lds<NI2>(s, CB * tj, LS2, stw2);
     for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
         const int t = f % tj, r = f / tj, i = r % NI, cc = r / NI;
         if (!act[cc / 3]) continue;
@@ -278,6 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
     constexpr int TL = TileYI<NJ2>::T;
     constexpr int LS = NJ2 + kLpad;
     __shared__ float2 s[TL * LS];
+    __shared__ float2 stw[NJ2];
     const int c = blockIdx.y, rhs = c / 3;
     if (st && iter >= st[rhs].stop_at) return;
     const int NL2 = 2 * d.nl;
@@ -288,8 +312,9 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
         const int t = f / NJ2, jj = f - t * NJ2;
         s[t * LS + perm_slot<NJ2>(jj)] = cconj(src[f]);
     }
+    for (int f = threadIdx.x; f < NJ2; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NJ2>(s, tl, LS, tw);
+    fft_lds<NJ2>(s, tl, LS, stw);
     const int nj = NJ2 / 2;
     float2* dst = b4 + (((long long)c * d.ni + i) * nj) * NL2 + L0;
     for (int f = threadIdx.x; f < tl * nj; f += blockDim.x) {
@@ -351,6 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restr
     constexpr int TJ = TileZI<NL2>::T;
     constexpr int LS = NL2 + kLpad;
     __shared__ float2 s[3 * TJ * LS];
+    __shared__ float2 stw[NL2];
     __shared__ double red[kSlots * 16];
     const int rhs = blockIdx.y;
     float rB = 0.f;
@@ -367,8 +393,9 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restr
         const float2 v = b4[(((long long)c * d.ni + i) * d.nj + j0 + t) * NL2 + LL];
         s[(comp * tj + t) * LS + perm_slot<NL2>(LL)] = cconj(v);
     }
+    for (int f = threadIdx.x; f < NL2; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NL2>(s, 3 * tj, LS, tw);
+    fft_lds<NL2>(s, 3 * tj, LS, stw);
 
     double acc[kSlots] = {0, 0, 0, 0, 0, 0};
     const float2* al = ea.alpha + (long long)rhs * ea.nmat;

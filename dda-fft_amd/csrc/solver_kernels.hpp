@@ -30,13 +30,15 @@ struct Geometry {
 };
 
 // P0 = alpha_m exp(i k.(r - c)) E on dipoles (dda_func.cpp:654-689).
-// einc: [nrhs][3] complex; alpha: [nrhs][nmat].
-__global__ __launch_bounds__(kBlock) void k_first_approx(Dims d, Geometry g, const int8_t* __restrict__ occ,
+// kinc: [nrhs][3]; einc: [nrhs][3] complex; alpha: [nrhs][nmat].
+__global__ __launch_bounds__(kBlock) void k_first_approx(Dims d, Geometry g, const float* __restrict__ kinc,
+                                                         const int8_t* __restrict__ occ,
                                                          const int* __restrict__ mv, const float2* __restrict__ alpha,
                                                          int nmat, const float2* __restrict__ einc,
                                                          float2* __restrict__ p) {
     const int rhs = blockIdx.y;
     const long long rb = (long long)rhs * 3 * d.n;
+    const float kx = kinc[3 * rhs], ky = kinc[3 * rhs + 1], kz = kinc[3 * rhs + 2];
     const float2 E0 = einc[rhs * 3 + 0], E1 = einc[rhs * 3 + 1], E2 = einc[rhs * 3 + 2];
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < d.n;
          k += (long long)gridDim.x * blockDim.x) {
@@ -48,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_first_approx(Dims d, Geometry g, con
         if (mv[m] != 0) {
             const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
             const float dx = rx - g.cx, dy = ry - g.cy, dz = rz - g.cz;
-            const float ph = g.kx * dx + g.ky * dy + g.kz * dz;
+            const float ph = kx * dx + ky * dy + kz * dz;
             const float2 c0 = exp_i(ph);
             const float2 c1 = cmul(c0, alpha[(long long)rhs * nmat + m]);
             o0 = cmul(c1, E0);
@@ -298,12 +300,14 @@ __global__ __launch_bounds__(kBlock) void k_restore(Dims d, SolveState* __restri
 // partials: [nrhs][gridDim.x][2]
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const int8_t* __restrict__ occ,
+__global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const float* __restrict__ kinc,
+                                                      const int8_t* __restrict__ occ,
                                                       const float2* __restrict__ alpha, int nmat,
                                                       const float2* __restrict__ einc, const float2* __restrict__ P,
                                                       double* __restrict__ partials) {
     __shared__ double red[2 * 16];
     const int rhs = blockIdx.y;
+    const float kx = kinc[3 * rhs], ky = kinc[3 * rhs + 1], kz = kinc[3 * rhs + 2];
     const long long rb = (long long)rhs * 3 * d.n;
     const float2 e0 = cconj(einc[rhs * 3]), e1 = cconj(einc[rhs * 3 + 1]), e2 = cconj(einc[rhs * 3 + 2]);
     const float r4 = (float)(2. / 3.);
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const 
         const long long ij = k / d.nl;
         const int j = (int)(ij % d.nj), i = (int)(ij / d.nj);
         const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
-        const float r3 = -(g.kx * (rx - g.ex) + g.ky * (ry - g.ey) + g.kz * (rz - g.ez));
+        const float r3 = -(kx * (rx - g.ex) + ky * (ry - g.ey) + kz * (rz - g.ez));
         const float2 c0 = exp_i(r3);
         const float2 px = P[rb + k], py = P[rb + d.n + k], pz = P[rb + 2 * d.n + k];
         const float2 c5 = cadd(cmul(pz, e2), cadd(cmul(px, e0), cmul(py, e1)));
@@ -431,10 +435,11 @@ __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const 
 }
 
 // Sums the chunks and applies the transverse projection:
-// out[nrhs][ndirs][3] complex float.
+// out[nrhs][ndirs][3] complex float.  Right-hand side r uses the directions
+// dirs + r * dir_stride (dir_stride 0: one set shared by all).
 __global__ __launch_bounds__(kBlock) void k_far_field_sum(const double* __restrict__ partials, int nchunk, int ndirs,
                                                           int nrhs, const float* __restrict__ dirs,
-                                                          float2* __restrict__ out) {
+                                                          long long dir_stride, float2* __restrict__ out) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nrhs * ndirs) return;
     const int dd = idx % ndirs, rhs = idx / ndirs;
@@ -444,7 +449,8 @@ __global__ __launch_bounds__(kBlock) void k_far_field_sum(const double* __restri
 #pragma unroll
         for (int e = 0; e < 6; ++e) s[e] += p[e];
     }
-    const double n[3] = {dirs[3 * dd], dirs[3 * dd + 1], dirs[3 * dd + 2]};
+    const float* dv = dirs + rhs * dir_stride + 3 * dd;
+    const double n[3] = {dv[0], dv[1], dv[2]};
     const double dr = n[0] * s[0] + n[1] * s[2] + n[2] * s[4];
     const double di = n[0] * s[1] + n[1] * s[3] + n[2] * s[5];
 #pragma unroll

@@ -24,8 +24,18 @@ EXPORTS = (
     "ddafft_get_dipoles", "ddafft_cabs_cext", "ddafft_far_field", "ddafft_profile", "ddafft_time_matvec",
     "ddafft_gradient", "ddafft_polarizability", "ddafft_polarizability_deriv", "ddafft_target_ellipsoid",
     "ddafft_target_parallelepiped", "ddafft_target_bisphere", "ddafft_geometry_fixed", "ddafft_muller",
-    "ddafft_muller_average",
+    "ddafft_muller_average", "ddafft_solve_ex", "ddafft_solve_batch", "ddafft_rng_worker_seed", "ddafft_rng_next",
+    "ddafft_target_random_ellipsoid", "ddafft_target_random_parallelepiped", "ddafft_target_nibble_ellipsoid",
+    "ddafft_target_hierarchical_ellipsoid", "ddafft_target_sphere_spec", "ddafft_worm_count", "ddafft_target_worms",
+    "ddafft_target_file_read", "ddafft_target_file_write", "ddafft_orientation_mc", "ddafft_rotate",
 )
+
+
+class Job(C.Structure):
+    """ddafft_job of include/ddafft.h."""
+    _fields_ = [("nrhs", C.c_int), ("alpha", C.c_void_p), ("kinc", C.c_void_p), ("einc", C.c_void_p),
+                ("ndirs", C.c_int), ("dirs", C.c_void_p), ("dirs_per_rhs", C.c_int), ("esc", C.c_void_p),
+                ("cabs", C.c_void_p), ("cext", C.c_void_p), ("info", C.c_void_p)]
 
 _lib = None
 
@@ -59,6 +69,12 @@ def load(path=LIB_PATH):
     lib.ddafft_profile.argtypes = [P, I, P, P]
     lib.ddafft_time_matvec.argtypes = [P, I, I, P, P]
     lib.ddafft_gradient.argtypes = [P, I, I, I, F, I, F, P, P, P, P, P, I, P, P, P, P, P, P, P, P, P, P, P]
+    lib.ddafft_solve_ex.argtypes = [P, C.POINTER(Job), I, F, I, F, P]
+    lib.ddafft_solve_batch.argtypes = [I, P, P, I, F, I, F, I]
+    lib.ddafft_orientation_mc.argtypes = [P, P]
+    lib.ddafft_rotate.argtypes = [P, I, P, P]
+    lib.ddafft_rng_worker_seed.argtypes = [C.c_ulonglong, I]
+    lib.ddafft_rng_next.argtypes = [P]
     lib.ddafft_polarizability.argtypes = [I, F, P, P, P, P]
     lib.ddafft_polarizability_deriv.argtypes = [I, F, P, P, P, P]
     D = C.c_double
@@ -69,8 +85,9 @@ def load(path=LIB_PATH):
     lib.ddafft_muller.argtypes = [P] * 8
     lib.ddafft_muller_average.argtypes = [I, I, P, P, P, P, P, P, P]
     for name in EXPORTS:
-        if name != "ddafft_last_error":
+        if name not in ("ddafft_last_error", "ddafft_rng_worker_seed"):
             getattr(lib, name).restype = I
+    lib.ddafft_rng_worker_seed.restype = C.c_ulonglong
     _lib = lib
     return lib
 
@@ -219,6 +236,55 @@ class Context:
         return dict(f=f, grad_m=gm, grad_chi=gchi, esc=esc[:, :nd], cabs=cabs, cext=cext, s11=s11[:nd],
                     info=info)
 
+    def _job(self, alpha, kinc, einc, dirs, ndirs):
+        """Builds a Job and its output arrays; keeps the inputs alive in the result."""
+        alpha = np.ascontiguousarray(np.atleast_2d(alpha), np.complex64)
+        einc = np.ascontiguousarray(np.atleast_2d(einc), np.complex64)
+        nrhs = einc.shape[0]
+        if alpha.shape[0] == 1 and nrhs > 1:
+            alpha = np.ascontiguousarray(np.repeat(alpha, nrhs, axis=0))
+        kinc = np.ascontiguousarray(np.broadcast_to(np.asarray(kinc, np.float32).reshape(-1, 3), (nrhs, 3)))
+        per_rhs = 0
+        if dirs is not None:
+            dirs = np.ascontiguousarray(dirs, np.float32)
+            if dirs.ndim == 3:
+                assert dirs.shape[0] == nrhs
+                per_rhs = 1
+                nd = dirs.shape[1]
+            else:
+                dirs = dirs.reshape(-1, 3)
+                nd = dirs.shape[0]
+        else:
+            nd = int(ndirs if ndirs is not None else getattr(self, "ndirs", 0))
+        out = dict(esc=np.zeros((nrhs, max(nd, 1), 3), np.complex64), cabs=np.zeros(nrhs, np.float32),
+                   cext=np.zeros(nrhs, np.float32), info=np.zeros((nrhs, 4), np.int32), nd=nd,
+                   keep=(alpha, kinc, einc, dirs))
+        j = Job(nrhs, _p(alpha), _p(kinc), _p(einc), nd, _p(dirs), per_rhs, _p(out["esc"]), _p(out["cabs"]),
+                _p(out["cext"]), _p(out["info"]))
+        return j, out
+
+    @staticmethod
+    def _results(out):
+        res = []
+        for r in range(out["info"].shape[0]):
+            res.append(dict(esc=out["esc"][r, : out["nd"]], cabs=out["cabs"][r], cext=out["cext"][r],
+                            steps=int(out["info"][r, 0]), step_min=int(out["info"][r, 1]),
+                            restored=bool(out["info"][r, 2]), matvecs=int(out["info"][r, 3])))
+        return res
+
+    def solve_ex(self, solver, alpha, kinc, einc, dirs=None, max_step=20, err_max=1e-3, tau=0.0, ndirs=None):
+        """Like solve, with kinc [nrhs, 3] per right-hand side and dirs either
+        [ndirs, 3] (shared) or [nrhs, ndirs, 3] (one set per right-hand side)."""
+        solver = SOLVERS.get(solver, solver)
+        j, out = self._job(alpha, kinc, einc, dirs, ndirs)
+        hist = np.zeros((j.nrhs, max_step + 1), np.float64)
+        _check(_lib.ddafft_solve_ex(self._h, C.byref(j), int(solver), C.c_float(tau), int(max_step),
+                                    C.c_float(err_max), _p(hist)))
+        res = self._results(out)
+        for r, d in enumerate(res):
+            d["err_hist"] = hist[r, : d["steps"] + 1]
+        return res
+
     def dipoles(self, rhs=0):
         p = np.zeros(self.dims + (3,), np.complex64)
         _check(_lib.ddafft_get_dipoles(self._h, int(rhs), _p(p)))
@@ -309,3 +375,37 @@ def geometry_fixed(nplanes, nangles, theta_min, theta_step, psi=0.0, iota=0.0, k
                                       float(iota), float(kappa), _p(dirs), _p(normals), _p(kinc), _p(einc)))
     return dirs, normals, kinc, (einc[..., 0] + 1j * einc[..., 1]).astype(np.complex64)
 
+
+def solve_batch(ctxs, jobs, solver, max_step=20, err_max=1e-3, tau=0.0, width=4):
+    """Solves several contexts together (ddafft_solve_batch).  jobs: one dict
+    per context with keys alpha, kinc, einc and optionally dirs; returns one
+    list of per-right-hand-side dicts per context."""
+    load()
+    solver = SOLVERS.get(solver, solver)
+    built = [ctx._job(jb["alpha"], jb["kinc"], jb["einc"], jb.get("dirs"), jb.get("ndirs")) for ctx, jb in
+             zip(ctxs, jobs)]
+    arr = (Job * len(built))(*[b[0] for b in built])
+    hs = (C.c_void_p * len(ctxs))(*[c._h.value for c in ctxs])
+    _check(_lib.ddafft_solve_batch(len(ctxs), hs, arr, int(solver), C.c_float(tau), int(max_step),
+                                   C.c_float(err_max), int(width)))
+    return [Context._results(b[1]) for b in built]
+
+
+
+def orientation_mc(state):
+    """Draws one Monte-Carlo orientation from the random stream `state` (a
+    uint64 array of one element, advanced in place); returns the 3x3 matrix."""
+    load()
+    rot = np.zeros(9, np.float32)
+    _check(_lib.ddafft_orientation_mc(_p(state), _p(rot)))
+    return rot.reshape(3, 3)
+
+
+def rotate(rot, v):
+    """Applies an orientation matrix to vectors [..., 3] as main.cpp does (float)."""
+    load()
+    v = np.ascontiguousarray(v, np.float32)
+    flat = v.reshape(-1, 3)
+    out = np.zeros_like(flat)
+    _check(_lib.ddafft_rotate(_p(np.ascontiguousarray(rot, np.float32)), flat.shape[0], _p(flat), _p(out)))
+    return out.reshape(v.shape)

@@ -102,6 +102,36 @@ int ddafft_cabs_cext(ddafft_ctx* ctx, const float* alpha, const float* kinc, con
  * (FarField, dda_func.cpp:1576-1625). */
 int ddafft_far_field(ddafft_ctx* ctx, const float* p, int ndirs, const float* dirs, float* esc);
 
+/* One instance's solve for ddafft_solve_ex / ddafft_solve_batch: nrhs
+ * right-hand sides, each with its own incidence (Monte-Carlo orientations x
+ * polarizations of one target, main.cpp:875-940), optionally each with its
+ * own (rotated) scattering directions.  Outputs as ddafft_solve's. */
+typedef struct {
+    int nrhs;
+    const float* alpha;   /* [nrhs][nmat][2] */
+    const float* kinc;    /* [nrhs][3] */
+    const float* einc;    /* [nrhs][3][2] */
+    int ndirs;
+    const float* dirs;    /* [ndirs][3], or [nrhs][ndirs][3] when dirs_per_rhs; NULL: resident set */
+    int dirs_per_rhs;
+    float* esc;           /* [nrhs][ndirs][3][2] */
+    float* cabs;          /* [nrhs] (may be NULL) */
+    float* cext;          /* [nrhs] (may be NULL) */
+    int* info;            /* [nrhs][4] (may be NULL) */
+} ddafft_job;
+
+/* ddafft_solve with per-right-hand-side incidence and directions. */
+int ddafft_solve_ex(ddafft_ctx* ctx, const ddafft_job* job, int solver, float tau, int max_step, float err_max,
+                    double* err_hist);
+
+/* Solves n instances (distinct contexts, e.g. random realizations of uneven
+ * lattices, each with its own A-hat) together: up to `width` of them (0: 4)
+ * run at once, each on its own stream, one iteration per instance per round,
+ * each right-hand side stopping on its own (the reference's sample loop,
+ * main.cpp:774-996, batched).  Results as ddafft_solve_ex for each job. */
+int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, int solver, float tau, int max_step,
+                       float err_max, int width);
+
 /* Forward + adjoint gradients (no counterpart in the reference, whose forward
  * model DDA_FFT, dda_func.cpp:1075-1570, CabsCext :1679-1727, FarField
  * :1576-1625 and BuildMatrixJones/JonesToMuller, mmuller.cpp:175-314, they

@@ -171,6 +171,12 @@ class Oracle(_Base):
     def _real_arg(self, x):
         return C.c_float(x) if self.real is np.float32 else C.c_double(x)
 
+    def set_phase_rounding(self, correctly_rounded):
+        """Test switch: incident/scattered phases from correctly rounded
+        double cos/sin (as the HIP library computes them) instead of the
+        reference's float libm calls (the default)."""
+        self.lib.orc_set_phase_rounding(_I(int(bool(correctly_rounded))))
+
     def matvec(self, ahat, occ, mv, alpha, x, conj=False):
         occ = np.ascontiguousarray(occ, np.int8)
         ni, nj, nl = occ.shape

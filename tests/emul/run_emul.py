@@ -87,10 +87,35 @@ def main():
                         "cabs_cext")
             k, t = ctx.time_matvec(2, 1)
             ok &= check(t >= 0, "time_matvec")
+    ok &= check_batch(orc)
     if not quick:
         ok &= check_gradient()
     print("ALL OK" if ok else "SOME CHECKS FAILED", flush=True)
     return 0 if ok else 1
+
+
+def check_batch(orc):
+    """ddafft_solve_batch over two uneven lattices (width 1: the refill path),
+    per-right-hand-side incidence and directions, against single oracle solves."""
+    from test_gpu_batch import check_instance, mc_instance
+    insts = [mc_instance(orc, (4, 4, 8), 0.6, 1.5 + 0.01j, 1, 3), mc_instance(orc, (8, 8, 8), 0.5, 1.33 + 0.05j, 1, 4)]
+    ctxs = []
+    for inst in insts:
+        ctx = ddafft.Context(inst["occ"].shape, inst["kd"], nrhs_max=2)
+        ctx.set_target(inst["occ"], inst["mv"])
+        ctxs.append(ctx)
+    jobs = [dict(alpha=i["alpha"], kinc=i["kinc"], einc=i["einc"], dirs=i["dirs"]) for i in insts]
+    out = ddafft.solve_batch(ctxs, jobs, "bicgstab", max_step=6, err_max=1e-6, width=1)
+    good = True
+    try:
+        for k, (inst, res) in enumerate(zip(insts, out)):
+            check_instance(orc, inst, res, 2, 6, 1e-6, "emul batch %d" % k)
+    except AssertionError as e:
+        print(e)
+        good = False
+    for c in ctxs:
+        c.close()
+    return check(good, "solve_batch")
 
 
 def check_gradient():
