@@ -76,6 +76,77 @@ def matvec_bytes(n, nrhs):
     }
 
 
+BATCH_X = (8.0, 9.0, 10.0, 11.0)     # one instance per size parameter, each with its own kd and A-hat
+BATCH_ORIENT = 8                     # Monte-Carlo orientations per instance
+
+
+def batched_run(w, steps=2, warmup=1):
+    """The batched form: len(BATCH_X) instances of the 64^3 spheroid, each with
+    its own size parameter (kd, A-hat), 8 Monte-Carlo orientations x 2
+    polarizations = 16 right-hand sides per instance, every orientation with
+    its own incidence and rotated 724 directions; ddafft_solve_batch runs the
+    instances together.  Returns solves/s over `steps` timed batches."""
+    state = np.array([20240611], np.uint64)
+    ctxs, jobs = [], []
+    m = complex(M_INDEX[0], M_INDEX[1])
+    for x in BATCH_X:
+        kd = np.float32(2.0 * np.float32(x) / np.float32(GRID[0]))
+        ctx = ddafft.Context(GRID, kd, nrhs_max=2 * BATCH_ORIENT)
+        ctx.set_target(w["occ"], [0, 1])
+        a = ddafft.polarizability(1, kd, m)
+        kinc, einc, dirs = [], [], []
+        for _ in range(BATCH_ORIENT):
+            R = ddafft.orientation_mc(state)
+            k = ddafft.rotate(R, w["kinc"])
+            e = ddafft.rotate(R, np.real(w["einc"]).astype(np.float32))
+            d = ddafft.rotate(R, w["dirs"])
+            for p in range(2):
+                kinc.append(k)
+                einc.append(e[p].astype(np.complex64))
+                dirs.append(d)
+        ctxs.append(ctx)
+        jobs.append(dict(alpha=np.array([0, a], np.complex64), kinc=np.array(kinc), einc=np.array(einc),
+                         dirs=np.array(dirs)))
+    for _ in range(warmup):
+        out = ddafft.solve_batch(ctxs, jobs, "bicgstab", max_step=MAX_STEPS, err_max=TOL, width=len(ctxs))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ddafft.solve_batch(ctxs, jobs, "bicgstab", max_step=MAX_STEPS, err_max=TOL, width=len(ctxs))
+    dt = time.perf_counter() - t0
+    solves = steps * sum(len(o) for o in out)
+    mvs = steps * sum(r["matvecs"] for o in out for r in o)
+    its = [r["steps"] for o in out for r in o]
+    for c in ctxs:
+        c.close()
+    return {"value": round(solves / dt, 3), "unit": "solves/s", "matvecs_per_s": round(mvs / dt, 1),
+            "instances": len(BATCH_X), "rhs_per_instance": 2 * BATCH_ORIENT, "steps": steps,
+            "iterations_min_max": [min(its), max(its)],
+            "workload": "64^3 oblate spheroid, x in %s (own kd and A-hat each), %d MC orientations x 2 "
+                        "polarizations per instance, 724 rotated directions each, BiCGStab resid 1e-3"
+                        % (list(BATCH_X), BATCH_ORIENT)}
+
+
+def gradient_run(ctx, w, fwd_ms, reps=5):
+    """The with-gradient form on the plain workload: forward solve of both
+    polarizations, adjoint solve of d(mean Cext)/d(m, chi), assembly."""
+    nd = w["dirs"].shape[0]
+    W = np.zeros((1, 2 + nd), np.float32)
+    W[0, 0] = 1.0
+    m = complex(M_INDEX[0], M_INDEX[1])
+    alpha = np.stack([w["alpha"], w["alpha"]])
+    dadm = np.array([[0, ddafft.polarizability_deriv(1, w["kd"], m)]] * 2, np.complex64)
+    args = dict(dirs=None, normals=None, max_step=MAX_STEPS, err_max=TOL, ndirs=nd)
+    res = ctx.gradient("bicgstab", alpha, dadm, [1.0, m], w["kinc"], w["einc"], W, **args)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = ctx.gradient("bicgstab", alpha, dadm, [1.0, m], w["kinc"], w["einc"], W, **args)
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    mvs = int(res["info"][:, 3].sum())
+    return {"ms_per_gradient": round(ms, 4), "forward_ms": round(fwd_ms, 4), "ratio_to_forward": round(ms / fwd_ms, 3),
+            "solves_per_s": round(2 / (ms * 1e-3), 3), "matvecs": mvs,
+            "objective": "d(mean Cext)/d(Re m, Im m) and d/d(chi) per dipole, both polarizations"}
+
+
 def cpu_baseline(w, budget_s=10.0):
     """The reference's own DDA_FFT (oracle/_ref, compiled from its sources) on
     this host for the same sample; falls back to the oracle restatement."""
@@ -116,6 +187,8 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-instance form")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the forward+adjoint form")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (torch), the reference kernel "
                          "against which rocprofv3 FETCH_SIZE / WRITE_SIZE counters are scaled")
@@ -201,9 +274,15 @@ def main():
             "traffic": traffic_from_profiles(),
         },
     }
+    if not args.no_gradient:
+        out["gradient"] = gradient_run(ctx, w, 1e3 * dt / args.steps)
+    ctx.close()
+    if not args.no_batched:
+        b = batched_run(w)
+        b["vs_plain"] = round(b["value"] / out["value"], 3)
+        out["batched"] = b
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
-    ctx.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -86,25 +86,6 @@ struct Plan {
     static_assert((1 << K) == N, "power-of-two line length");
 };
 
-// Slot k of the digit-reversed input order that receives input element j
-// (perm[k] == j, with perm as in fft_kernel.hpp:222-236).
-template <int N>
-__device__ __forceinline__ int perm_slot(int j) {
-    constexpr int K = Plan<N>::K;
-    int k = 0, mult = 1, rem = j, L = 1;
-#pragma unroll
-    for (int t = 0; t < Plan<N>::NST; ++t) {
-        const int r = plan_radix(K, t);
-        L *= r;
-        const int w = N / L;
-        const int d = rem / w;
-        rem -= d * w;
-        k += d * mult;
-        mult *= r;
-    }
-    return k;
-}
-
 // ---------------------------------------------------------------------------
 // Butterflies (fft_kernel.hpp:47-106)
 // ---------------------------------------------------------------------------
@@ -164,15 +145,57 @@ template <>
 __device__ __forceinline__ void butterfly<8>(float2* z) { bf8(z); }
 
 // ---------------------------------------------------------------------------
-// In-place line FFTs in LDS.  `nlines` lines of N points at stride `ls`,
-// input already in digit-reversed order; output in natural order.  Stage t
-// combines r_t sub-transforms of length h into transforms of length span
-// (fft_kernel.hpp:270-285, stage kernels :119-185).
+// Line FFTs in LDS.  `nlines` lines of N points at stride `ls`, input in
+// natural order, output in natural order.  Stage 0 gathers its inputs through
+// the digit-reversed order (fft_kernel.hpp:222-236) into registers, so loads
+// into LDS stay contiguous; every later stage t combines r_t sub-transforms of
+// length h into transforms of length span in place (fft_kernel.hpp:270-285,
+// stage kernels :119-185).
+//
+// Thread mapping: consecutive lanes take the same butterfly of consecutive
+// lines, so with the padded line stride their LDS addresses fall in
+// different banks.  The flat index f = b * nlines + line advances by the
+// block size without divisions (LineWalk).
 // ---------------------------------------------------------------------------
+
+struct LineWalk {
+    int nlines, dq, dr, b0, l0;
+    __device__ __forceinline__ explicit LineWalk(int nl) : nlines(nl) {
+        dq = kBlock / nl;
+        dr = kBlock - dq * nl;
+        b0 = (int)threadIdx.x / nl;
+        l0 = (int)threadIdx.x - b0 * nl;
+    }
+    __device__ __forceinline__ void step(int& b, int& line) const {
+        b += dq;
+        line += dr;
+        if (line >= nlines) {
+            line -= nlines;
+            ++b;
+        }
+    }
+};
+
+// Natural index of the element that digit-reversed slot g * r_0 + q holds,
+// less q * (N / r_0): the digit reversal of g over the radices r_1, r_2, ...
+template <int N>
+__device__ __forceinline__ int rest_natural(int g) {
+    constexpr int K = Plan<N>::K;
+    int j = 0, rem = g, L = plan_radix(K, 0);
+#pragma unroll
+    for (int t = 1; t < Plan<N>::NST; ++t) {
+        const int r = plan_radix(K, t);
+        L *= r;
+        const int d = rem % r;
+        rem /= r;
+        j += d * (N / L);
+    }
+    return j;
+}
 
 template <int N, int T>
 struct StageRunner {
-    static __device__ __forceinline__ void run(float2* s, int nlines, int ls, const float2* __restrict__ tw,
+    static __device__ __forceinline__ void run(float2* s, const LineWalk& w, int ls, const float2* __restrict__ tw,
                                                int span_prev) {
         if constexpr (T < Plan<N>::NST) {
             constexpr int K = Plan<N>::K;
@@ -181,13 +204,8 @@ struct StageRunner {
             const int H = span_prev;
             const int step = N / span;
             constexpr int NB = N / R;
-            const int total = nlines * NB;
-            // consecutive lanes take the same butterfly of consecutive lines:
-            // with the padded line stride their LDS addresses fall in
-            // different banks (lanes walking one line would hit one bank)
-            for (int f = threadIdx.x; f < total; f += blockDim.x) {
-                const int b = f / nlines;
-                const int line = f - b * nlines;
+            int b = w.b0, line = w.l0;
+            while (b < NB) {
                 const int g = b / H;
                 const int i = b - g * H;
                 float2* v = s + line * ls + g * span + i;
@@ -197,25 +215,61 @@ struct StageRunner {
                 if (R > 2 && i > 0) {
 #pragma unroll
                     for (int q = 1; q < R; ++q) {
-                        const float2 w = tw[q * i * step];
-                        const float tr = z[q].x * w.x - z[q].y * w.y;
-                        z[q].y = z[q].x * w.y + z[q].y * w.x;
+                        const float2 tv = tw[q * i * step];
+                        const float tr = z[q].x * tv.x - z[q].y * tv.y;
+                        z[q].y = z[q].x * tv.y + z[q].y * tv.x;
                         z[q].x = tr;
                     }
                 }
                 butterfly<R>(z);
 #pragma unroll
                 for (int q = 0; q < R; ++q) v[q * H] = z[q];
+                w.step(b, line);
             }
             __syncthreads();
-            StageRunner<N, T + 1>::run(s, nlines, ls, tw, span);
+            StageRunner<N, T + 1>::run(s, w, ls, tw, span);
         }
     }
 };
 
-template <int N>
+// MAXL: the most lines any call of this instance passes (sizes the registers
+// that carry stage 0 across its barrier).  Ends with a __syncthreads.
+template <int N, int MAXL>
 __device__ __forceinline__ void fft_lds(float2* s, int nlines, int ls, const float2* __restrict__ tw) {
-    StageRunner<N, 0>::run(s, nlines, ls, tw, 1);
+    constexpr int K = Plan<N>::K;
+    constexpr int R0 = plan_radix(K, 0);
+    constexpr int NB0 = N / R0;
+    constexpr int ITEMS = (MAXL * NB0 + kBlock - 1) / kBlock;
+    const LineWalk w(nlines);
+    // stage 0: gather in digit-reversed order (no twiddles: i = 0 throughout)
+    float2 z[ITEMS][R0];
+    int bb[ITEMS], ll[ITEMS];
+    {
+        int b = w.b0, line = w.l0;
+#pragma unroll
+        for (int it = 0; it < ITEMS; ++it) {
+            bb[it] = b;
+            ll[it] = line;
+            if (b < NB0) {
+                const float2* v = s + line * ls + rest_natural<N>(b);
+#pragma unroll
+                for (int q = 0; q < R0; ++q) z[it][q] = v[q * (N / R0)];
+                butterfly<R0>(z[it]);
+            }
+            w.step(b, line);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        if (bb[it] < NB0) {
+            float2* v = s + ll[it] * ls + bb[it] * R0;
+#pragma unroll
+            for (int q = 0; q < R0; ++q) v[q] = z[it][q];
+        }
+    }
+    __syncthreads();
+    StageRunner<N, 1>::run(s, w, ls, tw, R0);
 }
 
 // ---------------------------------------------------------------------------

@@ -117,16 +117,16 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
         const float2 v = prologue(pa, c1, c2, cb + k, dip);
         if (snap) pa.Pmin[cb + k] = pa.P[cb + k];
         const int t = f / NL, l = f - t * NL;
-        const int slot = perm_slot<NL>(l);
-        s[t * LS + slot] = v;
-        s[(tj + t) * LS + slot] = cmul(wpre[l], v);
+        s[t * LS + l] = v;
+        s[(tj + t) * LS + l] = cmul(wpre[l], v);
     }
     for (int f = threadIdx.x; f < NL; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NL>(s, 2 * tj, LS, stw);
+    fft_lds<NL, 2 * TJ>(s, 2 * tj, LS, stw);
+    const int ltj = __builtin_ctz(tj);
     float2* dst = b1 + ((long long)c * d.ni + i) * (2 * NL) * d.nj + j0;
     for (int f = threadIdx.x; f < 2 * NL * tj; f += blockDim.x) {
-        const int L = f / tj, t = f - L * tj;
+        const int L = f >> ltj, t = f - (L << ltj);
         const float2 v = (L & 1) ? s[(tj + t) * LS + (L >> 1)] : s[t * LS + (L >> 1)];
         dst[(long long)L * d.nj + t] = v;
     }
@@ -152,15 +152,15 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
     for (int f = threadIdx.x; f < ti * NJ; f += blockDim.x) {
         const int t = f / NJ, jj = f - t * NJ;
         const float2 v = b1[(((long long)c * d.ni + i0 + t) * NL2 + L) * NJ + jj];
-        const int slot = perm_slot<NJ>(jj);
-        s[t * LS + slot] = v;
-        s[(ti + t) * LS + slot] = cmul(wpre[jj], v);
+        s[t * LS + jj] = v;
+        s[(ti + t) * LS + jj] = cmul(wpre[jj], v);
     }
     for (int f = threadIdx.x; f < NJ; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NJ>(s, 2 * ti, LS, stw);
+    fft_lds<NJ, 2 * TI>(s, 2 * ti, LS, stw);
+    const int lti = __builtin_ctz(ti);
     for (int f = threadIdx.x; f < 2 * NJ * ti; f += blockDim.x) {
-        const int J = f / ti, t = f - J * ti;
+        const int J = f >> lti, t = f - (J << lti);
         const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
         b2[(((long long)c * 2 * NJ + J) * NL2 + L) * d.ni + i0 + t] = v;
     }
@@ -188,9 +188,9 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     constexpr int CB = 3 * RB;                       // component lines per (J, L)
     constexpr int TJ = TileX<NI, RB>::T;
     constexpr int NI2 = 2 * NI;
-    constexpr int LS1 = NI + kLpad, LS2 = 2 * LS1;
+    constexpr int LS1 = NI + kLpad, LS2 = NI2 + kLpad;   // forward / inverse line strides
     constexpr int ITEMS = (TJ * NI2 + kBlock - 1) / kBlock;
-    __shared__ float2 s[CB * TJ * LS2];
+    __shared__ float2 s[CB * TJ * 2 * LS1];
     __shared__ float2 stw1[NI];
     __shared__ float2 stw2[NI2];
     const int rhs0 = blockIdx.y * RB;
@@ -235,20 +235,20 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
         }
     }
 
+    const int ltj = __builtin_ctz(tj);
     for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
-        const int ii = f % NI, r = f / NI, t = r % tj, cc = r / tj;
+        const int ii = f % NI, r = f / NI, t = r & (tj - 1), cc = r >> ltj;
         const int c = rhs0 * 3 + cc;
         float2 v = make_float2(0.f, 0.f);
         if (act[cc / 3]) v = b2[(((long long)c * NJ2 + J0 + t) * NL2 + L) * NI + ii];
         const int line = (cc * tj + t) * 2;
-        const int slot = perm_slot<NI>(ii);
-        s[line * LS1 + slot] = v;
-        s[(line + 1) * LS1 + slot] = cmul(wpre[ii], v);
+        s[line * LS1 + ii] = v;
+        s[(line + 1) * LS1 + ii] = cmul(wpre[ii], v);
     }
     for (int f = threadIdx.x; f < NI; f += blockDim.x) stw1[f] = tw1[f];
     for (int f = threadIdx.x; f < NI2; f += blockDim.x) stw2[f] = tw2[f];
     __syncthreads();
-    fft_lds<NI>(s, 2 * CB * tj, LS1, stw1);
+    fft_lds<NI, 2 * CB * TJ>(s, 2 * CB * tj, LS1, stw1);
 
     // Â multiply in natural order of I, one Â read for the RB right-hand
     // sides; results land digit-reversed and conjugated for the inverse
@@ -275,15 +275,14 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
         const int f = threadIdx.x + it * kBlock;
         if (f < tj * NI2) {
             const int I = f % NI2, t = f / NI2;
-            const int slot = perm_slot<NI2>(I);
 #pragma unroll
-            for (int cc = 0; cc < CB; ++cc) s[(cc * tj + t) * LS2 + slot] = cconj(y[it][cc]);
+            for (int cc = 0; cc < CB; ++cc) s[(cc * tj + t) * LS2 + I] = cconj(y[it][cc]);
         }
     }
     __syncthreads();
-    fft_lds<NI2>(s, CB * tj, LS2, stw2);
+    fft_lds<NI2, CB * TJ>(s, CB * tj, LS2, stw2);
     for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
-        const int t = f % tj, r = f / tj, i = r % NI, cc = r / NI;
+        const int t = f & (tj - 1), r = f >> ltj, i = r % NI, cc = r / NI;
         if (!act[cc / 3]) continue;
         const int c = rhs0 * 3 + cc;
         b3[(((long long)c * d.ni + i) * NL2 + L) * NJ2 + J0 + t] = cconj(s[(cc * tj + t) * LS2 + i]);
@@ -310,15 +309,16 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
     const float2* src = b3 + (((long long)c * d.ni + i) * NL2 + L0) * NJ2;
     for (int f = threadIdx.x; f < tl * NJ2; f += blockDim.x) {
         const int t = f / NJ2, jj = f - t * NJ2;
-        s[t * LS + perm_slot<NJ2>(jj)] = cconj(src[f]);
+        s[t * LS + jj] = cconj(src[f]);
     }
     for (int f = threadIdx.x; f < NJ2; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NJ2>(s, tl, LS, stw);
+    fft_lds<NJ2, TL>(s, tl, LS, stw);
     const int nj = NJ2 / 2;
+    const int ltl = __builtin_ctz(tl);
     float2* dst = b4 + (((long long)c * d.ni + i) * nj) * NL2 + L0;
     for (int f = threadIdx.x; f < tl * nj; f += blockDim.x) {
-        const int j = f / tl, t = f - j * tl;
+        const int j = f >> ltl, t = f - (j << ltl);
         dst[(long long)j * NL2 + t] = cconj(s[t * LS + j]);
     }
 }
@@ -387,15 +387,16 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restr
     const int nl = NL2 / 2;
     const int tj = min(TJ, d.nj), njt = d.nj / tj;
     const int i = blockIdx.x / njt, j0 = (blockIdx.x - i * njt) * tj;
+    const int ltj = __builtin_ctz(tj);
     for (int f = threadIdx.x; f < 3 * tj * NL2; f += blockDim.x) {
-        const int LL = f % NL2, r = f / NL2, t = r % tj, comp = r / tj;
+        const int LL = f % NL2, r = f / NL2, t = r & (tj - 1), comp = r >> ltj;
         const int c = rhs * 3 + comp;
         const float2 v = b4[(((long long)c * d.ni + i) * d.nj + j0 + t) * NL2 + LL];
-        s[(comp * tj + t) * LS + perm_slot<NL2>(LL)] = cconj(v);
+        s[(comp * tj + t) * LS + LL] = cconj(v);
     }
     for (int f = threadIdx.x; f < NL2; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NL2>(s, 3 * tj, LS, stw);
+    fft_lds<NL2, 3 * TJ>(s, 3 * tj, LS, stw);
 
     double acc[kSlots] = {0, 0, 0, 0, 0, 0};
     const float2* al = ea.alpha + (long long)rhs * ea.nmat;
