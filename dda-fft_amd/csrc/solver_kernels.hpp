@@ -358,11 +358,25 @@ __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const 
 constexpr int kFD = 4;            // directions per block
 constexpr int kFFAxis = 3 * 256;  // largest ni + nj + nl
 
+// complex product and multiply-accumulate with fused multiply-adds
+__device__ __forceinline__ float2 cmul_fma(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ void cmac_fma(float2& acc, float2 a, float2 b) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
+}
+
+// The far field does not reproduce the reference's sequential float sum in
+// any case (its block sums are FP64), so its products use fused
+// multiply-adds: a complex multiply-accumulate is four FMAs.
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const int* __restrict__ cells, int ncells,
                                                       const float* __restrict__ dirs, int ndirs, int rhs0,
                                                       const float2* __restrict__ P, double* __restrict__ partials) {
-    constexpr int NV = kFD * R * 6;
+    constexpr int NV = R * 6;
     __shared__ float2 tab[kFD * kFFAxis];
     __shared__ double red[NV * 16];
     const int nchunk = gridDim.x;
@@ -388,12 +402,14 @@ __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const 
         for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) acc[q][r][c] = make_float2(0.f, 0.f);
+    // lattice dimensions are powers of two: cell index -> (i, j, l) by shifts
+    const int sl = __builtin_ctz(d.nl), sj = __builtin_ctz(d.nj);
     const int per = (ncells + nchunk - 1) / nchunk;
     const int lo = blockIdx.x * per, hi = min(ncells, lo + per);
     for (int e = lo + threadIdx.x; e < hi; e += blockDim.x) {
         const int k = cells[e];
-        const int l = k % d.nl, ij = k / d.nl;
-        const int j = ij % d.nj, i = ij / d.nj;
+        const int l = k & (d.nl - 1), ij = k >> sl;
+        const int j = ij & (d.nj - 1), i = ij >> sj;
         float2 p[R][3];
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -402,33 +418,31 @@ __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const 
 #pragma unroll
         for (int q = 0; q < kFD; ++q) {
             const float2* t = tab + q * kFFAxis;
-            const float2 ph = cmul(cmul(t[i], t[d.ni + j]), t[d.ni + d.nj + l]);
+            const float2 ph = cmul_fma(cmul_fma(t[i], t[d.ni + j]), t[d.ni + d.nj + l]);
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) acc[q][r][c] = cadd(acc[q][r][c], cmul(ph, p[r][c]));
+                for (int c = 0; c < 3; ++c) cmac_fma(acc[q][r][c], ph, p[r][c]);
         }
     }
-    double v[NV];
+    // one direction at a time through the FP64 block sum keeps few values live
 #pragma unroll
-    for (int q = 0; q < kFD; ++q)
+    for (int q = 0; q < kFD; ++q) {
+        double v[NV];
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                v[((q * R + r) * 3 + c) * 2] = acc[q][r][c].x;
-                v[((q * R + r) * 3 + c) * 2 + 1] = acc[q][r][c].y;
+                v[(r * 3 + c) * 2] = acc[q][r][c].x;
+                v[(r * 3 + c) * 2 + 1] = acc[q][r][c].y;
             }
-    block_sum<NV>(v, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < kFD; ++q) {
-            if (d0 + q >= ndirs) break;
+        block_sum<NV>(v, red);
+        if (threadIdx.x == 0 && d0 + q < ndirs) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double* o = partials + (((long long)(rhs0 + r) * nchunk + blockIdx.x) * ndirs + d0 + q) * 6;
 #pragma unroll
-                for (int e = 0; e < 6; ++e) o[e] = v[(q * R + r) * 6 + e];
+                for (int e = 0; e < 6; ++e) o[e] = v[r * 6 + e];
             }
         }
     }
