@@ -146,6 +146,10 @@ struct ddafft_ctx {
     float2* alpha = nullptr;           // [nrhs][nmat]
     float2* einc = nullptr;            // [nrhs][3]
     float* kinc = nullptr;             // [nrhs][3] incident direction per right-hand side
+    int* inc_axis = nullptr;           // [nrhs] axis of an axis-aligned incidence, else -1
+    float2* inc_tab = nullptr;         // [nrhs][2][max(ni, nj, nl)] tabulated incident phases
+    std::vector<int> inc_axis_h;       // host staging of the two (kept alive for async copies)
+    std::vector<float2> inc_tab_h;
     SolveState* st = nullptr;          // [nrhs]
     double* hist = nullptr;
     int hist_cap = 0;
@@ -454,6 +458,12 @@ int upload(ddafft_ctx* c, void* dst, const void* src, size_t bytes) {
     return DDAFFT_OK;
 }
 
+#define TRY(x)                  \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_) return rc_;    \
+    } while (0)
+
 int field_in(ddafft_ctx* c, const float* host_aos, float2* dev_soa, float2* staging) {
     const long long n = c->d.n;
     HIPCHECK(hipMemcpyAsync(staging, host_aos, (size_t)n * 3 * sizeof(float2), hipMemcpyHostToDevice, c->stream));
@@ -489,6 +499,44 @@ Geometry make_geometry(const ddafft_ctx* c, const float* kinc) {
     return g;
 }
 
+// Uploads the incidence of nrhs right-hand sides (kinc [nrhs][3]) and, for
+// each axis-aligned one, the tables of its FirstApproximation and CabsCext
+// phases evaluated as the reference evaluates them: the same float argument
+// (dda_func.cpp:663-675, 1694-1700) and the C library's cosf/sinf.
+int inc_tab_len(const ddafft_ctx* c) { return std::max(c->d.ni, std::max(c->d.nj, c->d.nl)); }
+
+IncPhase inc_phase(const ddafft_ctx* c) { return IncPhase{c->inc_axis, c->inc_tab, inc_tab_len(c)}; }
+
+int upload_incidence(ddafft_ctx* c, int nrhs, const float* kinc) {
+    HIPCHECK(hipStreamSynchronize(c->stream));   // the staging vectors are reused
+    const Geometry g = make_geometry(c, nullptr);
+    c->inc_axis_h.assign(nrhs, -1);
+    const int tn = inc_tab_len(c);
+    c->inc_tab_h.assign((size_t)nrhs * 2 * tn, make_float2(1.f, 0.f));
+    const int n[3] = {c->d.ni, c->d.nj, c->d.nl};
+    const float cen1[3] = {g.cx, g.cy, g.cz}, cen2[3] = {g.ex, g.ey, g.ez};
+    for (int r = 0; r < nrhs; ++r) {
+        const float* k = kinc + 3 * r;
+        const int nz = (k[0] != 0.f) + (k[1] != 0.f) + (k[2] != 0.f);
+        if (nz != 1) continue;
+        const int a = k[0] != 0.f ? 0 : (k[1] != 0.f ? 1 : 2);
+        c->inc_axis_h[r] = a;
+        float2* t1 = &c->inc_tab_h[(size_t)r * 2 * tn];
+        float2* t2 = t1 + tn;
+        for (int q = 0; q < n[a]; ++q) {
+            const float rq = (float)q * g.d0;
+            const float arg1 = k[a] * (rq - cen1[a]);
+            const float arg2 = -(k[a] * (rq - cen2[a]));
+            t1[q] = make_float2(std::cos(arg1), std::sin(arg1));
+            t2[q] = make_float2(std::cos(arg2), std::sin(arg2));
+        }
+    }
+    TRY(upload(c, c->kinc, kinc, (size_t)nrhs * 3 * sizeof(float)));
+    TRY(upload(c, c->inc_axis, c->inc_axis_h.data(), (size_t)nrhs * sizeof(int)));
+    TRY(upload(c, c->inc_tab, c->inc_tab_h.data(), c->inc_tab_h.size() * sizeof(float2)));
+    return DDAFFT_OK;
+}
+
 // 4 pi / Re(E . conj(E)) rounded to float (dda_func.cpp:1683)
 float cabs_prefactor(const float* e) {
     auto cm = [](float ar, float ai, float br, float bi, float& cr, float& ci) {
@@ -522,7 +570,7 @@ int ddafft_destroy(ddafft_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->ahat, c->tables, c->occ, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
                     c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
-                    c->dirs, c->esc, c->stop_dev, c->Pf, c->phase, c->gaux, c->kinc};
+                    c->dirs, c->esc, c->stop_dev, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->stop_host) hipHostFree(c->stop_host);
@@ -575,6 +623,8 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
         if (!rc) rc = dev_alloc(f, (size_t)(n * C));
     if (!rc) rc = dev_alloc(&c->einc, (size_t)3 * nrhs_max);
     if (!rc) rc = dev_alloc(&c->kinc, (size_t)3 * nrhs_max);
+    if (!rc) rc = dev_alloc(&c->inc_axis, (size_t)nrhs_max);
+    if (!rc) rc = dev_alloc(&c->inc_tab, (size_t)nrhs_max * 2 * inc_tab_len(c));
     if (!rc) rc = dev_alloc(&c->st, (size_t)nrhs_max);
     if (!rc) rc = dev_alloc(&c->occ, (size_t)n);
     if (!rc) rc = dev_alloc(&c->stop_dev, 1);
@@ -766,12 +816,6 @@ struct Seq {
     }
 };
 
-#define TRY(x)                  \
-    do {                        \
-        int rc_ = (x);          \
-        if (rc_) return rc_;    \
-    } while (0)
-
 ProArgs pro(int mode, float2* A, float2* B = nullptr, const float2* D = nullptr) {
     return ProArgs{mode, A, B, D, nullptr, nullptr};
 }
@@ -885,8 +929,8 @@ int observables(ddafft_ctx* c, int nrhs, const float* einc, int ndirs, const flo
                                                               (2048 + dtiles - 1) / std::max(dtiles, 1)})) : 0;
     TRY(ensure_partials(c, std::max<size_t>((size_t)nrhs * cb * 2, (size_t)nrhs * fchunk * std::max(ndirs, 1) * 6)));
     dim3 gc(cb, nrhs);
-    hipLaunchKernelGGL(k_cabs_cext, gc, kBlock, 0, c->stream, d, g, c->kinc, c->occ, c->alpha, c->nmat, c->einc, c->P,
-                       c->partials);
+    hipLaunchKernelGGL(k_cabs_cext, gc, kBlock, 0, c->stream, d, g, c->kinc, inc_phase(c), c->occ,
+                       c->alpha, c->nmat, c->einc, c->P, c->partials);
     HIPCHECK(hipGetLastError());
     std::vector<double> part((size_t)nrhs * cb * 2);
     HIPCHECK(hipMemcpyAsync(part.data(), c->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1111,10 +1155,11 @@ int job_prepare(ddafft_ctx* c, const ddafft_job* j, int solver, int max_step, co
     d.nrhs = nrhs;
     TRY(upload(c, c->alpha, j->alpha, (size_t)nrhs * c->nmat * sizeof(float2)));
     TRY(upload(c, c->einc, j->einc, (size_t)nrhs * 3 * sizeof(float2)));
-    TRY(upload(c, c->kinc, j->kinc, (size_t)nrhs * 3 * sizeof(float)));
+    TRY(upload_incidence(c, nrhs, j->kinc));
     const Geometry g = make_geometry(c, nullptr);
     dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), nrhs);
-    hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->kinc, c->occ, c->mv, c->alpha, c->nmat,
+    hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->kinc, inc_phase(c),
+                       c->occ, c->mv, c->alpha, c->nmat,
                        c->einc, c->P);
     HIPCHECK(hipGetLastError());
     return DDAFFT_OK;
@@ -1335,7 +1380,7 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
     TRY(ensure_buf(c, &c->gaux, c->gaux_cap, aux_bytes));
     TRY(upload(c, c->alpha, alpha_adj.data(), alpha_adj.size() * sizeof(float2)));
     const int naxis = d.ni + d.nj + d.nl;
-    if (ndirs > 0) {
+    if (ndirs > 0 && any_s11) {
         TRY(ensure_buf(c, &c->phase, c->phase_cap, (size_t)ndirs * naxis));
         const Geometry g = make_geometry(c, kinc);
         hipLaunchKernelGGL(k_axis_phase, (int)std::min<long long>(((long long)ndirs * naxis + kBlock - 1) / kBlock, 1024),
@@ -1358,13 +1403,13 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
         aa.cabs_c = reinterpret_cast<const float*>(c->gaux + off_ca);
         aa.u = reinterpret_cast<const float2*>(c->gaux + off_u);
         aa.tab = c->phase;
-        aa.ndirs = ndirs;
+        aa.ndirs = any_s11 ? ndirs : 0;   // no S11 weight: the far-field term of g is zero
         aa.npol = npol;
         aa.nmat = nmat;
         aa.tdirs = ndirs > 0 ? std::max(1, std::min(ndirs, kAdjLds / (naxis + 3))) : 1;
         aa.b = c->P;
         aa.partials = c->partials;
-        if (ndirs > 0 && !aa.tab) return fail(DDAFFT_ESTATE, "internal: phase table missing");
+        if (aa.ndirs > 0 && !aa.tab) return fail(DDAFFT_ESTATE, "internal: phase table missing");
         const Geometry g = make_geometry(c, kinc);
         hipLaunchKernelGGL(k_adj_source, dim3(nblk, nadj), kBlock, 0, c->stream, d, g, c->occ, c->mv, aa);
         HIPCHECK(hipGetLastError());
@@ -1469,7 +1514,7 @@ int ddafft_cabs_cext(ddafft_ctx* c, const float* alpha, const float* kinc, const
     if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
     TRY(upload(c, c->alpha, alpha, (size_t)c->nmat * sizeof(float2)));
     TRY(upload(c, c->einc, einc, 3 * sizeof(float2)));
-    TRY(upload(c, c->kinc, kinc, 3 * sizeof(float)));
+    TRY(upload_incidence(c, 1, kinc));
     TRY(field_in(c, p, c->P, c->Y));
     return observables(c, 1, einc, 0, nullptr, 0, nullptr, cabs, cext);
 }
@@ -1485,7 +1530,7 @@ int ddafft_far_field(ddafft_ctx* c, const float* p, int ndirs, const float* dirs
     const float kz[3] = {0.f, 0.f, 1.f};
     const float e[6] = {1.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     TRY(upload(c, c->einc, e, sizeof(e)));
-    TRY(upload(c, c->kinc, kz, sizeof(kz)));
+    TRY(upload_incidence(c, 1, kz));
     return observables(c, 1, e, ndirs, c->dirs, 0, esc, nullptr, nullptr);
 }
 

@@ -98,10 +98,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_source(Dims d, Geometry g, const
             if (dip) {
                 for (int q = 0; q < nt; ++q) {
                     const float2* t = tab + q * naxis;
-                    const float2 ph = cmul(cmul(t[i], t[d.ni + j]), t[d.ni + d.nj + l]);
-                    gx = cadd(gx, cmul(ph, uu[3 * q]));
-                    gy = cadd(gy, cmul(ph, uu[3 * q + 1]));
-                    gz = cadd(gz, cmul(ph, uu[3 * q + 2]));
+                    const float2 ph = cmul_fma(cmul_fma(t[i], t[d.ni + j]), t[d.ni + d.nj + l]);
+                    cmac_fma(gx, ph, uu[3 * q]);
+                    cmac_fma(gy, ph, uu[3 * q + 1]);
+                    cmac_fma(gz, ph, uu[3 * q + 2]);
                 }
             }
         }

@@ -29,10 +29,26 @@ struct Geometry {
     float ex, ey, ez;  // centre used by CabsCext: n d / 2 (dda_func.cpp:1685)
 };
 
+// Incident-wave phases exp(i k.(r - c)) of an axis-aligned incidence (the
+// reference's fixed orientation, kinc along a lattice axis) depend on one
+// coordinate only; the host tabulates them with the same float argument and
+// the C library's cosf/sinf, as the reference evaluates them
+// (complex.cpp:176-181), so they match it bit for bit.  axis[r] = -1 marks a
+// general incidence, whose phases are FP64 sincos rounded once to float.
+// tab: [nrhs][2][tn], tn = max(ni, nj, nl): FirstApproximation phases, then
+// CabsCext phases.
+struct IncPhase {
+    const int* axis;
+    const float2* tab;
+    int tn;
+};
+
+__device__ __forceinline__ int axis_coord(int ax, int i, int j, int l) { return ax == 0 ? i : (ax == 1 ? j : l); }
+
 // P0 = alpha_m exp(i k.(r - c)) E on dipoles (dda_func.cpp:654-689).
 // kinc: [nrhs][3]; einc: [nrhs][3] complex; alpha: [nrhs][nmat].
 __global__ __launch_bounds__(kBlock) void k_first_approx(Dims d, Geometry g, const float* __restrict__ kinc,
-                                                         const int8_t* __restrict__ occ,
+                                                         IncPhase ip, const int8_t* __restrict__ occ,
                                                          const int* __restrict__ mv, const float2* __restrict__ alpha,
                                                          int nmat, const float2* __restrict__ einc,
                                                          float2* __restrict__ p) {
@@ -51,7 +67,8 @@ __global__ __launch_bounds__(kBlock) void k_first_approx(Dims d, Geometry g, con
             const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
             const float dx = rx - g.cx, dy = ry - g.cy, dz = rz - g.cz;
             const float ph = kx * dx + ky * dy + kz * dz;
-            const float2 c0 = exp_i(ph);
+            const int ax = ip.axis[rhs];
+            const float2 c0 = ax >= 0 ? ip.tab[(size_t)rhs * 2 * ip.tn + axis_coord(ax, i, j, l)] : exp_i(ph);
             const float2 c1 = cmul(c0, alpha[(long long)rhs * nmat + m]);
             o0 = cmul(c1, E0);
             o1 = cmul(c1, E1);
@@ -301,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_restore(Dims d, SolveState* __restri
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const float* __restrict__ kinc,
-                                                      const int8_t* __restrict__ occ,
+                                                      IncPhase ip, const int8_t* __restrict__ occ,
                                                       const float2* __restrict__ alpha, int nmat,
                                                       const float2* __restrict__ einc, const float2* __restrict__ P,
                                                       double* __restrict__ partials) {
@@ -322,7 +339,8 @@ __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const 
         const int j = (int)(ij % d.nj), i = (int)(ij / d.nj);
         const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
         const float r3 = -(kx * (rx - g.ex) + ky * (ry - g.ey) + kz * (rz - g.ez));
-        const float2 c0 = exp_i(r3);
+        const int ax = ip.axis[rhs];
+        const float2 c0 = ax >= 0 ? ip.tab[((size_t)rhs * 2 + 1) * ip.tn + axis_coord(ax, i, j, l)] : exp_i(r3);
         const float2 px = P[rb + k], py = P[rb + d.n + k], pz = P[rb + 2 * d.n + k];
         const float2 c5 = cadd(cmul(pz, e2), cadd(cmul(px, e0), cmul(py, e1)));
         acc[0] += (double)cmul(c0, c5).y;
