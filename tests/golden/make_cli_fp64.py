@@ -1,0 +1,114 @@
+"""Writes tests/golden/<case>.fp64.npz for the seven reference CLI runs: the
+Mueller matrices, Cabs and Cext of the same sample with the far-field and
+cross-section sums taken in float64 over the same float dipoles.
+
+The dipoles come from the oracle's float solve, which reproduces the
+reference's bit for bit (and the HIP library's on axis-aligned incidence).
+The reference then sums 10^4..10^5 float terms sequentially; on strongly
+cancelling directions (backscatter) that float sum carries relative errors up
+to ~1e-2 of small Mueller elements, so the reference's own out.dat is not the
+accurate value there.  These fixtures are: geometry, polarizabilities and
+solve as the reference (main.cpp, dda_func.cpp), far field and cross sections
+in float64, Jones->Mueller from the far fields rounded to complex64 as the
+reference rounds them (mmuller.cpp:175-314 in float).
+
+    python tests/golden/make_cli_fp64.py
+"""
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "dda-fft_amd")]
+
+import ddafft  # noqa: E402  (host functions only: geometry, polarizability, targets)
+import gradient as G  # noqa: E402
+import oracle  # noqa: E402
+
+SOLVERS = {"cg": 0, "aa": 1, "bicgstab": 2, "simple": 3}
+MODELS = {"cm": 1, "cmrr": 2, "cmnr": 3, "ildr": 4, "ldr": 5}
+
+
+def parse(args):
+    a = args.split()
+    o = dict(t="sphere", g=None, aspect=(1.0, 1.0), x=None, m=None, P="cm", M="cg", e=1e-3, S=20, p=1, a=0.0,
+             b=180.0, s=10.0, tau=0.0, psi=0.0, iota=0.0, kappa=0.0)
+    i = 0
+    while i < len(a):
+        k = a[i]
+        if k == "-g":
+            v = []
+            while i + 1 < len(a) and a[i + 1].lstrip("-").replace(".", "").isdigit():
+                v.append(int(a[i + 1]))
+                i += 1
+            o["g"] = tuple(v * 3 if len(v) == 1 else v)
+        elif k == "--aspect":
+            o["aspect"] = (float(a[i + 1]), float(a[i + 2]))
+            i += 2
+        elif k == "-m":
+            o["m"] = complex(float(a[i + 1]), float(a[i + 2]))
+            i += 2
+        else:
+            o[k.lstrip("-")] = a[i + 1]
+            i += 1
+        i += 1
+    return o
+
+
+def case(name, args):
+    o = parse(args)
+    dims = o["g"]
+    x = np.float32(float(o["x"]))
+    kd = np.float32(2.0 * x / np.float32(dims[0]))
+    t = o["t"]
+    ba, ca = [float(np.float32(v)) for v in o["aspect"]]
+    if t == "sphere":
+        occ, _ = ddafft.target_ellipsoid(dims, 1.0, 1.0)
+    elif t == "ellipsoid":
+        occ, _ = ddafft.target_ellipsoid(dims, ba, ca)
+    else:
+        import ctypes as C
+        lib = ddafft.load()
+        occ = np.zeros(dims, np.int8)
+        lib.ddafft_target_parallelepiped(*dims, ba, ca, occ.ctypes.data_as(C.c_void_p))
+    npl, tmin, tstep = int(o["p"]), float(o["a"]), float(o["s"])
+    nang = int(round((float(o["b"]) - tmin) / tstep + 1))
+    dirs, normals, kinc, einc = ddafft.geometry_fixed(npl, nang, tmin, tstep, float(o["psi"]), float(o["iota"]),
+                                                      float(o["kappa"]))
+    dirs = dirs.reshape(-1, 3)
+    model, solver = MODELS[o["P"]], SOLVERS[o["M"]]
+    orc = oracle.Oracle("f32")
+    A = orc.build_ahat(dims, kd)
+    esc, cabs, cext = [], [], []
+    for p in range(2):
+        alpha = np.array([0, ddafft.polarizability(model, kd, o["m"], kinc, einc[p])], np.complex64)
+        out = orc.dda_fft(solver, occ, [0, 1], alpha, kd, kinc, einc[p], A, dirs, int(o["S"]), float(o["e"]),
+                          float(o["tau"]), return_p=True)
+        ce, cb, e = G.observables(occ, [0, 1], alpha, kd, kinc.astype(np.float64), einc[p], out["p"], dirs)
+        esc.append(e.astype(np.complex64))
+        cabs.append(cb)
+        cext.append(ce)
+    mm = np.zeros((nang, 4, 4))
+    for a in range(nang):
+        acc = np.zeros((4, 4), np.float32)
+        for pl in range(npl):
+            d = pl * nang + a
+            acc = acc + orc.muller(kinc, dirs[d], normals[pl], einc[0], einc[1], esc[0][d], esc[1][d])
+        mm[a] = acc / np.float32(npl)
+    np.savez_compressed(os.path.join(HERE, name + ".fp64.npz"), mm=mm, cabs=0.5 * (cabs[0] + cabs[1]),
+                        cext=0.5 * (cext[0] + cext[1]))
+    print(name, "done", flush=True)
+
+
+def main():
+    for name in ("sphere32", "spheroid64", "cg16", "aa16", "simple16", "ldr16", "ildr16"):
+        with open(os.path.join(HERE, name + ".steps.txt")) as f:
+            args = f.readline().split(":", 1)[1].strip()
+        case(name, args)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,182 @@
+"""The command-line program dda-fft_amd/lib/dda_fft against the reference's.
+
+CPU: option parsing, defaults, validation and exit codes (options.cpp,
+main.cpp:176-188).  GPU: out.dat and out.dst of the seven reference runs held
+in tests/golden (made by tests/golden/make_golden.py with the reference's
+dda_fft.out), run with the same arguments.
+"""
+
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CLI = os.path.join(ROOT, "dda-fft_amd", "lib", "dda_fft")
+GOLDEN = os.path.join(HERE, "golden")
+
+CASES = {}
+for name in ("sphere32", "spheroid64", "cg16", "aa16", "simple16", "ldr16", "ildr16"):
+    with open(os.path.join(GOLDEN, name + ".steps.txt")) as f:
+        CASES[name] = f.readline().split(":", 1)[1].strip()
+
+
+def run(args, cwd, timeout=300):
+    return subprocess.run([CLI] + args, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+def read_dat(path):
+    with open(path) as f:
+        head = f.readline().split()
+        cabs = float(head[0].split("=")[1])
+        cext = float(head[1].split("=")[1])
+        rows = [list(map(float, line.split())) for line in f if line.strip()]
+    a = np.array(rows)
+    return cabs, cext, a[:, 0], a[:, 1:].reshape(-1, 4, 4)
+
+
+def read_dst(path, nang):
+    b = open(path, "rb").read()
+    n = struct.unpack("<i", b[:4])[0]
+    cabs, cext = struct.unpack("<ff", b[4:12])
+    mm = np.frombuffer(b[12:], np.float32).reshape(nang, 4, 4)
+    return n, cabs, cext, mm
+
+
+@pytest.fixture(scope="module")
+def cli():
+    if not os.path.exists(CLI):
+        pytest.skip("dda_fft not built (make -C dda-fft_amd)")
+    return CLI
+
+
+# ---------------------------------------------------------------- CPU: parsing
+
+def test_help_and_version(cli, tmp_path):
+    assert run(["--help"], tmp_path).returncode == 0
+    assert run(["-V"], tmp_path).returncode == 0
+
+
+@pytest.mark.parametrize("args,msg", [
+    ([], "no options"),
+    (["-t", "nosuch", "-g", "16", "-x", "2", "-m", "1.5", "0.01"], "unknown target"),
+    (["-t", "sphere", "-g", "12", "-x", "2", "-m", "1.5", "0.01"], "powers of two"),
+    (["-t", "sphere", "-g", "16", "-x", "2"], "exactly one --material"),
+    (["-t", "sphere", "-g", "16", "-x", "2", "-m", "1.5", "0.01", "-M", "simple"], "tau"),
+    (["-t", "random-ellipsoid", "-g", "16", "-x", "2", "-m", "1.5", "0.01", "--aspect", "1", "1"], "--seeds"),
+    (["-t", "bisphere", "-g", "16", "-x", "2", "-m", "1.5", "0.01"], "exactly two"),
+    (["-t", "file", "-g", "16", "-x", "2", "-m", "1.5", "0.01"], "--shape-file"),
+    (["-t", "anthrax", "-g", "16", "-x", "2", "-m", "1.5", "0.01", "--packing", "0.1"], "--monomer-radius"),
+    (["-t", "sphere", "-g", "16", "16", "-x", "2", "-m", "1.5", "0.01"], "1 or 3 values"),
+    (["-t", "sphere", "-g", "16", "-x", "2", "-m", "1.5", "0.01", "-o", "a", "-D", "a"], "different files"),
+    (["-t", "sphere", "-g", "16", "-x", "2", "-m", "1.5", "0.01", "-P", "7"], "polarizability"),
+    (["-t", "sphere", "-g", "16", "-x", "2", "-m", "1.5", "0.01", "-a", "90", "-b", "10"], "theta-max"),
+    (["-t", "sphere", "-g", "16", "-x", "2", "-m", "1.5", "0.01", "--seed", "-4x"], "--seed"),
+])
+def test_validation_errors_exit_2(cli, tmp_path, args, msg):
+    r = run(args, tmp_path)
+    assert r.returncode == 2, r.stdout + r.stderr
+    assert msg in r.stderr
+
+
+def test_state_file_errors(cli, tmp_path):
+    # an unreadable (short) state file: exit 4 as main.cpp:243-262
+    (tmp_path / "bad.dst").write_bytes(b"\x01\x00")
+    r = run(["-t", "sphere", "-g", "8", "-x", "1", "-m", "1.5", "0.01", "-D", "bad.dst"], tmp_path)
+    assert r.returncode == 4
+
+
+# ---------------------------------------------------------------- GPU: results
+
+def dat_error(mm, rmm):
+    """The reference's own metric (bench/cmp_dat.py): the largest relative
+    error |a - b| / max(|a|, |b|) over the significant elements, those at least
+    1e-3 of the largest magnitude in the table."""
+    floor = 1e-3 * np.abs(rmm).max()
+    sig = (np.abs(rmm) >= floor) | (np.abs(mm) >= floor)
+    rel = np.abs(mm - rmm) / np.maximum(np.maximum(np.abs(mm), np.abs(rmm)), 1e-300)
+    return float(rel[sig].max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CASES))
+def test_out_dat_matches_reference(cli, tmp_path, name):
+    """out.dat/out.dst of the reference's own runs, with the same arguments.
+
+    The truth is tests/golden/<case>.fp64.npz: the reference's float dipoles
+    with the far-field and cross-section sums in float64.  The reference's
+    out.dat carries the error of its sequential float sums, up to 3.4e-3 on
+    small backscatter Mueller elements (spheroid64); this program sums in
+    FP64.  Checked: Mueller matrices within 1e-4 of the truth on the
+    significant elements, within the reference's own error + 1e-4 of the
+    reference's out.dat, Cabs and Cext within 2e-5 of both."""
+    args = CASES[name].split() + ["-j", "1", "--seed", "1", "-o", "out.dat", "-D", "out.dst"]
+    print("case", name, flush=True)
+    r = run(args, tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    ca, ce, th, mm = read_dat(tmp_path / "out.dat")
+    rca, rce, rth, rmm = read_dat(os.path.join(GOLDEN, name + ".out.dat"))
+    truth = np.load(os.path.join(GOLDEN, name + ".fp64.npz"))
+    assert np.array_equal(th, rth)
+    e_truth = dat_error(mm, truth["mm"])
+    e_ref = dat_error(mm, rmm)
+    e_ref_truth = dat_error(rmm, truth["mm"])
+    print(name, "Mueller worst rel: vs fp64 %.2e, vs reference %.2e (reference vs fp64 %.2e)" % (
+        e_truth, e_ref, e_ref_truth), "Cabs", ca, rca, float(truth["cabs"]), "Cext", ce, rce, float(truth["cext"]),
+          flush=True)
+    assert e_truth < 1e-4
+    assert e_ref < e_ref_truth + 1e-4
+    for v, rv, tv in ((ca, rca, truth["cabs"]), (ce, rce, truth["cext"])):
+        assert abs(v / rv - 1) < 2e-5 and abs(v / tv - 1) < 2e-5, (v, rv, float(tv))
+    n, dca, dce, dmm = read_dst(tmp_path / "out.dst", len(rth))
+    rn, rdca, rdce, rdmm = read_dst(os.path.join(GOLDEN, name + ".out.dst"), len(rth))
+    assert n == rn == 1
+    assert abs(dca / rdca - 1) < 2e-5 and abs(dce / rdce - 1) < 2e-5
+    assert dat_error(dmm.astype(np.float64), truth["mm"]) < 1e-4
+
+
+@pytest.mark.gpu
+def test_resume_and_mc_samples(cli, tmp_path):
+    """State-file resume (main.cpp:214-262): 2 samples, then resumed to 3; a
+    fixed target with Monte-Carlo orientations batched on the GPU."""
+    base = ["-t", "ellipsoid", "-g", "16", "--aspect", "1", "0.5", "-x", "3", "-m", "1.5", "0.01", "-M", "bicgstab",
+            "-e", "1e-4", "-S", "50", "-O", "mc", "-p", "2", "-s", "30", "--seed", "7", "-o", "o.dat", "-D", "o.dst"]
+    r = run(base + ["-n", "2"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    n2, ca2, ce2, mm2 = read_dst(tmp_path / "o.dst", 7)
+    assert n2 == 2
+    r = run(base + ["-n", "3"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    n3, ca3, ce3, mm3 = read_dst(tmp_path / "o.dst", 7)
+    assert n3 == 3 and ca3 > ca2 and ce3 > ce2
+    # already complete: nothing recomputed, out.dat rewritten from the state
+    r = run(base + ["-n", "3"], tmp_path)
+    assert r.returncode == 0
+    assert read_dst(tmp_path / "o.dst", 7)[0] == 3
+    ca, ce, th, mm = read_dat(tmp_path / "o.dat")
+    assert abs(ca - ca3 / 3) < 1e-6 * abs(ca) + 1e-8
+    # the same three samples in one run, batched (group 3) and one by one (group 1)
+    one = run(base[:-4] + ["-o", "a.dat", "-D", "a.dst", "-n", "3", "--group", "1"], tmp_path)
+    three = run(base[:-4] + ["-o", "b.dat", "-D", "b.dst", "-n", "3", "--group", "3"], tmp_path)
+    assert one.returncode == 0 and three.returncode == 0
+    assert (tmp_path / "a.dst").read_bytes() == (tmp_path / "b.dst").read_bytes()
+
+
+@pytest.mark.gpu
+def test_shape_file_target(cli, tmp_path):
+    """Target 11: a shape file written from a target, read back; out of range -> exit 3."""
+    import ddafft
+    occ, cnt = ddafft.target_ellipsoid((16, 16, 16), 1.0, 1.0)
+    lines = ["%d" % cnt] + ["%d %d %d 1" % (i, j, l) for i, j, l in np.argwhere(occ > 0)]
+    (tmp_path / "shape.txt").write_text("\n".join(lines) + "\n")
+    common = ["-g", "16", "-x", "3", "-m", "1.5", "0.01", "-M", "bicgstab", "-e", "1e-4", "-S", "40", "-s", "30"]
+    r1 = run(["-t", "file", "-f", "shape.txt", "-o", "f.dat", "-D", "f.dst"] + common, tmp_path)
+    r2 = run(["-t", "sphere", "-o", "s.dat", "-D", "s.dst"] + common, tmp_path)
+    assert r1.returncode == 0 and r2.returncode == 0, r1.stderr + r2.stderr
+    assert (tmp_path / "f.dat").read_text() == (tmp_path / "s.dat").read_text()
+    (tmp_path / "bad.txt").write_text("1\n20 0 0 1\n")
+    r3 = run(["-t", "file", "-f", "bad.txt", "-o", "g.dat", "-D", "g.dst"] + common, tmp_path)
+    assert r3.returncode == 3
