@@ -28,6 +28,7 @@ EXPORTS = (
     "ddafft_target_random_ellipsoid", "ddafft_target_random_parallelepiped", "ddafft_target_nibble_ellipsoid",
     "ddafft_target_hierarchical_ellipsoid", "ddafft_target_sphere_spec", "ddafft_worm_count", "ddafft_target_worms",
     "ddafft_target_file_read", "ddafft_target_file_write", "ddafft_orientation_mc", "ddafft_rotate",
+    "ddafft_target_mixture", "ddafft_agglomerate_count", "ddafft_target_agglomerate",
 )
 
 
@@ -75,6 +76,10 @@ def load(path=LIB_PATH):
     lib.ddafft_rotate.argtypes = [P, I, P, P]
     lib.ddafft_rng_worker_seed.argtypes = [C.c_ulonglong, I]
     lib.ddafft_rng_next.argtypes = [P]
+    D = C.c_double
+    lib.ddafft_target_mixture.argtypes = [I, P, I, D, P, P]
+    lib.ddafft_agglomerate_count.argtypes = [I, I, I, D, D]
+    lib.ddafft_target_agglomerate.argtypes = [I, I, I, D, D, C.c_longlong, P, P, P]
     lib.ddafft_polarizability.argtypes = [I, F, P, P, P, P]
     lib.ddafft_polarizability_deriv.argtypes = [I, F, P, P, P, P]
     D = C.c_double
@@ -361,6 +366,37 @@ def target_ellipsoid(dims, b_a=1.0, c_a=1.0):
     if cnt < 0:
         raise DdafftError("ellipsoid does not fit the lattice")
     return occ, cnt
+
+
+def target_mixture(base, material, fraction, state):
+    """Each cell of material 1 in `base` becomes `material` with probability
+    `fraction` (ddafft_target_mixture); `state` is a uint64[1] stream state,
+    advanced.  Returns (occ, changed)."""
+    load()
+    base = np.ascontiguousarray(base, np.int8)
+    occ = np.zeros_like(base)
+    n = _lib.ddafft_target_mixture(int(base.size), _p(base), int(material), float(fraction), _p(state), _p(occ))
+    if n < 0:
+        raise DdafftError("bad mixture arguments")
+    return occ, n
+
+
+def target_agglomerate(dims, r_mono, packing, state, max_attempts=10 ** 7):
+    """Random-packed agglomerate of sphere monomers (ddafft_target_agglomerate).
+    Returns (occ, count, centres[K, 3])."""
+    load()
+    k = _lib.ddafft_agglomerate_count(*[int(v) for v in dims], float(r_mono), float(packing))
+    if k < 0:
+        raise DdafftError("bad agglomerate arguments")
+    occ = np.zeros(tuple(dims), np.int8)
+    cen = np.zeros((k, 3), np.float64)
+    n = _lib.ddafft_target_agglomerate(*[int(v) for v in dims], float(r_mono), float(packing), int(max_attempts),
+                                       _p(state), _p(occ), _p(cen))
+    if n == -3:
+        raise DdafftError("agglomerate placement gave up after %d draws" % max_attempts)
+    if n < 0:
+        raise DdafftError("bad agglomerate arguments")
+    return occ, n, cen
 
 
 def geometry_fixed(nplanes, nangles, theta_min, theta_step, psi=0.0, iota=0.0, kappa=0.0):

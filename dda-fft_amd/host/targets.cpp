@@ -9,6 +9,7 @@
 // With the same seed the occupancy maps are therefore identical to the
 // reference's (particle.cpp:21-995, randomize.cpp:44-58, main.cpp:875-922).
 // Compiled with -ffp-contract=off.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -577,6 +578,97 @@ int ddafft_target_file_write(const char* path, int ni, int nj, int nl, const sig
             }
     std::fclose(f);
     return n;
+}
+
+// ---- generators of the benchmark workloads that the reference has no target for.
+// They draw from the same xorshift64* stream; the arithmetic is double.
+
+int ddafft_target_mixture(int n, const signed char* base, int material, double fraction,
+                          unsigned long long* state, signed char* occ) {
+    if (n < 0 || !base || !occ || !state || material < 1 || material > 64) return -2;
+    if (*state == 0) *state = 0x9E3779B97F4A7C15ULL;
+    Stream rng{state};
+    int changed = 0;
+    for (int k = 0; k < n; ++k) {
+        occ[k] = base[k];
+        if (base[k] == 1) {
+            const double u = (double)rng.next() / 2147483647.0;
+            if (u < fraction) {
+                occ[k] = (signed char)material;
+                ++changed;
+            }
+        }
+    }
+    return changed;
+}
+
+// consecutive rejected centre draws after which a packing starts over
+const long long kAggStall = 100000;
+
+int ddafft_agglomerate_count(int ni, int nj, int nl, double r_mono, double packing) {
+    if (ni < 1 || nj < 1 || nl < 1 || !(r_mono > 0.) || !(packing > 0.)) return -2;
+    const double R = 0.5 * (double)std::min(ni, std::min(nj, nl));
+    const double ratio = R / r_mono;
+    return (int)std::floor(packing * ratio * ratio * ratio + 0.5);
+}
+
+int ddafft_target_agglomerate(int ni, int nj, int nl, double r_mono, double packing, long long max_attempts,
+                              unsigned long long* state, signed char* occ, double* centres) {
+    const int K = ddafft_agglomerate_count(ni, nj, nl, r_mono, packing);
+    if (K < 0 || !state || !occ) return -2;
+    if (*state == 0) *state = 0x9E3779B97F4A7C15ULL;
+    Stream rng{state};
+    const double R = 0.5 * (double)std::min(ni, std::min(nj, nl));
+    const double Rc = R - r_mono;
+    if (!(Rc > 0.)) return -2;
+    const double cx = 0.5 * (ni - 1), cy = 0.5 * (nj - 1), cz = 0.5 * (nl - 1);
+    std::vector<double> c((size_t)3 * K);
+    long long attempts = 0, stall = 0;
+    const double dmin2 = 4. * r_mono * r_mono;
+    for (int k = 0; k < K;) {
+        if (max_attempts > 0 && ++attempts > max_attempts) return -3;
+        // a packing jammed short of K (no free room left for a centre) starts over
+        if (++stall > kAggStall) {
+            k = 0;
+            stall = 0;
+        }
+        double v[3];
+        for (double& q : v) q = Rc * (2. * ((double)rng.next() / 2147483647.0) - 1.);
+        if (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > Rc * Rc) continue;
+        const double x = cx + v[0], y = cy + v[1], z = cz + v[2];
+        bool ok = true;
+        for (int q = 0; q < k && ok; ++q) {
+            const double dx = x - c[3 * q], dy = y - c[3 * q + 1], dz = z - c[3 * q + 2];
+            ok = dx * dx + dy * dy + dz * dz >= dmin2;
+        }
+        if (!ok) continue;
+        c[3 * k] = x;
+        c[3 * k + 1] = y;
+        c[3 * k + 2] = z;
+        ++k;
+        stall = 0;
+    }
+    const double r2 = r_mono * r_mono;
+    std::memset(occ, 0, (size_t)ni * nj * nl);
+    int count = 0;
+    for (int k = 0; k < K; ++k) {
+        const double x = c[3 * k], y = c[3 * k + 1], z = c[3 * k + 2];
+        const int i0 = std::max(0, (int)std::floor(x - r_mono)), i1 = std::min(ni - 1, (int)std::ceil(x + r_mono));
+        const int j0 = std::max(0, (int)std::floor(y - r_mono)), j1 = std::min(nj - 1, (int)std::ceil(y + r_mono));
+        const int l0 = std::max(0, (int)std::floor(z - r_mono)), l1 = std::min(nl - 1, (int)std::ceil(z + r_mono));
+        for (int i = i0; i <= i1; ++i)
+            for (int j = j0; j <= j1; ++j)
+                for (int l = l0; l <= l1; ++l) {
+                    const double dx = i - x, dy = j - y, dz = l - z;
+                    signed char& o = occ[at(i, j, l, nj, nl)];
+                    if (dx * dx + dy * dy + dz * dz <= r2 && o == 0) {
+                        o = 1;
+                        ++count;
+                    }
+                }
+    }
+    if (centres) std::memcpy(centres, c.data(), c.size() * sizeof(double));
+    return count;
 }
 
 }  // extern "C"

@@ -216,6 +216,31 @@ int ddafft_worm_count(int ni, int nj, int nl, double rho, double radius_worm, do
 int ddafft_target_worms(int kind, int ni, int nj, int nl, double radius_worm, double length_worm, int nworms,
                         long long max_attempts, unsigned long long* state, signed char* occ);
 
+/* Generators of benchmark workloads the reference has no target for; their
+ * maps reach the reference through the target 11 shape file.
+ *   mixture       occ[k] = base[k], and every cell with base[k] == 1 becomes
+ *                 `material` when its draw u = next / (2^31 - 1) is below
+ *                 `fraction` (one draw per such cell, in storage order).
+ *                 Returns the number of cells changed.
+ *   agglomerate   ddafft_agglomerate_count(...) = round(packing (R / r_mono)^3)
+ *                 non-overlapping spheres of radius r_mono (cells) with
+ *                 centres drawn uniformly in the ball of radius R - r_mono,
+ *                 R = min(ni, nj, nl) / 2, about the lattice centre
+ *                 ((ni - 1) / 2, ...), by rejection: each centre takes three
+ *                 draws c = (R - r_mono)(2u - 1) per axis and is kept when it
+ *                 lies in that ball and at least 2 r_mono from every kept one;
+ *                 after 100000 consecutive rejected draws (a jammed packing)
+ *                 all kept centres are dropped and placement starts over.
+ *                 A cell is material 1 when within r_mono of a centre.
+ *                 Returns the occupied count, -3 after max_attempts draws of a
+ *                 centre (max_attempts > 0).  centres, when not NULL, receives
+ *                 the 3 * ddafft_agglomerate_count(...) centre coordinates. */
+int ddafft_target_mixture(int n, const signed char* base, int material, double fraction,
+                          unsigned long long* state, signed char* occ);
+int ddafft_agglomerate_count(int ni, int nj, int nl, double r_mono, double packing);
+int ddafft_target_agglomerate(int ni, int nj, int nl, double r_mono, double packing, long long max_attempts,
+                              unsigned long long* state, signed char* occ, double* centres);
+
 /* Target 11 shape file (main.cpp:645-677): the count N, then N lines
  * "X Y Z T".  read returns N, -3 for a cell outside the lattice (the
  * reference's exit code 3), -4 for a short file; write lists the nonzero cells. */

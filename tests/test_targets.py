@@ -181,3 +181,54 @@ def test_worm_attempt_cap(lib):
     occ = np.zeros((8, 8, 8), np.int8)
     st = np.array([5], np.uint64)
     assert lib.ddafft_target_worms(6, 8, 8, 8, 1.5, 3.0, 500, 20000, _p(st), _p(occ)) == -3
+
+
+# ---- the two benchmark generators without a reference target (mixture of the
+# adjoint workload, agglomerates of the batched Monte-Carlo workload)
+
+def _gen():
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
+    import generators
+    return generators
+
+
+@pytest.mark.parametrize("seed", [7, 2024])
+def test_mixture_bit_exact(seed):
+    gen = _gen()
+    base, _ = ddafft.target_ellipsoid((32, 32, 64), 1.0, 2.0)
+    st = np.array([seed], np.uint64)
+    occ, n = ddafft.target_mixture(base, 2, 0.2, st)
+    ref, s_end = gen.mixture(base, 2, 0.2, seed)
+    assert np.array_equal(occ, ref) and int(st[0]) == s_end
+    n1 = int((base == 1).sum())
+    assert n == int((occ == 2).sum())
+    assert abs(n - 0.2 * n1) < 5 * np.sqrt(n1 * 0.2 * 0.8)
+    assert np.array_equal(occ != 0, base != 0)
+
+
+@pytest.mark.parametrize("dims,seed", [((32, 32, 32), 1), ((32, 32, 64), 99), ((64, 64, 64), 5),
+                                       ((32, 32, 64), 0xEA52A8A33F56EC11)])   # this one jams and starts over
+def test_agglomerate(dims, seed):
+    gen = _gen()
+    st = np.array([seed], np.uint64)
+    occ, cnt, cen = ddafft.target_agglomerate(dims, 3.0, 0.3, st)
+    ref, rcen, s_end = gen.agglomerate(dims, 3.0, 0.3, seed)
+    assert np.array_equal(occ, ref) and np.array_equal(cen, rcen) and int(st[0]) == s_end
+    # invariants: count, inside the ball, disjoint, union of digital spheres
+    K = gen.agglomerate_count(dims, 3.0, 0.3)
+    assert len(cen) == K == int(np.floor(0.3 * (min(dims) / 6.0) ** 3 + 0.5))
+    c0 = (np.array(dims) - 1) / 2.0
+    assert np.all(np.linalg.norm(cen - c0, axis=1) <= min(dims) / 2.0 - 3.0)
+    d = np.linalg.norm(cen[:, None] - cen[None], axis=2) + np.eye(K) * 100
+    assert d.min() >= 6.0
+    assert cnt == int(occ.sum())
+    # disjoint monomers: each covers the same number of cells up to its lattice offset
+    per = cnt / K
+    assert 100 <= per <= 130   # digital ball of radius 3: 4/3 pi 27 = 113 +- boundary
+
+
+def test_agglomerate_attempt_cap():
+    st = np.array([3], np.uint64)
+    with pytest.raises(ddafft.DdafftError):
+        ddafft.target_agglomerate((16, 16, 16), 3.0, 0.9, st, max_attempts=10000)
