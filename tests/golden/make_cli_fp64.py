@@ -12,6 +12,11 @@ solve as the reference (main.cpp, dda_func.cpp), far field and cross sections
 in float64, Jones->Mueller from the far fields rounded to complex64 as the
 reference rounds them (mmuller.cpp:175-314 in float).
 
+The second set, <case>.f64solve.npz, is the whole sample in FP64: the
+oracle's double build (kd, polarizabilities, A-hat, FFTs, solver, far field,
+Cabs/Cext, Jones->Mueller all in double) on the same occupancy and geometry.
+It gives the error of the float path against an FP64 build of the same solve.
+
     python tests/golden/make_cli_fp64.py
 """
 
@@ -103,11 +108,61 @@ def case(name, args):
     print(name, "done", flush=True)
 
 
+def case_f64(name, args):
+    o = parse(args)
+    dims = o["g"]
+    kd = 2.0 * float(o["x"]) / float(dims[0])
+    t = o["t"]
+    ba, ca = [float(np.float32(v)) for v in o["aspect"]]
+    if t == "sphere":
+        occ, _ = ddafft.target_ellipsoid(dims, 1.0, 1.0)
+    elif t == "ellipsoid":
+        occ, _ = ddafft.target_ellipsoid(dims, ba, ca)
+    else:
+        import ctypes as C
+        lib = ddafft.load()
+        occ = np.zeros(dims, np.int8)
+        lib.ddafft_target_parallelepiped(*dims, ba, ca, occ.ctypes.data_as(C.c_void_p))
+    npl, tmin, tstep = int(o["p"]), float(o["a"]), float(o["s"])
+    nang = int(round((float(o["b"]) - tmin) / tstep + 1))
+    dirs, normals, kinc, einc = ddafft.geometry_fixed(npl, nang, tmin, tstep, float(o["psi"]), float(o["iota"]),
+                                                      float(o["kappa"]))
+    dirs = dirs.reshape(-1, 3).astype(np.float64)
+    normals = normals.astype(np.float64)
+    kinc = kinc.astype(np.float64)
+    einc = einc.astype(np.complex128)
+    model, solver = MODELS[o["P"]], SOLVERS[o["M"]]
+    orc = oracle.Oracle("f64")
+    A = orc.build_ahat(dims, kd)
+    esc, cabs, cext, steps = [], [], [], []
+    for p in range(2):
+        a = orc.polarizability(model, kd, o["m"], kinc, np.stack([einc[p].real, einc[p].imag], 1))
+        alpha = np.array([0, a], np.complex128)
+        out = orc.dda_fft(solver, occ, [0, 1], alpha, kd, kinc, einc[p], A, dirs, int(o["S"]), float(o["e"]),
+                          float(o["tau"]))
+        esc.append(out["esc"])
+        cabs.append(out["cabs"])
+        cext.append(out["cext"])
+        steps.append(out["steps"])
+    mm = np.zeros((nang, 4, 4))
+    for a in range(nang):
+        acc = np.zeros((4, 4))
+        for pl in range(npl):
+            d = pl * nang + a
+            acc = acc + orc.muller(kinc, dirs[d], normals[pl], einc[0], einc[1], esc[0][d], esc[1][d])
+        mm[a] = acc / npl
+    np.savez_compressed(os.path.join(HERE, name + ".f64solve.npz"), mm=mm, cabs=0.5 * (cabs[0] + cabs[1]),
+                        cext=0.5 * (cext[0] + cext[1]), steps=np.array(steps))
+    print(name, "f64 solve done, steps", steps, flush=True)
+
+
 def main():
     for name in ("sphere32", "spheroid64", "cg16", "aa16", "simple16", "ldr16", "ildr16"):
         with open(os.path.join(HERE, name + ".steps.txt")) as f:
             args = f.readline().split(":", 1)[1].strip()
-        case(name, args)
+        if "--f64-solve-only" not in sys.argv:
+            case(name, args)
+        case_f64(name, args)
 
 
 if __name__ == "__main__":

@@ -112,7 +112,8 @@ def test_out_dat_matches_reference(cli, tmp_path, name):
     small backscatter Mueller elements (spheroid64); this program sums in
     FP64.  Checked: Mueller matrices within 1e-4 of the truth on the
     significant elements, within the reference's own error + 1e-4 of the
-    reference's out.dat, Cabs and Cext within 2e-5 of both."""
+    reference's out.dat, Cabs and Cext within 2e-5 of both; and no further
+    from the FP64 build of the whole sample than the reference is."""
     args = CASES[name].split() + ["-j", "1", "--seed", "1", "-o", "out.dat", "-D", "out.dst"]
     print("case", name, flush=True)
     r = run(args, tmp_path)
@@ -131,6 +132,15 @@ def test_out_dat_matches_reference(cli, tmp_path, name):
     assert e_ref < e_ref_truth + 1e-4
     for v, rv, tv in ((ca, rca, truth["cabs"]), (ce, rce, truth["cext"])):
         assert abs(v / rv - 1) < 2e-5 and abs(v / tv - 1) < 2e-5, (v, rv, float(tv))
+    # against the FP64 build of the whole sample (tests/golden/make_cli_fp64.py):
+    # never further from it than the reference's own float result
+    f64 = np.load(os.path.join(GOLDEN, name + ".f64solve.npz"))
+    e64, r64 = dat_error(mm, f64["mm"]), dat_error(rmm, f64["mm"])
+    c64 = max(abs(ca / f64["cabs"] - 1), abs(ce / f64["cext"] - 1))
+    rc64 = max(abs(rca / f64["cabs"] - 1), abs(rce / f64["cext"] - 1))
+    print(name, "error vs the FP64 solve: Mueller %.2e (reference %.2e), Cabs/Cext %.2e (reference %.2e)" % (
+        e64, r64, c64, rc64), flush=True)
+    assert e64 <= r64 + 1e-5 and c64 <= rc64 + 1e-5
     n, dca, dce, dmm = read_dst(tmp_path / "out.dst", len(rth))
     rn, rdca, rdce, rdmm = read_dst(os.path.join(GOLDEN, name + ".out.dst"), len(rth))
     assert n == rn == 1
