@@ -130,6 +130,7 @@ struct ddafft_ctx {
 
     // target
     int8_t* occ = nullptr;
+    int8_t* dipf = nullptr;            // [n] 1 where the cell holds a dipole (mv[occ] != 0)
     int* mv = nullptr;
     int nmat = 0;
     int* cells = nullptr;   // dipole cell indices (far field)
@@ -170,6 +171,14 @@ struct ddafft_ctx {
     int esc_cap = 0;
     int* stop_dev = nullptr;
     int* stop_host = nullptr;          // pinned ring
+    hipEvent_t fin_ev = nullptr;       // marks the results of a batched solve as copied back
+    // pinned staging of the per-solve transfers, so that they are truly
+    // asynchronous: uploads are packed at up_off (reset when a solve starts,
+    // with the stream idle), results come back into dn
+    char* up = nullptr;
+    size_t up_cap = 0, up_off = 0;
+    char* dn = nullptr;
+    size_t dn_cap = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     // opt-in per-pass timing of every matvec (ddafft_profile)
@@ -337,7 +346,7 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, const SolveState* 
     {
         const int tj = std::min(tile_zf(d.nl), d.nj);
         dim3 grid(d.ni * (d.nj / tj), C);
-        DDA_SWITCH_SMALL(d.nl, hipLaunchKernelGGL((k_zf<NN>), grid, kBlock, 0, s, d, pa, st, iter, c->occ, c->mv, c->b1, c->wpre_l,
+        DDA_SWITCH_SMALL(d.nl, hipLaunchKernelGGL((k_zf<NN>), grid, kBlock, 0, s, d, pa, st, iter, c->dipf, c->b1, c->wpre_l,
                                                                   c->tw_l));
     }
     if (ev) HIPCHECK(hipEventRecord(ev[1], s));
@@ -369,7 +378,7 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, const SolveState* 
     if (ev) HIPCHECK(hipEventRecord(ev[4], s));
     {
         dim3 grid(zi_blocks(d), nrhs);
-        DDA_SWITCH_LARGE(2 * d.nl, hipLaunchKernelGGL((k_zi<NN>), grid, kBlock, 0, s, d, st, iter, c->b1, ea, c->occ, c->mv,
+        DDA_SWITCH_LARGE(2 * d.nl, hipLaunchKernelGGL((k_zi<NN>), grid, kBlock, 0, s, d, st, iter, c->b1, ea, c->occ, c->dipf,
                                                                       c->tw_2l));
     }
     if (ev) HIPCHECK(hipEventRecord(ev[5], s));
@@ -453,9 +462,47 @@ int prof_collect(ddafft_ctx* c, int last_iter = 1 << 30) {
     return DDAFFT_OK;
 }
 
+// Host-to-device copy on the context's stream.  Packed into the pinned
+// staging buffer while it has room (a pinned source keeps the copy
+// asynchronous), otherwise from the caller's memory.
 int upload(ddafft_ctx* c, void* dst, const void* src, size_t bytes) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (c->up && c->up_off + need <= c->up_cap) {
+        char* q = c->up + c->up_off;
+        std::memcpy(q, src, bytes);
+        c->up_off += need;
+        HIPCHECK(hipMemcpyAsync(dst, q, bytes, hipMemcpyHostToDevice, c->stream));
+        return DDAFFT_OK;
+    }
+    // unstaged: complete the copy before the caller may reuse its memory
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
     return DDAFFT_OK;
+}
+
+// (Re)allocates pinned host memory of at least `bytes`; waits for the stream
+// first, since pending copies may use the old buffer.
+int pinned_reserve(ddafft_ctx* c, char** p, size_t* cap, size_t bytes) {
+    if (*p && *cap >= bytes) return DDAFFT_OK;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (*p) hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const size_t n = std::max<size_t>(bytes, 1 << 20);
+    if (hipHostMalloc((void**)p, n, hipHostMallocDefault) != hipSuccess) {
+        *p = nullptr;
+        return fail(DDAFFT_ENOMEM, "hipHostMalloc of pinned staging failed");
+    }
+    *cap = n;
+    return DDAFFT_OK;
+}
+
+// Starts the uploads of one solve: the stream is idle then (every solve ends
+// with a synchronization), so the staging buffer is free to reuse.
+int upload_begin(ddafft_ctx* c, size_t bytes) {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->up_off = 0;
+    return pinned_reserve(c, &c->up, &c->up_cap, bytes);
 }
 
 #define TRY(x)                  \
@@ -508,7 +555,7 @@ int inc_tab_len(const ddafft_ctx* c) { return std::max(c->d.ni, std::max(c->d.nj
 IncPhase inc_phase(const ddafft_ctx* c) { return IncPhase{c->inc_axis, c->inc_tab, inc_tab_len(c)}; }
 
 int upload_incidence(ddafft_ctx* c, int nrhs, const float* kinc) {
-    HIPCHECK(hipStreamSynchronize(c->stream));   // the staging vectors are reused
+    if (!c->up) HIPCHECK(hipStreamSynchronize(c->stream));   // unstaged copies read the host vectors below
     const Geometry g = make_geometry(c, nullptr);
     c->inc_axis_h.assign(nrhs, -1);
     const int tn = inc_tab_len(c);
@@ -568,15 +615,18 @@ int ddafft_version(void) { return 100; }
 int ddafft_destroy(ddafft_ctx* c) {
     if (!c) return DDAFFT_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->ahat, c->tables, c->occ, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
+    void* bufs[] = {c->ahat, c->tables, c->occ, c->dipf, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
                     c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
                     c->dirs, c->esc, c->stop_dev, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->stop_host) hipHostFree(c->stop_host);
+    if (c->up) hipHostFree(c->up);
+    if (c->dn) hipHostFree(c->dn);
     for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
     for (hipEvent_t& e : c->ev)
         if (e) hipEventDestroy(e);
+    if (c->fin_ev) hipEventDestroy(c->fin_ev);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return DDAFFT_OK;
@@ -605,6 +655,7 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
     chk(hipGetDevice(&c->device), "hipGetDevice");
     chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (hipEvent_t& e : c->ev) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    chk(hipEventCreateWithFlags(&c->fin_ev, hipEventDisableTiming), "hipEventCreate");
     if (rc) {
         ddafft_destroy(c);
         return rc;
@@ -627,6 +678,7 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
     if (!rc) rc = dev_alloc(&c->inc_tab, (size_t)nrhs_max * 2 * inc_tab_len(c));
     if (!rc) rc = dev_alloc(&c->st, (size_t)nrhs_max);
     if (!rc) rc = dev_alloc(&c->occ, (size_t)n);
+    if (!rc) rc = dev_alloc(&c->dipf, (size_t)n);
     if (!rc) rc = dev_alloc(&c->stop_dev, 1);
     if (!rc && hipHostMalloc((void**)&c->stop_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         rc = fail(DDAFFT_ENOMEM, "hipHostMalloc");
@@ -716,6 +768,9 @@ int ddafft_set_target(ddafft_ctx* c, const signed char* occ, int nmat, const int
     c->ncells = (int)cells.size();
     c->ndip = (long long)cells.size();
     if ((rc = upload(c, c->occ, occ, (size_t)n))) return rc;
+    std::vector<int8_t> dipf((size_t)n);
+    for (long long k = 0; k < n; ++k) dipf[k] = mv[occ[k]] ? 1 : 0;
+    if ((rc = upload(c, c->dipf, dipf.data(), (size_t)n))) return rc;
     if ((rc = upload(c, c->mv, mv, (size_t)nmat * sizeof(int)))) return rc;
     if ((rc = upload(c, c->cells, cells.data(), cells.size() * sizeof(int)))) return rc;
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -810,7 +865,7 @@ struct Seq {
         if (!ua.P || !ua.R || !ua.S || !ua.W || !ua.partials || (ua.kind == UK_BICG && !ua.G))
             return fail(DDAFFT_ESTATE, "internal: update launched with a missing buffer");
         dim3 g(update_blocks(d), nrhs);
-        hipLaunchKernelGGL(k_update, g, kBlock, 0, c->stream, d, c->st, iter, ua, c->occ, c->mv);
+        hipLaunchKernelGGL(k_update, g, kBlock, 0, c->stream, d, c->st, iter, ua, c->dipf);
         HIPCHECK(hipGetLastError());
         return DDAFFT_OK;
     }
@@ -913,42 +968,72 @@ __global__ void k_stop_max(const SolveState* st, int nrhs, int* out) {
     *out = m;
 }
 
+// Results of a solve on their way back to the host: offsets into the
+// context's pinned download buffer, filled by stream-ordered copies, read
+// after the stream (or an event after the copies) has completed.
+struct Fin {
+    int nrhs = 0, ndirs = 0, cb = 0, fchunk = 0, max_step = 0;
+    bool state = false, hist = false;
+    size_t o_st = 0, o_hist = 0, o_part = 0, o_esc = 0;
+};
+
+// Sizes the downloads and reserves the pinned buffer (before anything of this
+// solve is queued to it).
+int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int ndirs, Fin& f) {
+    f = Fin{};
+    f.nrhs = nrhs;
+    f.ndirs = ndirs;
+    f.state = state;
+    f.hist = state && hist;
+    f.max_step = max_step;
+    f.cb = (int)std::min<long long>((c->d.n + kBlock - 1) / kBlock, 512);
+    const int dtiles = (ndirs + kFD - 1) / kFD;
+    // dipole chunks per direction tile: enough blocks to fill the chip
+    f.fchunk = ndirs > 0 ? (int)std::max(1, std::min({64, (c->ncells + 4095) / 4096,
+                                                      (2048 + dtiles - 1) / std::max(dtiles, 1)})) : 0;
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        const size_t o = off;
+        off += (b + 255) & ~(size_t)255;
+        return o;
+    };
+    if (f.state) f.o_st = take((size_t)nrhs * sizeof(SolveState));
+    if (f.hist) f.o_hist = take((size_t)nrhs * (max_step + 1) * sizeof(double));
+    f.o_part = take((size_t)nrhs * f.cb * 2 * sizeof(double));
+    if (ndirs > 0) f.o_esc = take((size_t)nrhs * ndirs * 3 * sizeof(float2));
+    TRY(ensure_partials(c, std::max<size_t>((size_t)nrhs * f.cb * 2, (size_t)nrhs * f.fchunk * std::max(ndirs, 1) * 6)));
+    if (ndirs > 0 && nrhs * ndirs * 3 > c->esc_cap) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->esc) hipFree(c->esc);
+        c->esc = nullptr;
+        c->esc_cap = 0;
+        TRY(dev_alloc(&c->esc, (size_t)nrhs * ndirs * 3));
+        c->esc_cap = nrhs * ndirs * 3;
+    }
+    return pinned_reserve(c, &c->dn, &c->dn_cap, off);
+}
+
 // Cross sections and far fields of the nrhs solutions in c->P; c->kinc and
 // c->einc hold each right-hand side's incidence.  Right-hand side r uses the
-// directions ddirs + r * dir_stride.
-int observables(ddafft_ctx* c, int nrhs, const float* einc, int ndirs, const float* ddirs, long long dir_stride,
-                float* esc, float* cabs, float* cext) {
+// directions ddirs + r * dir_stride.  Queues the kernels and the copies of
+// their results into the pinned buffer.
+int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_stride) {
+    const int nrhs = f.nrhs, ndirs = f.ndirs;
     Dims d = c->d;
     d.nrhs = nrhs;
     const Geometry g = make_geometry(c, nullptr);
-    // cross sections
-    const int cb = (int)std::min<long long>((d.n + kBlock - 1) / kBlock, 512);
-    // dipole chunks per direction tile: enough blocks to fill the chip
-    const int dtiles = (ndirs + kFD - 1) / kFD;
-    const int fchunk = ndirs > 0 ? (int)std::max(1, std::min({64, (c->ncells + 4095) / 4096,
-                                                              (2048 + dtiles - 1) / std::max(dtiles, 1)})) : 0;
-    TRY(ensure_partials(c, std::max<size_t>((size_t)nrhs * cb * 2, (size_t)nrhs * fchunk * std::max(ndirs, 1) * 6)));
-    dim3 gc(cb, nrhs);
+    dim3 gc(f.cb, nrhs);
     hipLaunchKernelGGL(k_cabs_cext, gc, kBlock, 0, c->stream, d, g, c->kinc, inc_phase(c), c->occ,
                        c->alpha, c->nmat, c->einc, c->P, c->partials);
     HIPCHECK(hipGetLastError());
-    std::vector<double> part((size_t)nrhs * cb * 2);
-    HIPCHECK(hipMemcpyAsync(part.data(), c->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // copied before the far field reuses the partials (stream order)
+    HIPCHECK(hipMemcpyAsync(c->dn + f.o_part, c->partials, (size_t)nrhs * f.cb * 2 * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
     if (ndirs > 0) {
-        if (nrhs * ndirs * 3 > c->esc_cap) {
-            HIPCHECK(hipStreamSynchronize(c->stream));
-            if (c->esc) hipFree(c->esc);
-            c->esc = nullptr;
-            c->esc_cap = 0;
-            TRY(dev_alloc(&c->esc, (size_t)nrhs * ndirs * 3));
-            c->esc_cap = nrhs * ndirs * 3;
-        }
-        // partials for cabs were copied above (stream-ordered), so reuse is safe
-        const int tot = nrhs * ndirs * 3;
         // right-hand sides in pairs that share one direction set share the phases
         for (int r0 = 0; r0 < nrhs;) {
             const bool pair = r0 + 1 < nrhs && (dir_stride == 0 || c->dir_pairs[r0 / 2]);
-            dim3 gf(fchunk, (ndirs + kFD - 1) / kFD);
+            dim3 gf(f.fchunk, (ndirs + kFD - 1) / kFD);
             const float* dd = ddirs + r0 * dir_stride;
             if (pair)
                 hipLaunchKernelGGL((k_far_field<2>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, dd, ndirs,
@@ -960,21 +1045,39 @@ int observables(ddafft_ctx* c, int nrhs, const float* einc, int ndirs, const flo
             r0 += pair ? 2 : 1;
         }
         hipLaunchKernelGGL(k_far_field_sum, (nrhs * ndirs + kBlock - 1) / kBlock, kBlock, 0, c->stream, c->partials,
-                           fchunk, ndirs, nrhs, ddirs, dir_stride, c->esc);
+                           f.fchunk, ndirs, nrhs, ddirs, dir_stride, c->esc);
         HIPCHECK(hipGetLastError());
-        HIPCHECK(hipMemcpyAsync(esc, c->esc, (size_t)tot * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipMemcpyAsync(c->dn + f.o_esc, c->esc, (size_t)nrhs * ndirs * 3 * sizeof(float2),
+                                hipMemcpyDeviceToHost, c->stream));
     }
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    for (int r = 0; r < nrhs; ++r) {
+    return DDAFFT_OK;
+}
+
+// After the copies have completed: Cabs, Cext (fixed-order FP64 sums of the
+// block partials) and the far fields.
+void obs_complete(ddafft_ctx* c, const Fin& f, const float* einc, float* esc, float* cabs, float* cext) {
+    const double* part = reinterpret_cast<const double*>(c->dn + f.o_part);
+    for (int r = 0; r < f.nrhs; ++r) {
         double s1 = 0, s2 = 0;
-        for (int b = 0; b < cb; ++b) {
-            s1 += part[((size_t)r * cb + b) * 2];
-            s2 += part[((size_t)r * cb + b) * 2 + 1];
+        for (int b = 0; b < f.cb; ++b) {
+            s1 += part[((size_t)r * f.cb + b) * 2];
+            s2 += part[((size_t)r * f.cb + b) * 2 + 1];
         }
         const float r0 = cabs_prefactor(einc + 6 * r);
         if (cext) cext[r] = r0 * (float)s1;
         if (cabs) cabs[r] = r0 * (float)s2;
     }
+    if (f.ndirs > 0 && esc) std::memcpy(esc, c->dn + f.o_esc, (size_t)f.nrhs * f.ndirs * 3 * sizeof(float2));
+}
+
+// Standalone cross sections / far fields (one synchronization).
+int observables(ddafft_ctx* c, int nrhs, const float* einc, int ndirs, const float* ddirs, long long dir_stride,
+                float* esc, float* cabs, float* cext) {
+    Fin f;
+    TRY(fin_layout(c, nrhs, false, false, 0, ndirs, f));
+    TRY(obs_enqueue(c, f, ddirs, dir_stride));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    obs_complete(c, f, einc, esc, cabs, cext);
     return DDAFFT_OK;
 }
 
@@ -1050,28 +1153,36 @@ int run_wait(Run& r) {
     return DDAFFT_OK;
 }
 
-// best-iterate restore, final states, history and counters
-int run_finish(Run& r, double* err_hist, int* info, std::vector<SolveState>& sf) {
+// best-iterate restore, then the copies of the final states (and history)
+int fin_state_enqueue(Run& r, const Fin& f) {
     ddafft_ctx* c = r.c;
-    const int nrhs = r.nrhs, max_step = r.max_step;
     Dims d = c->d;
-    d.nrhs = nrhs;
+    d.nrhs = r.nrhs;
     {
-        dim3 grid((int)std::min<long long>((3 * d.n + kBlock - 1) / kBlock, 2048), nrhs);
+        dim3 grid((int)std::min<long long>((3 * d.n + kBlock - 1) / kBlock, 2048), r.nrhs);
         hipLaunchKernelGGL(k_restore, grid, kBlock, 0, c->stream, d, c->st, c->P, c->Pmin);
         HIPCHECK(hipGetLastError());
     }
-    sf.resize(nrhs);
-    HIPCHECK(hipMemcpyAsync(sf.data(), c->st, nrhs * sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpyAsync(c->dn + f.o_st, c->st, r.nrhs * sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
+    if (f.hist)
+        HIPCHECK(hipMemcpyAsync(c->dn + f.o_hist, c->hist, (size_t)r.nrhs * (r.max_step + 1) * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+    return DDAFFT_OK;
+}
+
+// final states, history and counters, once the copies have completed
+int fin_state_complete(Run& r, const Fin& f, double* err_hist, int* info, std::vector<SolveState>& sf) {
+    ddafft_ctx* c = r.c;
+    const int nrhs = r.nrhs, max_step = r.max_step;
+    sf.assign(reinterpret_cast<const SolveState*>(c->dn + f.o_st),
+              reinterpret_cast<const SolveState*>(c->dn + f.o_st) + nrhs);
     if (c->prof) {
         int last = 0;
         for (int q = 0; q < nrhs; ++q) last = std::max(last, sf[q].last);
         TRY(prof_collect(c, last));
     }
-    if (err_hist) {
-        std::vector<double> h((size_t)nrhs * (max_step + 1));
-        HIPCHECK(hipMemcpy(h.data(), c->hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (err_hist && f.hist) {
+        const double* h = reinterpret_cast<const double*>(c->dn + f.o_hist);
         for (int q = 0; q < nrhs; ++q)
             for (int o = 0; o <= max_step; ++o)
                 err_hist[(size_t)q * (max_step + 1) + o] = o <= sf[q].last ? h[(size_t)q * (max_step + 1) + o] : NAN;
@@ -1093,6 +1204,14 @@ int run_finish(Run& r, double* err_hist, int* info, std::vector<SolveState>& sf)
         }
     }
     return DDAFFT_OK;
+}
+
+int run_finish(Run& r, double* err_hist, int* info, std::vector<SolveState>& sf) {
+    Fin f;
+    TRY(fin_layout(r.c, r.nrhs, true, err_hist != nullptr, r.max_step, 0, f));
+    TRY(fin_state_enqueue(r, f));
+    HIPCHECK(hipStreamSynchronize(r.c->stream));
+    return fin_state_complete(r, f, err_hist, info, sf);
 }
 
 int solve_core(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, double* err_hist,
@@ -1123,6 +1242,13 @@ int job_prepare(ddafft_ctx* c, const ddafft_job* j, int solver, int max_step, co
     if (solver < 0 || solver > 3 || max_step < 0) return fail(DDAFFT_EINVAL, "bad solver or max_step");
     const int ndirs = j->ndirs;
     if (ndirs < 0 || (ndirs > 0 && !j->esc)) return fail(DDAFFT_EINVAL, "bad direction arguments");
+    {
+        const size_t dir_bytes = (ndirs > 0 && j->dirs) ? (size_t)ndirs * 3 * sizeof(float) * (j->dirs_per_rhs ? nrhs : 1) : 0;
+        const size_t bytes = dir_bytes + (size_t)nrhs * (c->nmat * sizeof(float2) + 3 * sizeof(float2) +
+                                                         3 * sizeof(float) + sizeof(int) +
+                                                         2 * (size_t)inc_tab_len(c) * sizeof(float2) + sizeof(SolveState));
+        TRY(upload_begin(c, bytes + 16 * 256));
+    }
     *ddirs = nullptr;
     *dir_stride = 0;
     if (ndirs > 0) {
@@ -1172,9 +1298,27 @@ int ddafft_solve_ex(ddafft_ctx* c, const ddafft_job* job, int solver, float tau,
     const float* ddirs = nullptr;
     long long stride = 0;
     TRY(job_prepare(c, job, solver, max_step, &ddirs, &stride));
+    Run r;
+    r.c = c;
+    r.nrhs = job->nrhs;
+    r.solver = solver;
+    r.tau = tau;
+    r.max_step = max_step;
+    r.err_max = err_max;
+    TRY(run_begin(r));
+    while (run_more(r)) {
+        TRY(run_enqueue(r));
+        TRY(run_wait(r));
+    }
+    Fin f;
+    TRY(fin_layout(c, job->nrhs, true, err_hist != nullptr, max_step, job->ndirs, f));
+    TRY(fin_state_enqueue(r, f));
+    TRY(obs_enqueue(c, f, ddirs, stride));
+    HIPCHECK(hipStreamSynchronize(c->stream));
     std::vector<SolveState> sf;
-    TRY(solve_core(c, job->nrhs, solver, tau, max_step, err_max, err_hist, job->info, sf));
-    return observables(c, job->nrhs, job->einc, job->ndirs, ddirs, stride, job->esc, job->cabs, job->cext);
+    TRY(fin_state_complete(r, f, err_hist, job->info, sf));
+    obs_complete(c, f, job->einc, job->esc, job->cabs, job->cext);
+    return DDAFFT_OK;
 }
 
 int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max, const float* alpha,
@@ -1212,6 +1356,8 @@ int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, i
         int job = -1;
         const float* ddirs = nullptr;
         long long stride = 0;
+        Fin f;
+        bool finishing = false;   // results queued back, event recorded
     };
     std::vector<Slot> act;
     int next = 0;
@@ -1233,19 +1379,44 @@ int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, i
     while (!act.empty()) {
         // one iteration for every instance still running, then the stop flags
         for (Slot& s : act)
-            if (run_more(s.r)) TRY(run_enqueue(s.r));
-        for (Slot& s : act) TRY(run_wait(s.r));
+            if (!s.finishing && run_more(s.r)) TRY(run_enqueue(s.r));
+        for (Slot& s : act)
+            if (!s.finishing) TRY(run_wait(s.r));
+        // instances that are done queue their results back without waiting
+        for (Slot& s : act) {
+            if (s.finishing || run_more(s.r)) continue;
+            const ddafft_job& j = jobs[s.job];
+            TRY(fin_layout(s.r.c, j.nrhs, true, false, max_step, j.ndirs, s.f));
+            TRY(fin_state_enqueue(s.r, s.f));
+            TRY(obs_enqueue(s.r.c, s.f, s.ddirs, s.stride));
+            HIPCHECK(hipEventRecord(s.r.c->fin_ev, s.r.c->stream));
+            s.finishing = true;
+        }
+        // collect what has arrived; block only when nothing else can progress
+        bool all_finishing = true;
+        for (const Slot& s : act) all_finishing = all_finishing && s.finishing;
         for (size_t k = 0; k < act.size();) {
-            if (run_more(act[k].r)) {
+            Slot& s = act[k];
+            if (!s.finishing) {
                 ++k;
                 continue;
             }
-            Slot s = act[k];
-            act.erase(act.begin() + (long)k);
+            hipError_t q = hipEventQuery(s.r.c->fin_ev);
+            if (q == hipErrorNotReady && all_finishing) {
+                HIPCHECK(hipEventSynchronize(s.r.c->fin_ev));
+                q = hipSuccess;
+                all_finishing = false;   // one wait per round
+            }
+            if (q == hipErrorNotReady) {
+                ++k;
+                continue;
+            }
+            HIPCHECK(q);
             const ddafft_job& j = jobs[s.job];
             std::vector<SolveState> sf;
-            TRY(run_finish(s.r, nullptr, j.info, sf));
-            TRY(observables(s.r.c, j.nrhs, j.einc, j.ndirs, s.ddirs, s.stride, j.esc, j.cabs, j.cext));
+            TRY(fin_state_complete(s.r, s.f, nullptr, j.info, sf));
+            obs_complete(s.r.c, s.f, j.einc, j.esc, j.cabs, j.cext);
+            act.erase(act.begin() + (long)k);
             if (next < n) TRY(start(next++));
         }
     }

@@ -79,6 +79,13 @@ __host__ __device__ constexpr int pow2floor(int v) {
 }
 __host__ __device__ constexpr int clamp16(int v) { return v < 1 ? 1 : (v > 16 ? 16 : v); }
 
+// product of the radices of the stages before stage t
+__host__ __device__ constexpr int span_before(int n, int t) {
+    int h = 1;
+    for (int u = 0; u < t; ++u) h *= plan_radix(ilog2c(n), u);
+    return h;
+}
+
 template <int N>
 struct Plan {
     static constexpr int K = ilog2c(N);
@@ -181,16 +188,16 @@ struct LineWalk {
 template <int N>
 __device__ __forceinline__ int rest_natural(int g) {
     constexpr int K = Plan<N>::K;
-    int j = 0, rem = g, L = plan_radix(K, 0);
+    unsigned j = 0, rem = (unsigned)g, L = plan_radix(K, 0);
 #pragma unroll
     for (int t = 1; t < Plan<N>::NST; ++t) {
-        const int r = plan_radix(K, t);
+        const unsigned r = plan_radix(K, t);
         L *= r;
-        const int d = rem % r;
+        const unsigned d = rem % r;
         rem /= r;
         j += d * (N / L);
     }
-    return j;
+    return (int)j;
 }
 
 template <int N, int T>
@@ -206,16 +213,19 @@ struct StageRunner {
             constexpr int NB = N / R;
             int b = w.b0, line = w.l0;
             while (b < NB) {
-                const int g = b / H;
-                const int i = b - g * H;
-                float2* v = s + line * ls + g * span + i;
+                const unsigned g = (unsigned)b / (unsigned)H;   // H, span, step: powers of two
+                const int i = b - (int)g * H;
+                float2* v = s + line * ls + (int)g * span + i;
                 float2 z[R];
 #pragma unroll
                 for (int q = 0; q < R; ++q) z[q] = v[q * H];
                 if (R > 2 && i > 0) {
+                    const int ts = i * step;
+                    int tq = 0;
 #pragma unroll
                     for (int q = 1; q < R; ++q) {
-                        const float2 tv = tw[q * i * step];
+                        tq += ts;   // q * i * step
+                        const float2 tv = tw[tq];
                         const float tr = z[q].x * tv.x - z[q].y * tv.y;
                         z[q].y = z[q].x * tv.y + z[q].y * tv.x;
                         z[q].x = tr;
@@ -232,10 +242,95 @@ struct StageRunner {
     }
 };
 
+// The same transforms for a line count fixed at compile time: every lane's
+// (line, butterfly) pairs are constants of its thread index, the loops unroll
+// completely and the index arithmetic reduces to shifts, masks and constant
+// multiplies.  Same lane mapping, same operations in the same order.
+template <int N, int NL, int T>
+struct FixedStage {
+    static __device__ __forceinline__ void run(float2* s, int ls, const float2* __restrict__ tw) {
+        if constexpr (T < Plan<N>::NST) {
+            constexpr int K = Plan<N>::K;
+            constexpr int R = plan_radix(K, T);
+            constexpr int H = span_before(N, T);
+            constexpr int span = H * R;
+            constexpr int step = N / span;
+            constexpr int W = NL * (N / R);
+            constexpr int IT = (W + kBlock - 1) / kBlock;
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const unsigned f = threadIdx.x + it * kBlock;
+                if (W % kBlock == 0 || f < (unsigned)W) {
+                    const unsigned line = f % NL, b = f / NL;
+                    const unsigned g = b / H, i = b % H;
+                    float2* v = s + line * ls + g * span + i;
+                    float2 z[R];
+#pragma unroll
+                    for (int q = 0; q < R; ++q) z[q] = v[q * H];
+                    if (R > 2 && i > 0) {
+                        const unsigned ts = i * step;
+                        unsigned tq = 0;
+#pragma unroll
+                        for (int q = 1; q < R; ++q) {
+                            tq += ts;   // q * i * step
+                            const float2 tv = tw[tq];
+                            const float tr = z[q].x * tv.x - z[q].y * tv.y;
+                            z[q].y = z[q].x * tv.y + z[q].y * tv.x;
+                            z[q].x = tr;
+                        }
+                    }
+                    butterfly<R>(z);
+#pragma unroll
+                    for (int q = 0; q < R; ++q) v[q * H] = z[q];
+                }
+            }
+            __syncthreads();
+            FixedStage<N, NL, T + 1>::run(s, ls, tw);
+        }
+    }
+};
+
+template <int N, int NL>
+__device__ __forceinline__ void fft_lds_fixed(float2* s, int ls, const float2* __restrict__ tw) {
+    constexpr int K = Plan<N>::K;
+    constexpr int R0 = plan_radix(K, 0);
+    constexpr int W = NL * (N / R0);
+    constexpr int IT = (W + kBlock - 1) / kBlock;
+    float2 z[IT][R0];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const unsigned f = threadIdx.x + it * kBlock;
+        if (W % kBlock == 0 || f < (unsigned)W) {
+            const unsigned line = f % NL, b = f / NL;
+            const float2* v = s + line * ls + rest_natural<N>(b);
+#pragma unroll
+            for (int q = 0; q < R0; ++q) z[it][q] = v[q * (N / R0)];
+            butterfly<R0>(z[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const unsigned f = threadIdx.x + it * kBlock;
+        if (W % kBlock == 0 || f < (unsigned)W) {
+            const unsigned line = f % NL, b = f / NL;
+            float2* v = s + line * ls + b * R0;
+#pragma unroll
+            for (int q = 0; q < R0; ++q) v[q] = z[it][q];
+        }
+    }
+    __syncthreads();
+    FixedStage<N, NL, 1>::run(s, ls, tw);
+}
+
 // MAXL: the most lines any call of this instance passes (sizes the registers
 // that carry stage 0 across its barrier).  Ends with a __syncthreads.
 template <int N, int MAXL>
 __device__ __forceinline__ void fft_lds(float2* s, int nlines, int ls, const float2* __restrict__ tw) {
+    if (nlines == MAXL) {   // the usual case: full tiles
+        fft_lds_fixed<N, MAXL>(s, ls, tw);
+        return;
+    }
     constexpr int K = Plan<N>::K;
     constexpr int R0 = plan_radix(K, 0);
     constexpr int NB0 = N / R0;

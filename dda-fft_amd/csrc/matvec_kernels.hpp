@@ -57,46 +57,18 @@ struct ProArgs {
     float2* Pmin;       // snapshot destination (P_min, dda_func.cpp:1305-1313)
 };
 
-__device__ __forceinline__ float2 prologue(const ProArgs& pa, float2 c1, float2 c2, long long ck, bool dip) {
-    switch (pa.mode) {
-        case PRO_LC1: {
-            float2 a = pa.A[ck];
-            if (dip) {
-                a = cadd(a, cmul(c1, pa.B[ck]));
-                pa.A[ck] = a;
-            }
-            return a;
-        }
-        case PRO_LC4: {
-            float2 v = make_float2(0.f, 0.f);
-            if (dip) v = cadd(cmul(c2, pa.D[ck]), cadd(pa.A[ck], cmul(c1, pa.B[ck])));
-            pa.B[ck] = v;
-            return v;
-        }
-        case PRO_LC2: {
-            float2 v = pa.B[ck];
-            if (dip) {
-                v = cadd(pa.A[ck], cmul(c1, v));
-                pa.B[ck] = v;
-            }
-            return v;
-        }
-        default:
-            return pa.A[ck];
-    }
-}
-
 // ---------------------------------------------------------------------------
 // ZF: lines (c, i, j) along l; n_l inputs -> 2 n_l outputs.
 // ---------------------------------------------------------------------------
 
 template <int NL>
 __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveState* __restrict__ st, int iter,
-                                               const int8_t* __restrict__ occ, const int* __restrict__ mv,
-                                               float2* __restrict__ b1, const float2* __restrict__ wpre,
-                                               const float2* __restrict__ tw) {
+                                               const int8_t* __restrict__ dipf, float2* __restrict__ b1,
+                                               const float2* __restrict__ wpre, const float2* __restrict__ tw) {
     constexpr int TJ = TileZF<NL>::T;
     constexpr int LS = NL + kLpad;
+    constexpr int LNL = ilog2c(NL);
+    constexpr int NLD = (TJ * NL + kBlock - 1) / kBlock;
     __shared__ float2 s[2 * TJ * LS];
     __shared__ float2 stw[NL];
     const int c = blockIdx.y, rhs = c / 3;
@@ -112,25 +84,74 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
     const int tj = min(TJ, d.nj), ntile = d.nj / tj;
     const int i = blockIdx.x / ntile, j0 = (blockIdx.x - i * ntile) * tj;
     const long long cell0 = ((long long)i * d.nj + j0) * NL;
-    const long long cb = (long long)c * d.n;
-    for (int f = threadIdx.x; f < tj * NL; f += blockDim.x) {
-        const long long k = cell0 + f;
-        const bool dip = mv[occ[k]] != 0;
-        const float2 v = prologue(pa, c1, c2, cb + k, dip);
-        if (snap) pa.Pmin[cb + k] = pa.P[cb + k];
-        const int t = f / NL, l = f - t * NL;
-        s[t * LS + l] = v;
-        s[(tj + t) * LS + l] = cmul(wpre[l], v);
+    const long long cb = (long long)c * d.n + cell0;   // this block's cells of component c
+    const int nld = tj * NL;
+    // every load of the thread is issued before the first is used: flags and
+    // the prologue's operands (the prologue writes only cells it has read)
+    bool dip[NLD];
+    float2 va[NLD], vb[NLD], vd[NLD], vp[NLD];
+    const int mode = pa.mode;
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+        const int f = threadIdx.x + q * kBlock;
+        dip[q] = false;
+        va[q] = vb[q] = vd[q] = vp[q] = make_float2(0.f, 0.f);
+        if (f < nld) {
+            dip[q] = dipf[cell0 + f] != 0;
+            va[q] = pa.A[cb + f];
+            if (mode != PRO_COPY) vb[q] = pa.B[cb + f];
+            if (mode == PRO_LC4) vd[q] = pa.D[cb + f];
+            if (snap) vp[q] = pa.P[cb + f];   // P is never a prologue output
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+        const int f = threadIdx.x + q * kBlock;
+        if (f < nld) {
+            float2 v;
+            switch (mode) {
+                case PRO_LC1:   // dda_func.cpp:884-905
+                    v = va[q];
+                    if (dip[q]) {
+                        v = cadd(va[q], cmul(c1, vb[q]));
+                        pa.A[cb + f] = v;
+                    }
+                    break;
+                case PRO_LC4:   // :843-882
+                    v = dip[q] ? cadd(cmul(c2, vd[q]), cadd(va[q], cmul(c1, vb[q]))) : make_float2(0.f, 0.f);
+                    pa.B[cb + f] = v;
+                    break;
+                case PRO_LC2:   // :906-924
+                    v = vb[q];
+                    if (dip[q]) {
+                        v = cadd(va[q], cmul(c1, vb[q]));
+                        pa.B[cb + f] = v;
+                    }
+                    break;
+                default:
+                    v = va[q];
+            }
+            if (snap) pa.Pmin[cb + f] = vp[q];
+            const int t = f >> LNL, l = f & (NL - 1);
+            s[t * LS + l] = v;
+            s[(tj + t) * LS + l] = cmul(wpre[l], v);
+        }
     }
     for (int f = threadIdx.x; f < NL; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
     fft_lds<NL, 2 * TJ>(s, 2 * tj, LS, stw);
     const int ltj = __builtin_ctz(tj);
     float2* dst = b1 + ((long long)c * d.ni + i) * (2 * NL) * d.nj + j0;
-    for (int f = threadIdx.x; f < 2 * NL * tj; f += blockDim.x) {
-        const int L = f >> ltj, t = f - (L << ltj);
-        const float2 v = (L & 1) ? s[(tj + t) * LS + (L >> 1)] : s[t * LS + (L >> 1)];
-        dst[(long long)L * d.nj + t] = v;
+    const int lnj = __builtin_ctz(d.nj);
+    constexpr int NST = (2 * NL * TJ + kBlock - 1) / kBlock;
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int f = threadIdx.x + q * kBlock;
+        if (f < 2 * NL * tj) {
+            const int L = f >> ltj, t = f - (L << ltj);
+            const float2 v = (L & 1) ? s[(tj + t) * LS + (L >> 1)] : s[t * LS + (L >> 1)];
+            dst[(L << lnj) + t] = v;
+        }
     }
 }
 
@@ -148,23 +169,43 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
     __shared__ float2 stw[NJ];
     const int c = blockIdx.y, rhs = c / 3;
     if (st && iter >= st[rhs].stop_at) return;
+    constexpr int LNJ = ilog2c(NJ);
     const int NL2 = 2 * d.nl;
     const int ti = min(TI, d.ni), nit = d.ni / ti;
     const int L = blockIdx.x / nit, i0 = (blockIdx.x - L * nit) * ti;
-    for (int f = threadIdx.x; f < ti * NJ; f += blockDim.x) {
-        const int t = f / NJ, jj = f - t * NJ;
-        const float2 v = b1[(((long long)c * d.ni + i0 + t) * NL2 + L) * NJ + jj];
-        s[t * LS + jj] = v;
-        s[(ti + t) * LS + jj] = cmul(wpre[jj], v);
+    const int lnl2 = __builtin_ctz(NL2), lni = __builtin_ctz(d.ni), lti = __builtin_ctz(ti);
+    {
+        constexpr int NLD = (TI * NJ + kBlock - 1) / kBlock;
+        const float2* src = b1 + (((long long)c * d.ni + i0) * NL2 + L) * NJ;
+        float2 v[NLD];
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < ti * NJ) v[q] = src[((f >> LNJ) << (lnl2 + LNJ)) + (f & (NJ - 1))];
+        }
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < ti * NJ) {
+                const int t = f >> LNJ, jj = f & (NJ - 1);
+                s[t * LS + jj] = v[q];
+                s[(ti + t) * LS + jj] = cmul(wpre[jj], v[q]);
+            }
+        }
     }
     for (int f = threadIdx.x; f < NJ; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
     fft_lds<NJ, 2 * TI>(s, 2 * ti, LS, stw);
-    const int lti = __builtin_ctz(ti);
-    for (int f = threadIdx.x; f < 2 * NJ * ti; f += blockDim.x) {
-        const int J = f >> lti, t = f - (J << lti);
-        const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
-        b2[(((long long)c * 2 * NJ + J) * NL2 + L) * d.ni + i0 + t] = v;
+    float2* dst = b2 + ((long long)c * 2 * NJ * NL2 + L) * d.ni + i0;
+    constexpr int NST = (2 * NJ * TI + kBlock - 1) / kBlock;
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int f = threadIdx.x + q * kBlock;
+        if (f < 2 * NJ * ti) {
+            const int J = f >> lti, t = f - (J << lti);
+            const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
+            dst[(J << (lnl2 + lni)) + t] = v;
+        }
     }
 }
 
@@ -238,14 +279,33 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     }
 
     const int ltj = __builtin_ctz(tj);
-    for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
-        const int ii = f % NI, r = f / NI, t = r & (tj - 1), cc = r >> ltj;
-        const int c = rhs0 * 3 + cc;
-        float2 v = make_float2(0.f, 0.f);
-        if (act[cc / 3]) v = b2[(((long long)c * NJ2 + J0 + t) * NL2 + L) * NI + ii];
-        const int line = (cc * tj + t) * 2;
-        s[line * LS1 + ii] = v;
-        s[(line + 1) * LS1 + ii] = cmul(wpre[ii], v);
+    constexpr int LNI = ilog2c(NI);
+    {
+        // all loads of this thread are issued before the first is used; the
+        // block's lines lie at 32-bit offsets from one 64-bit base
+        constexpr int NLD = (CB * TJ * NI + kBlock - 1) / kBlock;
+        const int nld = CB * tj * NI;
+        const float2* src = b2 + ((long long)(rhs0 * 3) * NJ2 + J0) * ((long long)NL2 * NI) + (long long)L * NI;
+        const int lcs = __builtin_ctz(NJ2) + __builtin_ctz(NL2) + LNI, lts = __builtin_ctz(NL2) + LNI;
+        float2 v[NLD];
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            v[q] = make_float2(0.f, 0.f);
+            if (f < nld) {
+                const int ii = f & (NI - 1), r = f >> LNI, t = r & (tj - 1), cc = r >> ltj;
+                if (act[RB == 1 ? 0 : cc / 3]) v[q] = src[(cc << lcs) + (t << lts) + ii];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < nld) {
+                const int ii = f & (NI - 1), line = (f >> LNI) * 2;   // (cc * tj + t) * 2
+                s[line * LS1 + ii] = v[q];
+                s[(line + 1) * LS1 + ii] = cmul(wpre[ii], v[q]);
+            }
+        }
     }
     for (int f = threadIdx.x; f < NI; f += blockDim.x) stw1[f] = tw1[f];
     for (int f = threadIdx.x; f < NI2; f += blockDim.x) stw2[f] = tw2[f];
@@ -283,11 +343,19 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     }
     __syncthreads();
     fft_lds<NI2, CB * TJ>(s, CB * tj, LS2, stw2);
-    for (int f = threadIdx.x; f < CB * tj * NI; f += blockDim.x) {
-        const int t = f & (tj - 1), r = f >> ltj, i = r % NI, cc = r / NI;
-        if (!act[cc / 3]) continue;
-        const int c = rhs0 * 3 + cc;
-        b3[(((long long)c * d.ni + i) * NL2 + L) * NJ2 + J0 + t] = cconj(s[(cc * tj + t) * LS2 + i]);
+    {
+        float2* dst = b3 + (long long)(rhs0 * 3) * NI * ((long long)NL2 * NJ2) + (long long)L * NJ2 + J0;
+        const int lrow = __builtin_ctz(NL2) + __builtin_ctz(NJ2);
+        constexpr int NST = (CB * TJ * NI + kBlock - 1) / kBlock;
+        const int nst = CB * tj * NI;
+#pragma unroll
+        for (int q = 0; q < NST; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < nst) {
+                const int t = f & (tj - 1), r = f >> ltj, i = r & (NI - 1), cc = r >> LNI;
+                if (act[RB == 1 ? 0 : cc / 3]) dst[(r << lrow) + t] = cconj(s[(cc * tj + t) * LS2 + i]);
+            }
+        }
     }
 }
 
@@ -308,20 +376,36 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
     const int NL2 = 2 * d.nl;
     const int tl = min(TL, NL2), nlt = NL2 / tl;
     const int i = blockIdx.x / nlt, L0 = (blockIdx.x - i * nlt) * tl;
+    constexpr int LNJ2 = ilog2c(NJ2);
     const float2* src = b3 + (((long long)c * d.ni + i) * NL2 + L0) * NJ2;
-    for (int f = threadIdx.x; f < tl * NJ2; f += blockDim.x) {
-        const int t = f / NJ2, jj = f - t * NJ2;
-        s[t * LS + jj] = cconj(src[f]);
+    {
+        constexpr int NLD = (TL * NJ2 + kBlock - 1) / kBlock;
+        float2 v[NLD];
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < tl * NJ2) v[q] = src[f];
+        }
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < tl * NJ2) s[(f >> LNJ2) * LS + (f & (NJ2 - 1))] = cconj(v[q]);
+        }
     }
     for (int f = threadIdx.x; f < NJ2; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
     fft_lds<NJ2, TL>(s, tl, LS, stw);
-    const int nj = NJ2 / 2;
-    const int ltl = __builtin_ctz(tl);
-    float2* dst = b4 + (((long long)c * d.ni + i) * nj) * NL2 + L0;
-    for (int f = threadIdx.x; f < tl * nj; f += blockDim.x) {
-        const int j = f >> ltl, t = f - (j << ltl);
-        dst[(long long)j * NL2 + t] = cconj(s[t * LS + j]);
+    constexpr int NJ = NJ2 / 2;
+    const int ltl = __builtin_ctz(tl), lnl2 = __builtin_ctz(NL2);
+    float2* dst = b4 + (((long long)c * d.ni + i) * NJ) * NL2 + L0;
+    constexpr int NST = (TL * NJ + kBlock - 1) / kBlock;
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int f = threadIdx.x + q * kBlock;
+        if (f < tl * NJ) {
+            const int j = f >> ltl, t = f - (j << ltl);
+            dst[(j << lnl2) + t] = cconj(s[t * LS + j]);
+        }
     }
 }
 
@@ -373,10 +457,14 @@ __device__ __forceinline__ double norm_term(float2 a) { return (double)(a.x * a.
 template <int NL2>
 __global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restrict__ st, int iter,
                                                const float2* __restrict__ b4, EpiArgs ea,
-                                               const int8_t* __restrict__ occ, const int* __restrict__ mv,
+                                               const int8_t* __restrict__ occ, const int8_t* __restrict__ dipf,
                                                const float2* __restrict__ tw) {
     constexpr int TJ = TileZI<NL2>::T;
     constexpr int LS = NL2 + kLpad;
+    constexpr int NLH = NL2 / 2;
+    constexpr int LNL2 = ilog2c(NL2), LNLH = LNL2 - 1;
+    constexpr int NLD = (3 * TJ * NL2 + kBlock - 1) / kBlock;
+    constexpr int NEP = (TJ * NLH + kBlock - 1) / kBlock;
     __shared__ float2 s[3 * TJ * LS];
     __shared__ float2 stw[NL2];
     __shared__ double red[kSlots * 16];
@@ -386,38 +474,82 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restr
         if (iter >= st[rhs].stop_at) return;
         rB = st[rhs].rB;
     }
-    const int nl = NL2 / 2;
     const int tj = min(TJ, d.nj), njt = d.nj / tj;
     const int i = blockIdx.x / njt, j0 = (blockIdx.x - i * njt) * tj;
-    const int ltj = __builtin_ctz(tj);
-    for (int f = threadIdx.x; f < 3 * tj * NL2; f += blockDim.x) {
-        const int LL = f % NL2, r = f / NL2, t = r & (tj - 1), comp = r >> ltj;
-        const int c = rhs * 3 + comp;
-        const float2 v = b4[(((long long)c * d.ni + i) * d.nj + j0 + t) * NL2 + LL];
-        s[(comp * tj + t) * LS + LL] = cconj(v);
+    const int ltj = __builtin_ctz(tj), lnj = __builtin_ctz(d.nj), lni = __builtin_ctz(d.ni);
+    {
+        const float2* src = b4 + (((long long)rhs * 3 * d.ni + i) * d.nj + j0) * NL2;
+        const int lcs = lni + lnj + LNL2;   // one component plane of B4
+        const int nld = 3 * tj * NL2;
+        float2 v[NLD];
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < nld) {
+                const int LL = f & (NL2 - 1), r = f >> LNL2, t = r & (tj - 1), comp = r >> ltj;
+                v[q] = src[(comp << lcs) + (t << LNL2) + LL];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < nld) s[(f >> LNL2) * LS + (f & (NL2 - 1))] = cconj(v[q]);   // line comp * tj + t
+        }
     }
     for (int f = threadIdx.x; f < NL2; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
     fft_lds<NL2, 3 * TJ>(s, 3 * tj, LS, stw);
 
-    double acc[kSlots] = {0, 0, 0, 0, 0, 0};
+    // epilogue operands of every cell of this thread are loaded before the
+    // first is used (each cell's outputs overwrite only its own operands)
+    const int kind = ea.kind;
+    const bool need_x = kind != EK_CONV && kind != EK_BICG_R0 && kind != EK_CG_R0 && kind != EK_AA_R0;
+    const bool need_y = kind == EK_BICG_H || kind == EK_CG_W || kind == EK_SIM;
+    const bool need_o2 = kind == EK_AA_SUPD;
     const float2* al = ea.alpha + (long long)rhs * ea.nmat;
     const long long rb = (long long)rhs * 3 * d.n;
-    for (int f = threadIdx.x; f < tj * nl; f += blockDim.x) {
-        const int t = f / nl, l = f - t * nl;
-        const long long k = ((long long)i * d.nj + j0 + t) * nl + l;
-        const int m = occ[k];
-        const bool dip = mv[m] != 0;
-        float2 a = al[m];
-        if (ea.conj) a.y = -a.y;
+    const long long k0 = ((long long)i * d.nj + j0) * NLH;   // first cell of the block
+    const int nep = tj * NLH;
+    int kq[NEP];
+    bool dip[NEP];
+    float2 a[NEP], xr[NEP][3], yr[NEP][3], o2[NEP][3];
+#pragma unroll
+    for (int q = 0; q < NEP; ++q) {
+        const int f = threadIdx.x + q * kBlock;
+        kq[q] = f;   // cell k0 + f: row t = f / nl, l = f % nl
+        dip[q] = false;
+        a[q] = make_float2(0.f, 0.f);
+        if (f < nep) {
+            const int m = occ[k0 + f];
+            dip[q] = dipf[k0 + f] != 0;
+            a[q] = al[m];
+#pragma unroll
+            for (int c3 = 0; c3 < 3; ++c3) {
+                const long long ck = rb + c3 * d.n + k0 + f;
+                if (need_x) xr[q][c3] = ea.x[ck];
+                if (need_y) yr[q][c3] = ea.y[ck];
+                if (need_o2) o2[q][c3] = ea.out2[ck];
+            }
+        }
+    }
+    double acc[kSlots] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < NEP; ++q) {
+        const int f = kq[q];
+        if (f >= nep) continue;
+        const int t = f >> LNLH, l = f & (NLH - 1);
+        const long long k = k0 + f;
+        const bool dp = dip[q];
+        float2 av = a[q];
+        if (ea.conj) av.y = -av.y;
         float2 cv[3], o[3];
 #pragma unroll
         for (int comp = 0; comp < 3; ++comp) cv[comp] = cconj(s[(comp * tj + t) * LS + l]);
         double nrm = 0;
-        switch (ea.kind) {
+        switch (kind) {
             case EK_CONV:
 #pragma unroll
-                for (int q = 0; q < 3; ++q) ea.out[rb + q * d.n + k] = cv[q];
+                for (int c3 = 0; c3 < 3; ++c3) ea.out[rb + c3 * d.n + k] = cv[c3];
                 break;
             case EK_MV:
             case EK_BICG_H:
@@ -426,69 +558,69 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restr
             case EK_AA_S0:
             case EK_AA_W:
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const long long ck = rb + q * d.n + k;
-                    const float2 x = ea.x[ck];
-                    o[q] = dip ? csub(x, cmul(a, cv[q])) : make_float2(0.f, 0.f);
-                    ea.out[ck] = o[q];
-                    if (!dip) continue;
-                    if (ea.kind == EK_BICG_H) dot_acc(ea.y[ck], o[q], acc[0], acc[1]);
-                    if (ea.kind == EK_BICG_W) {
-                        dot_acc(o[q], x, acc[0], acc[1]);
-                        dot_acc(o[q], o[q], acc[2], acc[3]);
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    const long long ck = rb + c3 * d.n + k;
+                    const float2 x = xr[q][c3];
+                    o[c3] = dp ? csub(x, cmul(av, cv[c3])) : make_float2(0.f, 0.f);
+                    ea.out[ck] = o[c3];
+                    if (!dp) continue;
+                    if (kind == EK_BICG_H) dot_acc(yr[q][c3], o[c3], acc[0], acc[1]);
+                    if (kind == EK_BICG_W) {
+                        dot_acc(o[c3], x, acc[0], acc[1]);
+                        dot_acc(o[c3], o[c3], acc[2], acc[3]);
                     }
-                    if (ea.kind == EK_CG_W) {
-                        dot_acc(x, o[q], acc[0], acc[1]);
-                        dot_acc(ea.y[ck], x, acc[2], acc[3]);
+                    if (kind == EK_CG_W) {
+                        dot_acc(x, o[c3], acc[0], acc[1]);
+                        dot_acc(yr[q][c3], x, acc[2], acc[3]);
                     }
-                    if (ea.kind == EK_AA_S0) nrm += norm_term(x);
-                    if (ea.kind == EK_AA_W) acc[4] += norm_term(x);
+                    if (kind == EK_AA_S0) nrm += norm_term(x);
+                    if (kind == EK_AA_W) acc[4] += norm_term(x);
                 }
-                if (ea.kind == EK_AA_S0 && dip) acc[5] += sqrt(nrm);
+                if (kind == EK_AA_S0 && dp) acc[5] += sqrt(nrm);
                 break;
             case EK_BICG_R0:
             case EK_CG_R0:
             case EK_AA_R0:
             case EK_SIM_R0:
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const long long ck = rb + q * d.n + k;
-                    o[q] = dip ? cneg(cmul(a, cv[q])) : make_float2(0.f, 0.f);
-                    ea.out[ck] = o[q];
-                    if (ea.kind == EK_BICG_R0) {
-                        ea.out2[ck] = o[q];
-                        ea.out3[ck] = o[q];
-                        if (dip) dot_acc(o[q], o[q], acc[0], acc[1]);
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    const long long ck = rb + c3 * d.n + k;
+                    o[c3] = dp ? cneg(cmul(av, cv[c3])) : make_float2(0.f, 0.f);
+                    ea.out[ck] = o[c3];
+                    if (kind == EK_BICG_R0) {
+                        ea.out2[ck] = o[c3];
+                        ea.out3[ck] = o[c3];
+                        if (dp) dot_acc(o[c3], o[c3], acc[0], acc[1]);
                     }
-                    if (ea.kind == EK_CG_R0) ea.out2[ck] = o[q];
-                    if (ea.kind == EK_AA_R0 && dip) acc[4] += norm_term(o[q]);
-                    if (ea.kind == EK_SIM_R0) {
-                        const float2 x = ea.x[ck];
+                    if (kind == EK_CG_R0) ea.out2[ck] = o[c3];
+                    if (kind == EK_AA_R0 && dp) acc[4] += norm_term(o[c3]);
+                    if (kind == EK_SIM_R0) {
+                        const float2 x = xr[q][c3];
                         ea.out2[ck] = x;                       // S = P0 (Copying)
-                        ea.out3[ck] = csub(x, o[q]);           // P = S - R (all cells)
+                        ea.out3[ck] = csub(x, o[c3]);          // P = S - R (all cells)
                     }
-                    if (dip) nrm += norm_term(o[q]);
+                    if (dp) nrm += norm_term(o[c3]);
                 }
-                if ((ea.kind == EK_CG_R0 || ea.kind == EK_SIM_R0) && dip) acc[5] += sqrt(nrm);
+                if ((kind == EK_CG_R0 || kind == EK_SIM_R0) && dp) acc[5] += sqrt(nrm);
                 break;
             case EK_AA_SUPD:
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const long long ck = rb + q * d.n + k;
-                    if (dip) {
-                        const float2 w = csub(ea.x[ck], cmul(a, cv[q]));
-                        ea.out2[ck] = cadd(w, rmul(rB, ea.out2[ck]));
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    const long long ck = rb + c3 * d.n + k;
+                    if (dp) {
+                        const float2 w = csub(xr[q][c3], cmul(av, cv[c3]));
+                        ea.out2[ck] = cadd(w, rmul(rB, o2[q][c3]));
                     }
                 }
                 break;
             case EK_SIM:
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const long long ck = rb + q * d.n + k;
-                    if (dip) {
-                        const float2 x = ea.x[ck];
-                        const float2 w = csub(x, cmul(a, cv[q]));
-                        const float2 rr = csub(w, ea.y[ck]);
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    const long long ck = rb + c3 * d.n + k;
+                    if (dp) {
+                        const float2 x = xr[q][c3];
+                        const float2 w = csub(x, cmul(av, cv[c3]));
+                        const float2 rr = csub(w, yr[q][c3]);
                         ea.out[ck] = rr;
                         ea.out2[ck] = csub(x, rmul(ea.tau, rr));
                         nrm += norm_term(rr);
@@ -497,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restr
                         ea.out2[ck] = make_float2(0.f, 0.f);
                     }
                 }
-                if (dip) acc[5] += sqrt(nrm);
+                if (dp) acc[5] += sqrt(nrm);
                 break;
             default:
                 break;

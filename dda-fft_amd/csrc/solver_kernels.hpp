@@ -101,7 +101,7 @@ struct UpdArgs {
 };
 
 __global__ __launch_bounds__(kBlock) void k_update(Dims d, const SolveState* __restrict__ st, int iter, UpdArgs ua,
-                                                   const int8_t* __restrict__ occ, const int* __restrict__ mv) {
+                                                   const int8_t* __restrict__ dipf) {
     __shared__ double red[kSlots * 16];
     const int rhs = blockIdx.y;
     if (iter >= st[rhs].stop_at) return;
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, const SolveState* __r
     double acc[kSlots] = {0, 0, 0, 0, 0, 0};
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < d.n;
          k += (long long)gridDim.x * blockDim.x) {
-        const bool dip = mv[occ[k]] != 0;
+        const bool dip = dipf[k] != 0;
         double nrm = 0;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {

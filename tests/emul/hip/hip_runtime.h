@@ -210,7 +210,7 @@ inline unsigned __brev(unsigned x) {
 // ---------------------------------------------------------------------------
 
 typedef int hipError_t;
-enum { hipSuccess = 0, hipErrorOutOfMemory = 2 };
+enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorNotReady = 600 };
 enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
 typedef struct emul_stream* hipStream_t;
 struct emul_event {
@@ -272,6 +272,7 @@ inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
     return hipSuccess;
 }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }   // work runs synchronously here
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
     *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
     return hipSuccess;
