@@ -174,7 +174,9 @@ def gradient_run(ctx, w, fwd_ms, reps=5):
     m = complex(M_INDEX[0], M_INDEX[1])
     alpha = np.stack([w["alpha"], w["alpha"]])
     dadm = np.array([[0, ddafft.polarizability_deriv(1, w["kd"], m)]] * 2, np.complex64)
-    args = dict(dirs=None, normals=None, max_step=MAX_STEPS, err_max=TOL, ndirs=nd)
+    # grad_chi lands in a page-locked buffer that the caller reuses across calls
+    gchi = ddafft.pinned_empty((1,) + GRID + (2,), np.float32)
+    args = dict(dirs=None, normals=None, max_step=MAX_STEPS, err_max=TOL, ndirs=nd, grad_chi_out=gchi)
     res = ctx.gradient("bicgstab", alpha, dadm, [1.0, m], w["kinc"], w["einc"], W, **args)
     t0 = time.perf_counter()
     for _ in range(reps):

@@ -169,8 +169,7 @@ struct ddafft_ctx {
     size_t gaux_cap = 0;
     float2* esc = nullptr;
     int esc_cap = 0;
-    int* stop_dev = nullptr;
-    int* stop_host = nullptr;          // pinned ring
+    SolveState* state_ring = nullptr;  // pinned [4][nrhs_max]: solver states after each queued iteration
     hipEvent_t fin_ev = nullptr;       // marks the results of a batched solve as copied back
     // pinned staging of the per-solve transfers, so that they are truly
     // asynchronous: uploads are packed at up_off (reset when a solve starts,
@@ -318,7 +317,7 @@ int check_args(const ProArgs& pa, const EpiArgs& ea, bool solver) {
 }
 
 // One matvec of nrhs right-hand sides: ZF -> YF -> X -> YI -> ZI.
-int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, const SolveState* st, int iter, const EpiArgs& ea,
+int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, int iter, const EpiArgs& ea,
                   int conj, hipEvent_t* ev = nullptr) {
     if (!ev && c->prof) {
         const size_t need = (size_t)6 * (c->prof_used + 1);
@@ -612,15 +611,30 @@ const char* ddafft_last_error(void) { return g_err.c_str(); }
 
 int ddafft_version(void) { return 100; }
 
+int ddafft_host_alloc(size_t bytes, void** out) {
+    if (!out || bytes == 0) return fail(DDAFFT_EINVAL, "bad argument");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return fail(DDAFFT_ENOMEM, "hipHostMalloc failed");
+    }
+    return DDAFFT_OK;
+}
+
+int ddafft_host_free(void* p) {
+    if (p) HIPCHECK(hipHostFree(p));
+    return DDAFFT_OK;
+}
+
 int ddafft_destroy(ddafft_ctx* c) {
     if (!c) return DDAFFT_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->ahat, c->tables, c->occ, c->dipf, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
                     c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
-                    c->dirs, c->esc, c->stop_dev, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab};
+                    c->dirs, c->esc, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab};
     for (void* b : bufs)
         if (b) hipFree(b);
-    if (c->stop_host) hipHostFree(c->stop_host);
+    if (c->state_ring) hipHostFree(c->state_ring);
     if (c->up) hipHostFree(c->up);
     if (c->dn) hipHostFree(c->dn);
     for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
@@ -679,8 +693,8 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
     if (!rc) rc = dev_alloc(&c->st, (size_t)nrhs_max);
     if (!rc) rc = dev_alloc(&c->occ, (size_t)n);
     if (!rc) rc = dev_alloc(&c->dipf, (size_t)n);
-    if (!rc) rc = dev_alloc(&c->stop_dev, 1);
-    if (!rc && hipHostMalloc((void**)&c->stop_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (!rc && hipHostMalloc((void**)&c->state_ring, (size_t)4 * nrhs_max * sizeof(SolveState),
+                             hipHostMallocDefault) != hipSuccess)
         rc = fail(DDAFFT_ENOMEM, "hipHostMalloc");
     if (!rc) rc = ensure_partials(c, (size_t)nrhs_max * std::max<long long>(zi_blocks(c->d), 1024) * kSlots);
     if (rc) {
@@ -840,6 +854,7 @@ struct Seq {
     int nrhs;
     FinArgs fa;
 
+    // the scalar step `kind` over nblk block partials per right-hand side
     int fin(int kind, int iter, int nblk) {
         FinArgs f = fa;
         f.kind = kind;
@@ -851,14 +866,17 @@ struct Seq {
         HIPCHECK(hipGetLastError());
         return DDAFFT_OK;
     }
-    int mv(int iter, ProArgs pa, EpiArgs ea, int conj) {
+    // a matvec, then (fin_kind >= 0) the scalar step over its Z-inverse partials
+    int mv(int iter, ProArgs pa, EpiArgs ea, int conj, int fin_kind = -1) {
         ea.alpha = c->alpha;
         ea.nmat = c->nmat;
         ea.partials = c->partials;
         ea.conj = conj;
-        return launch_matvec(c, nrhs, pa, c->st, iter, ea, conj);
+        TRY(launch_matvec(c, nrhs, pa, c->st, iter, ea, conj));
+        return fin_kind >= 0 ? fin(fin_kind, iter, zi_blocks(c->d)) : DDAFFT_OK;
     }
-    int upd(int iter, UpdArgs ua) {
+    // the solver update, then (fin_kind >= 0) the scalar step over its partials
+    int upd(int iter, UpdArgs ua, int fin_kind = -1) {
         Dims d = c->d;
         d.nrhs = nrhs;
         ua.partials = c->partials;
@@ -867,7 +885,7 @@ struct Seq {
         dim3 g(update_blocks(d), nrhs);
         hipLaunchKernelGGL(k_update, g, kBlock, 0, c->stream, d, c->st, iter, ua, c->dipf);
         HIPCHECK(hipGetLastError());
-        return DDAFFT_OK;
+        return fin_kind >= 0 ? fin(fin_kind, iter, update_blocks(d)) : DDAFFT_OK;
     }
 };
 
@@ -894,28 +912,20 @@ int step0(Seq& q, int solver) {
     const int ub = update_blocks(c->d);
     switch (solver) {
         case DDAFFT_SOLVER_CG:
-            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_CG_R0, nullptr, c->R, c->S), 0));
-            TRY(q.fin(FK_CG_E0, 0, zb));
+            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_CG_R0, nullptr, c->R, c->S), 0, FK_CG_E0));
             break;
         case DDAFFT_SOLVER_AA:
-            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_AA_R0, nullptr, c->R), 0));
-            TRY(q.fin(FK_AA_0, 0, zb));
-            TRY(q.mv(0, pro(PRO_COPY, c->R), epi(EK_AA_S0, c->R, c->S), 1));
-            TRY(q.fin(FK_AA_E0, 0, zb));
+            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_AA_R0, nullptr, c->R), 0, FK_AA_0));
+            TRY(q.mv(0, pro(PRO_COPY, c->R), epi(EK_AA_S0, c->R, c->S), 1, FK_AA_E0));
             break;
         case DDAFFT_SOLVER_BICGSTAB:
-            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_BICG_R0, nullptr, c->R, c->S, c->G), 0));
-            TRY(q.fin(FK_BICG_0, 0, zb));
-            TRY(q.mv(0, pro(PRO_COPY, c->S), epi(EK_BICG_H, c->S, c->H, nullptr, nullptr, c->G), 0));
-            TRY(q.fin(FK_BICG_A, 0, zb));
-            TRY(q.mv(0, pro(PRO_LC1, c->R, c->H), epi(EK_BICG_W, c->R, c->W), 0));
-            TRY(q.fin(FK_BICG_W, 0, zb));
-            TRY(q.upd(0, UpdArgs{UK_BICG, c->P, c->R, c->S, c->W, c->G, nullptr}));
-            TRY(q.fin(FK_BICG_E, 0, ub));
+            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_BICG_R0, nullptr, c->R, c->S, c->G), 0, FK_BICG_0));
+            TRY(q.mv(0, pro(PRO_COPY, c->S), epi(EK_BICG_H, c->S, c->H, nullptr, nullptr, c->G), 0, FK_BICG_A));
+            TRY(q.mv(0, pro(PRO_LC1, c->R, c->H), epi(EK_BICG_W, c->R, c->W), 0, FK_BICG_W));
+            TRY(q.upd(0, UpdArgs{UK_BICG, c->P, c->R, c->S, c->W, c->G, nullptr}, FK_BICG_E));
             break;
         case DDAFFT_SOLVER_SIMPLE:
-            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_SIM_R0, c->P, c->R, c->S, c->P), 0));
-            TRY(q.fin(FK_SIM_E, 0, zb));
+            TRY(q.mv(0, pro(PRO_COPY, c->P), epi(EK_SIM_R0, c->P, c->R, c->S, c->P), 0, FK_SIM_E));
             break;
     }
     return DDAFFT_OK;
@@ -933,39 +943,33 @@ int iteration(Seq& q, int solver, int o, float tau) {
     switch (solver) {
         case DDAFFT_SOLVER_CG: {
             ProArgs p = (o == 1) ? pro(PRO_COPY, c->S) : pro(PRO_LC2, c->R, c->S);
-            TRY(q.mv(o, snap(p), epi(EK_CG_W, c->S, c->W, nullptr, nullptr, c->R), 0));
-            TRY(q.fin(FK_CG_A, o, zb));
-            TRY(q.upd(o, UpdArgs{UK_CG, c->P, c->R, c->S, c->W, nullptr, nullptr}));
-            TRY(q.fin(FK_CG_E, o, ub));
+            TRY(q.mv(o, snap(p), epi(EK_CG_W, c->S, c->W, nullptr, nullptr, c->R), 0, FK_CG_A));
+            TRY(q.upd(o, UpdArgs{UK_CG, c->P, c->R, c->S, c->W, nullptr, nullptr}, FK_CG_E));
             break;
         }
         case DDAFFT_SOLVER_AA:
-            TRY(q.mv(o, snap(pro(PRO_COPY, c->S)), epi(EK_AA_W, c->S, c->W), 0));
-            TRY(q.fin(FK_AA_A, o, zb));
-            TRY(q.upd(o, UpdArgs{UK_AA, c->P, c->R, c->S, c->W, nullptr, nullptr}));
-            TRY(q.fin(FK_AA_E, o, ub));
+            TRY(q.mv(o, snap(pro(PRO_COPY, c->S)), epi(EK_AA_W, c->S, c->W), 0, FK_AA_A));
+            TRY(q.upd(o, UpdArgs{UK_AA, c->P, c->R, c->S, c->W, nullptr, nullptr}, FK_AA_E));
             TRY(q.mv(o, pro(PRO_COPY, c->R), epi(EK_AA_SUPD, c->R, nullptr, c->S), 1));
             break;
         case DDAFFT_SOLVER_BICGSTAB:
-            TRY(q.mv(o, snap(pro(PRO_LC4, c->R, c->S, c->H)), epi(EK_BICG_H, c->S, c->H, nullptr, nullptr, c->G), 0));
-            TRY(q.fin(FK_BICG_A, o, zb));
-            TRY(q.mv(o, pro(PRO_LC1, c->R, c->H), epi(EK_BICG_W, c->R, c->W), 0));
-            TRY(q.fin(FK_BICG_W, o, zb));
-            TRY(q.upd(o, UpdArgs{UK_BICG, c->P, c->R, c->S, c->W, c->G, nullptr}));
-            TRY(q.fin(FK_BICG_E, o, ub));
+            TRY(q.mv(o, snap(pro(PRO_LC4, c->R, c->S, c->H)), epi(EK_BICG_H, c->S, c->H, nullptr, nullptr, c->G), 0, FK_BICG_A));
+            TRY(q.mv(o, pro(PRO_LC1, c->R, c->H), epi(EK_BICG_W, c->R, c->W), 0, FK_BICG_W));
+            TRY(q.upd(o, UpdArgs{UK_BICG, c->P, c->R, c->S, c->W, c->G, nullptr}, FK_BICG_E));
             break;
         case DDAFFT_SOLVER_SIMPLE:
-            TRY(q.mv(o, snap(pro(PRO_COPY, c->P)), epi(EK_SIM, c->P, c->R, c->P, nullptr, c->S, tau), 0));
-            TRY(q.fin(FK_SIM_E, o, zb));
+            TRY(q.mv(o, snap(pro(PRO_COPY, c->P)), epi(EK_SIM, c->P, c->R, c->P, nullptr, c->S, tau), 0, FK_SIM_E));
             break;
     }
     return DDAFFT_OK;
 }
 
-__global__ void k_stop_max(const SolveState* st, int nrhs, int* out) {
-    int m = 0;
-    for (int r = 0; r < nrhs; ++r) m = max(m, st[r].stop_at);
-    *out = m;
+// Copies the solver states into slot `slot` of the pinned ring; the host
+// reads the stopping decisions from there once the slot's event has passed.
+int copy_states(ddafft_ctx* c, int nrhs, int slot) {
+    HIPCHECK(hipMemcpyAsync(c->state_ring + (size_t)slot * c->nrhs_max, c->st, (size_t)nrhs * sizeof(SolveState),
+                            hipMemcpyDeviceToHost, c->stream));
+    return DDAFFT_OK;
 }
 
 // Results of a solve on their way back to the host: offsets into the
@@ -1122,8 +1126,7 @@ int run_begin(Run& r) {
     r.q.fa.err_max = (double)r.err_max;
     r.q.fa.inv_ndip = 1.0 / (double)c->ndip;
     TRY(step0(r.q, r.solver));
-    hipLaunchKernelGGL(k_stop_max, 1, 1, 0, c->stream, c->st, r.nrhs, c->stop_dev);
-    HIPCHECK(hipMemcpyAsync(&c->stop_host[0], c->stop_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TRY(copy_states(c, r.nrhs, 0));
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
     r.o = 1;
     r.known_stop = r.max_step + 1;
@@ -1137,8 +1140,7 @@ int run_enqueue(Run& r) {
     ddafft_ctx* c = r.c;
     const int o = r.o;
     TRY(iteration(r.q, r.solver, o, r.tau));
-    hipLaunchKernelGGL(k_stop_max, 1, 1, 0, c->stream, c->st, r.nrhs, c->stop_dev);
-    HIPCHECK(hipMemcpyAsync(&c->stop_host[o & 3], c->stop_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TRY(copy_states(c, r.nrhs, o & 3));
     HIPCHECK(hipEventRecord(c->ev[o & 3], c->stream));
     r.o = o + 1;
     return DDAFFT_OK;
@@ -1149,7 +1151,10 @@ int run_wait(Run& r) {
     const int w = r.o - 2;
     if (w < 0) return DDAFFT_OK;
     HIPCHECK(hipEventSynchronize(r.c->ev[w & 3]));
-    r.known_stop = std::min(r.known_stop, r.c->stop_host[w & 3]);
+    const SolveState* sr = r.c->state_ring + (size_t)(w & 3) * r.c->nrhs_max;
+    int m = 0;
+    for (int q = 0; q < r.nrhs; ++q) m = std::max(m, sr[q].stop_at);
+    r.known_stop = std::min(r.known_stop, m);
     return DDAFFT_OK;
 }
 
@@ -1500,15 +1505,28 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
     const size_t off_s = al16(off_ex + nadj * sizeof(float2));
     const size_t off_is = al16(off_s + nadj * sizeof(float));
     const size_t off_dc = al16(off_is + nadj * sizeof(float));
-    const size_t aux_bytes = al16(off_dc + (size_t)npol * nmat * sizeof(float2));
+    const size_t off_na = al16(off_dc + (size_t)npol * nmat * sizeof(float2));
+    const size_t aux_bytes = al16(off_na + nadj * sizeof(double));
     std::vector<char> aux(aux_bytes, 0);
     float2* hu = reinterpret_cast<float2*>(aux.data() + off_u);
     float2* hce = reinterpret_cast<float2*>(aux.data() + off_ce);
     float* hca = reinterpret_cast<float*>(aux.data() + off_ca);
     float2* hex = reinterpret_cast<float2*>(aux.data() + off_ex);
-    float* hs = reinterpret_cast<float*>(aux.data() + off_s);
-    float* his = reinterpret_cast<float*>(aux.data() + off_is);
     float2* hdc = reinterpret_cast<float2*>(aux.data() + off_dc);
+    double* hna = reinterpret_cast<double*>(aux.data() + off_na);
+    // size of each forward source alpha_j E_j (summed over the dipoles): the
+    // adjoint source is scaled to it on the device, so that err_max means the
+    // same relative residual for both solves
+    for (int r = 0; r < nadj; ++r) {
+        const int p = r % npol;
+        double sa = 0.;
+        for (int q = 0; q < nmat; ++q)
+            sa += (double)c->mat_count[q] *
+                  std::hypot((double)alpha[((size_t)p * nmat + q) * 2], (double)alpha[((size_t)p * nmat + q) * 2 + 1]);
+        double e2 = 0.;
+        for (int q = 0; q < 6; ++q) e2 += (double)einc[6 * p + q] * einc[6 * p + q];
+        hna[r] = sa * std::sqrt(e2);
+    }
     std::vector<float2> alpha_adj((size_t)nadj * nmat);
     for (int o = 0; o < nobj; ++o) {
         const float* w = weights + (size_t)o * nw;
@@ -1558,8 +1576,8 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
                            kBlock, 0, c->stream, d, g, c->dirs, ndirs, c->phase);
         HIPCHECK(hipGetLastError());
     }
-    // the weights and coefficients (the scales are filled in below)
-    HIPCHECK(hipMemcpyAsync(c->gaux, aux.data(), off_s, hipMemcpyHostToDevice, c->stream));
+    // the weights and coefficients (the scales are computed on the device)
+    TRY(upload(c, c->gaux, aux.data(), aux_bytes));
 
     // adjoint sources b = diag(alpha) g and their mean norms
     d.nrhs = nadj;
@@ -1585,26 +1603,10 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
         hipLaunchKernelGGL(k_adj_source, dim3(nblk, nadj), kBlock, 0, c->stream, d, g, c->occ, c->mv, aa);
         HIPCHECK(hipGetLastError());
     }
-    std::vector<double> nb((size_t)nadj * nblk);
-    HIPCHECK(hipMemcpyAsync(nb.data(), c->partials, nb.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    // Scale each source to the size of the forward source alpha_j E_j so that
-    // err_max means the same relative residual for both solves.
-    for (int r = 0; r < nadj; ++r) {
-        const int p = r % npol;
-        double sb = 0., sa = 0.;
-        for (int b = 0; b < nblk; ++b) sb += nb[(size_t)r * nblk + b];
-        for (int q = 0; q < nmat; ++q)
-            sa += (double)c->mat_count[q] *
-                  std::hypot((double)alpha[((size_t)p * nmat + q) * 2], (double)alpha[((size_t)p * nmat + q) * 2 + 1]);
-        double e2 = 0.;
-        for (int q = 0; q < 6; ++q) e2 += (double)einc[6 * p + q] * einc[6 * p + q];
-        const double na = sa * std::sqrt(e2);
-        const double sc = (sb > 0. && na > 0.) ? sb / na : 1.;
-        hs[r] = (float)sc;
-        his[r] = (float)(1. / sc);
-    }
-    HIPCHECK(hipMemcpyAsync(c->gaux + off_s, aux.data() + off_s, aux_bytes - off_s, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_adj_scale, 1, 64, 0, c->stream, c->partials, nblk, nadj,
+                       reinterpret_cast<const double*>(c->gaux + off_na), reinterpret_cast<float*>(c->gaux + off_s),
+                       reinterpret_cast<float*>(c->gaux + off_is));
+    HIPCHECK(hipGetLastError());
     {
         dim3 grid((int)std::min<long long>((3 * n + kBlock - 1) / kBlock, 2048), nadj);
         hipLaunchKernelGGL(k_scale_rhs, grid, kBlock, 0, c->stream, d, reinterpret_cast<const float*>(c->gaux + off_is),
@@ -1612,10 +1614,25 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
         HIPCHECK(hipGetLastError());
     }
 
-    // adjoint solve: the same iteration from the sources in c->P
+    // adjoint solve: the same iteration from the sources in c->P; its final
+    // states come back with the gradients (one synchronization at the end)
     std::vector<SolveState> sfa;
     std::vector<int> info_a(4 * nadj);
-    TRY(solve_core(c, nadj, solver, tau, max_step, err_max, nullptr, info_a.data(), sfa));
+    Run ra;
+    ra.c = c;
+    ra.nrhs = nadj;
+    ra.solver = solver;
+    ra.tau = tau;
+    ra.max_step = max_step;
+    ra.err_max = err_max;
+    TRY(run_begin(ra));
+    while (run_more(ra)) {
+        TRY(run_enqueue(ra));
+        TRY(run_wait(ra));
+    }
+    Fin fa;
+    TRY(fin_layout(c, nadj, true, false, max_step, 0, fa));
+    TRY(fin_state_enqueue(ra, fa));
 
     // df/dalpha per dipole, the chi map, and the per-material sums
     {
@@ -1643,6 +1660,7 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
     if (grad_chi)
         HIPCHECK(hipMemcpyAsync(grad_chi, c->S, (size_t)nobj * n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    TRY(fin_state_complete(ra, fa, nullptr, info_a.data(), sfa));
     if (grad_m) {
         for (int o = 0; o < nobj; ++o)
             for (int q = 0; q < nmat; ++q) {

@@ -131,6 +131,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_source(Dims d, Geometry g, const
     if (threadIdx.x == 0) aa.partials[(long long)r * gridDim.x + blockIdx.x] = acc[0];
 }
 
+// Scale of each adjoint source: s = sum_b partials[r][b] / na[r] (the sum in
+// block order, in FP64), 1 when either is zero; inv_s = 1 / s.
+__global__ void k_adj_scale(const double* __restrict__ partials, int nblk, int nadj, const double* __restrict__ na,
+                            float* __restrict__ s, float* __restrict__ inv_s) {
+    const int r = threadIdx.x;
+    if (r >= nadj) return;
+    double sb = 0.;
+    for (int b = 0; b < nblk; ++b) sb += partials[(long long)r * nblk + b];
+    const double sc = (sb > 0. && na[r] > 0.) ? sb / na[r] : 1.;
+    s[r] = (float)sc;
+    inv_s[r] = (float)(1. / sc);
+}
+
 // x[r] *= scale[r] over the 3 n components of right-hand side r.
 __global__ __launch_bounds__(kBlock) void k_scale_rhs(Dims d, const float* __restrict__ scale, float2* __restrict__ x) {
     const int r = blockIdx.y;

@@ -23,6 +23,7 @@
 
 #include "device_types.hpp"
 #include "fft_common.hpp"
+#include "finalize.hpp"
 
 namespace ddafft {
 
@@ -430,7 +431,6 @@ enum EpiKind : int {
     EK_SIM = 12,      // W = x - a*conv; R = W - S; P = x - tau*R; err(R)     (:1471-1521)
 };
 
-constexpr int kSlots = 6;  // partial sums: dot1 (re,im), dot2 (re,im), real1, err
 
 struct EpiArgs {
     int kind;
@@ -455,7 +455,7 @@ __device__ __forceinline__ void dot_acc(float2 a, float2 b, double& re, double& 
 __device__ __forceinline__ double norm_term(float2 a) { return (double)(a.x * a.x + a.y * a.y); }
 
 template <int NL2>
-__global__ __launch_bounds__(kBlock) void k_zi(Dims d, const SolveState* __restrict__ st, int iter,
+__global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ st, int iter,
                                                const float2* __restrict__ b4, EpiArgs ea,
                                                const int8_t* __restrict__ occ, const int8_t* __restrict__ dipf,
                                                const float2* __restrict__ tw) {

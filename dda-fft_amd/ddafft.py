@@ -29,6 +29,7 @@ EXPORTS = (
     "ddafft_target_hierarchical_ellipsoid", "ddafft_target_sphere_spec", "ddafft_worm_count", "ddafft_target_worms",
     "ddafft_target_file_read", "ddafft_target_file_write", "ddafft_orientation_mc", "ddafft_rotate",
     "ddafft_target_mixture", "ddafft_agglomerate_count", "ddafft_target_agglomerate",
+    "ddafft_host_alloc", "ddafft_host_free",
 )
 
 
@@ -80,6 +81,8 @@ def load(path=LIB_PATH):
     lib.ddafft_target_mixture.argtypes = [I, P, I, D, P, P]
     lib.ddafft_agglomerate_count.argtypes = [I, I, I, D, D]
     lib.ddafft_target_agglomerate.argtypes = [I, I, I, D, D, C.c_longlong, P, P, P]
+    lib.ddafft_host_alloc.argtypes = [C.c_size_t, C.POINTER(P)]
+    lib.ddafft_host_free.argtypes = [P]
     lib.ddafft_polarizability.argtypes = [I, F, P, P, P, P]
     lib.ddafft_polarizability_deriv.argtypes = [I, F, P, P, P, P]
     D = C.c_double
@@ -200,13 +203,15 @@ class Context:
         return out
 
     def gradient(self, solver, alpha, dalpha_dm, m, kinc, einc, weights, dirs=None, normals=None, max_step=20,
-                 err_max=1e-3, tau=0.0, ndirs=None):
+                 err_max=1e-3, tau=0.0, ndirs=None, grad_chi_out=None):
         """Forward + adjoint gradients (ddafft_gradient).
 
         alpha, dalpha_dm: [npol, nmat] complex; m: [nmat] complex; einc: [npol, 3];
         weights: [nobj, 2 + ndirs] rows (w_ext, w_abs, w_s11...); normals: [ndirs, 3].
         Returns a dict: f [nobj], grad_m [nobj, nmat, 2], grad_chi [nobj, ni, nj, nl, 2],
         esc [npol, ndirs, 3], cabs/cext [npol], s11 [ndirs], info [npol*(1+nobj), 4].
+        grad_chi_out: an optional float32 array of that shape to receive grad_chi
+        (for example one from pinned_empty, reused across calls).
         """
         solver = SOLVERS.get(solver, solver)
         alpha = np.ascontiguousarray(np.atleast_2d(alpha), np.complex64)
@@ -231,7 +236,13 @@ class Context:
         cext = np.zeros(npol, np.float32)
         s11 = np.zeros(max(nd, 1), np.float32)
         gm = np.zeros((nobj, self.nmat, 2), np.float32)
-        gchi = np.zeros((nobj,) + self.dims + (2,), np.float32)
+        shape = (nobj,) + self.dims + (2,)
+        if grad_chi_out is not None:
+            if grad_chi_out.shape != shape or grad_chi_out.dtype != np.float32 or not grad_chi_out.flags.c_contiguous:
+                raise ValueError("grad_chi_out must be a C-contiguous float32 array of shape %s" % (shape,))
+            gchi = grad_chi_out
+        else:
+            gchi = np.zeros(shape, np.float32)
         info = np.zeros((npol * (1 + nobj), 4), np.int32)
         want_s11 = normals is not None and npol == 2 and nd > 0
         _check(_lib.ddafft_gradient(self._h, npol, nobj, int(solver), C.c_float(tau), int(max_step),
@@ -366,6 +377,33 @@ def target_ellipsoid(dims, b_a=1.0, c_a=1.0):
     if cnt < 0:
         raise DdafftError("ellipsoid does not fit the lattice")
     return occ, cnt
+
+
+class _Pinned:
+    """Owner of one page-locked host allocation (freed with the last array view)."""
+
+    def __init__(self, nbytes):
+        load()
+        self.ptr = C.c_void_p()
+        _check(_lib.ddafft_host_alloc(int(nbytes), C.byref(self.ptr)))
+        self.nbytes = int(nbytes)
+
+    def __del__(self):
+        if _lib is not None and self.ptr:
+            _lib.ddafft_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def pinned_empty(shape, dtype=np.float32):
+    """A numpy array in page-locked host memory (ddafft_host_alloc): results
+    copied into it arrive at DMA speed.  The memory lives as long as the array."""
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape)) * dtype.itemsize
+    owner = _Pinned(max(n, 1))
+    buf = (C.c_char * max(n, 1)).from_address(owner.ptr.value)
+    buf._owner = owner
+    arr = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+    return arr
 
 
 def target_mixture(base, material, fraction, state):

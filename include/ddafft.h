@@ -23,6 +23,8 @@
 #ifndef DDAFFT_H
 #define DDAFFT_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -46,6 +48,13 @@ const char* ddafft_last_error(void);
 
 /* Library version, major*10000 + minor*100 + patch. */
 int ddafft_version(void);
+
+/* Page-locked host memory for inputs and results (no counterpart in the
+ * reference).  Results copied into such buffers (dipoles, grad_chi, far
+ * fields) move at DMA speed instead of through the runtime's staging copies
+ * of pageable memory.  Free with ddafft_host_free. */
+int ddafft_host_alloc(size_t bytes, void** out);
+int ddafft_host_free(void* p);
 
 /* Creates a context for an ni x nj x nl lattice (powers of two, 4..256 per
  * axis) with lattice spacing kd, able to solve up to nrhs_max right-hand

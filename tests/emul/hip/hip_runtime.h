@@ -257,6 +257,10 @@ inline hipError_t hipStreamDestroy(hipStream_t s) {
     return hipSuccess;
 }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipMemset(void* p, int v, size_t n) {
+    std::memset(p, v, n);
+    return hipSuccess;
+}
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 inline hipError_t hipEventCreate(hipEvent_t* e) {
     *e = new emul_event();

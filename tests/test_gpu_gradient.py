@@ -120,3 +120,28 @@ def test_gradient_workload_against_fp64_differences():
     print("material grad rel err (max over objectives)", em.max(), "dipole grad rel err", ed.max(), flush=True)
     assert em.max() < 2e-3
     assert ed.max() < 2e-3
+
+
+def test_gradient_into_pinned_buffer():
+    """grad_chi written into a page-locked buffer (ddafft_host_alloc) that is
+    reused across calls equals the freshly allocated result, bit for bit."""
+    dims, kd = (8, 8, 16), 0.5
+    occ, _ = ddafft.target_ellipsoid(dims, 1.0, 2.0)
+    kinc = np.array([0, 0, 1.0], np.float32)
+    einc = np.array([[1, 0, 0], [0, 1, 0]], np.complex64)
+    m = [1, 1.5 + 0.01j]
+    alpha = np.array([[0, ddafft.polarizability(1, kd, m[1])]] * 2)
+    dadm = np.array([[0, ddafft.polarizability_deriv(1, kd, m[1])]] * 2)
+    W = np.array([[1.0, 0.5]], np.float32)
+    out = ddafft.pinned_empty((1,) + dims + (2,), np.float32)
+    print("case pinned grad_chi", dims, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, [0, 1])
+        ref = ctx.gradient("bicgstab", alpha, dadm, m, kinc, einc, W, ndirs=0, max_step=50, err_max=1e-5)
+        for _ in range(2):
+            out[:] = np.nan
+            res = ctx.gradient("bicgstab", alpha, dadm, m, kinc, einc, W, ndirs=0, max_step=50, err_max=1e-5,
+                               grad_chi_out=out)
+            assert res["grad_chi"] is out
+            assert np.array_equal(out.view(np.uint32), ref["grad_chi"].view(np.uint32))
+            assert np.array_equal(res["grad_m"], ref["grad_m"])
