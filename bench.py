@@ -165,6 +165,28 @@ def batched_run(frac=1.0, width=8):
                             for a, b, d, c in MC_LATTICES), MC_ORIENT)}
 
 
+def matvec_scaling(reps=50):
+    """Average duration of each matvec pass (HIP events, ddafft_time_matvec)
+    for 1..16 right-hand sides sharing one operator, with the algorithmic
+    GB/s of each pass."""
+    print("# lattice nrhs | total_us us_per_rhs | zf yf x yi zi (us) | zf yf x yi zi (GB/s) | matvec GB/s")
+    for dims, rmax in (((32, 32, 32), 16), ((64, 64, 64), 16), ((128, 128, 128), 4)):
+        n = int(np.prod(dims))
+        ctx = ddafft.Context(dims, 0.3, nrhs_max=rmax)
+        occ, _ = ddafft.target_ellipsoid(dims, 1.0, 1.0)
+        ctx.set_target(occ, [0, 1])
+        r = 1
+        while r <= rmax:
+            km, tot = ctx.time_matvec(r, reps)
+            by = matvec_bytes(n, r)
+            gbs = [by[k] / (t * 1e-3) / 1e9 for k, t in zip(("zf", "yf", "x", "yi", "zi"), km)]
+            print("%s %2d | %8.1f %8.1f | %s | %s | %6.0f" % (
+                "x".join(map(str, dims)), r, tot * 1e3, tot * 1e3 / r, " ".join("%6.1f" % (v * 1e3) for v in km),
+                " ".join("%5.0f" % g for g in gbs), sum(by.values()) / (tot * 1e-3) / 1e9), flush=True)
+            r *= 2
+        ctx.close()
+
+
 def gradient_run(ctx, w, fwd_ms, reps=5):
     """The with-gradient form on the plain workload: forward solve of both
     polarizations, adjoint solve of d(mean Cext)/d(m, chi), assembly."""
@@ -233,6 +255,8 @@ def main():
                     help="fraction of each lattice group of the 128-realization batched workload")
     ap.add_argument("--batched-width", type=int, default=8, help="realizations in flight in the batch driver")
     ap.add_argument("--no-gradient", action="store_true", help="skip the forward+adjoint form")
+    ap.add_argument("--matvec-scaling", action="store_true",
+                    help="print the per-pass matvec times against the number of right-hand sides and exit")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (torch), the reference kernel "
                          "against which rocprofv3 FETCH_SIZE / WRITE_SIZE counters are scaled")
@@ -252,6 +276,9 @@ def main():
         del src, dst
 
     lib = ddafft.load()
+    if args.matvec_scaling:
+        matvec_scaling()
+        return 0
     w = build_workload(lib)
     ctx = ddafft.Context(GRID, w["kd"], nrhs_max=2)
     ctx.set_target(w["occ"], [0, 1])
