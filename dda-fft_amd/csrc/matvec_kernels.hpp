@@ -32,8 +32,12 @@ namespace ddafft {
 // points; its tile stays near 16 KB (two (J, L) columns at ni = 64) so that
 // several blocks per CU hide the latency of its two global phases (measured
 // on MI355X: 73 us at 8 columns, 57 at 4, 52 at 2, 69 at 1, 64^3, 2 RHS).
+// Longer lines keep two columns, so that the 384 butterflies of a stage fill
+// the block's threads evenly (256^3, 2 RHS: 7.16 ms at one column, 6.66 at two).
 template <int N> struct TileZF { static constexpr int T = clamp16(pow2floor(6144 / (2 * (N + kLpad)))); };
-template <int N, int RB> struct TileX { static constexpr int T = clamp16(pow2floor(2048 / (6 * RB * (N + kLpad)))); };
+template <int N, int RB> struct TileX {
+    static constexpr int T = N >= 128 ? 2 : clamp16(pow2floor(2048 / (6 * RB * (N + kLpad))));
+};
 template <int N> struct TileYI { static constexpr int T = clamp16(pow2floor(6144 / (N + kLpad))); };
 template <int N> struct TileZI { static constexpr int T = clamp16(pow2floor(6144 / (3 * (N + kLpad)))); };
 

@@ -90,9 +90,39 @@ def main():
     ok &= check_batch(orc)
     if not quick:
         ok &= check_gradient()
+        ok &= check_long_axes(orc)
     print("ALL OK" if ok else "SOME CHECKS FAILED", flush=True)
     return 0 if ok else 1
 
+
+
+def check_long_axes(orc):
+    """256- and 128-point axes (the transform sizes of the 256^3 workload, 512
+    points padded): A-hat, both convolutions and a short solve, bit for bit."""
+    ok = True
+    rng = np.random.default_rng(5)
+    for dims in [(256, 4, 4), (4, 256, 4), (4, 4, 256), (128, 8, 4)]:
+        kd = np.float32(0.3)
+        occ, _ = orc.target_ellipsoid(dims, min(1.0, dims[1] / dims[0]), min(1.0, dims[2] / dims[0]))
+        mv = np.array([0, 1], np.int32)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.01j)], np.complex64)
+        with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+            A = ctx.ahat()
+            ok &= check(np.array_equal(bits(A), bits(orc.build_ahat(dims, kd))), "ahat %s" % (dims,))
+            ctx.set_target(occ, mv)
+            x = rnd(rng, dims + (3,))
+            for conj in (False, True):
+                ok &= check(np.array_equal(bits(ctx.convolution(x, conj)), bits(orc.convolution(A, x, conj))),
+                            "conv %s %d" % (dims, conj))
+            kinc = np.array([0, 0, 1], np.float32)
+            einc = np.eye(3, dtype=np.complex64)[:2]
+            res = ctx.solve("bicgstab", np.stack([alpha, alpha]), kinc, einc, max_step=3, err_max=1e-9)
+            for p in range(2):
+                o = orc.dda_fft(2, occ, mv, alpha, kd, kinc, einc[p], A, np.zeros((0, 3), np.float32), 3, 1e-9,
+                                return_p=True)
+                ok &= check(res[p]["steps"] == o["steps"] and np.array_equal(bits(ctx.dipoles(p)), bits(o["p"])),
+                            "solve %s pol %d" % (dims, p))
+    return ok
 
 def check_batch(orc):
     """ddafft_solve_batch over two uneven lattices (width 1: the refill path),
