@@ -233,6 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
                                               const float2* __restrict__ ahat, int conj,
                                               const float2* __restrict__ wpre, const float2* __restrict__ tw1,
                                               const float2* __restrict__ tw2) {
+    static_assert(RB == 1 || RB == 2, "one or two right-hand sides per X tile");
     constexpr int CB = 3 * RB;                       // component lines per (J, L)
     constexpr int TJ = TileX<NI, RB>::T;
     constexpr int NI2 = 2 * NI;
@@ -263,13 +264,14 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
         const int f = threadIdx.x + it * kBlock;
         if (f < tj * NI2) {
             const int I = f % NI2, t = f / NI2;
-            int Ja = J0 + t, La = L, Ia = I;
+            unsigned Ja = J0 + t, La = L, Ia = I;
             if (conj) {
                 Ja = (NJ2 - Ja) & (NJ2 - 1);
                 La = (NL2 - La) & (NL2 - 1);
                 Ia = (NI2 - Ia) & (NI2 - 1);
             }
-            const float4* ap = reinterpret_cast<const float4*>(ahat + (((long long)Ja * NL2 + La) * NI2 + Ia) * 6);
+            // < 6 * 8 N <= 6 * 2^27: a 32-bit offset from the base
+            const float4* ap = reinterpret_cast<const float4*>(ahat + ((Ja * NL2 + La) * NI2 + Ia) * 6u);
             const float4 q0 = ap[0], q1 = ap[1], q2 = ap[2];
             a[it][0] = make_float2(q0.x, q0.y);
             a[it][1] = make_float2(q0.z, q0.w);
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
             v[q] = make_float2(0.f, 0.f);
             if (f < nld) {
                 const int ii = f & (NI - 1), r = f >> LNI, t = r & (tj - 1), cc = r >> ltj;
-                if (act[RB == 1 ? 0 : cc / 3]) v[q] = src[(cc << lcs) + (t << lts) + ii];
+                if (act[RB == 1 ? 0 : (cc >= 3)]) v[q] = src[(cc << lcs) + (t << lts) + ii];
             }
         }
 #pragma unroll
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
             const int f = threadIdx.x + q * kBlock;
             if (f < nst) {
                 const int t = f & (tj - 1), r = f >> ltj, i = r & (NI - 1), cc = r >> LNI;
-                if (act[RB == 1 ? 0 : cc / 3]) dst[(r << lrow) + t] = cconj(s[(cc * tj + t) * LS2 + i]);
+                if (act[RB == 1 ? 0 : (cc >= 3)]) dst[(r << lrow) + t] = cconj(s[(cc * tj + t) * LS2 + i]);
             }
         }
     }
