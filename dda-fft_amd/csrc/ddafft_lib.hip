@@ -1362,6 +1362,7 @@ int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, i
         const float* ddirs = nullptr;
         long long stride = 0;
         Fin f;
+        int next_read = 0;        // next iteration whose solver states the host reads
         bool finishing = false;   // results queued back, event recorded
     };
     std::vector<Slot> act;
@@ -1381,15 +1382,46 @@ int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, i
         return DDAFFT_OK;
     };
     while (next < n && (int)act.size() < width) TRY(start(next++));
+    // Event-driven: each instance keeps up to two iterations queued beyond the
+    // last one whose stopping decisions the host has read, and reads them as
+    // they arrive, so fast instances run ahead of slow ones; the host blocks
+    // only when no instance can make progress.
+    auto read_states = [&](Slot& s, bool block) -> int {
+        const int it = s.next_read;
+        hipEvent_t e = s.r.c->ev[it & 3];
+        if (block) {
+            HIPCHECK(hipEventSynchronize(e));
+        } else {
+            const hipError_t q = hipEventQuery(e);
+            if (q == hipErrorNotReady) return 0;
+            HIPCHECK(q);
+        }
+        const SolveState* sr = s.r.c->state_ring + (size_t)(it & 3) * s.r.c->nrhs_max;
+        int m = 0;
+        for (int q = 0; q < s.r.nrhs; ++q) m = std::max(m, sr[q].stop_at);
+        s.r.known_stop = std::min(s.r.known_stop, m);
+        ++s.next_read;
+        return 1;
+    };
     while (!act.empty()) {
-        // one iteration for every instance still running, then the stop flags
-        for (Slot& s : act)
-            if (!s.finishing && run_more(s.r)) TRY(run_enqueue(s.r));
-        for (Slot& s : act)
-            if (!s.finishing) TRY(run_wait(s.r));
+        bool progress = false;
+        for (Slot& s : act) {
+            if (s.finishing) continue;
+            while (s.next_read < s.r.o) {   // iterations queued and not yet read
+                const int got = read_states(s, false);
+                if (got < 0) return got;
+                if (!got) break;
+                progress = true;
+            }
+            while (run_more(s.r) && s.r.o - s.next_read < 2) {
+                TRY(run_enqueue(s.r));
+                progress = true;
+            }
+        }
         // instances that are done queue their results back without waiting
         for (Slot& s : act) {
             if (s.finishing || run_more(s.r)) continue;
+            progress = true;
             const ddafft_job& j = jobs[s.job];
             TRY(fin_layout(s.r.c, j.nrhs, true, false, max_step, j.ndirs, s.f));
             TRY(fin_state_enqueue(s.r, s.f));
@@ -1397,32 +1429,36 @@ int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, i
             HIPCHECK(hipEventRecord(s.r.c->fin_ev, s.r.c->stream));
             s.finishing = true;
         }
-        // collect what has arrived; block only when nothing else can progress
-        bool all_finishing = true;
-        for (const Slot& s : act) all_finishing = all_finishing && s.finishing;
+        // collect what has arrived
         for (size_t k = 0; k < act.size();) {
             Slot& s = act[k];
             if (!s.finishing) {
                 ++k;
                 continue;
             }
-            hipError_t q = hipEventQuery(s.r.c->fin_ev);
-            if (q == hipErrorNotReady && all_finishing) {
-                HIPCHECK(hipEventSynchronize(s.r.c->fin_ev));
-                q = hipSuccess;
-                all_finishing = false;   // one wait per round
-            }
+            const hipError_t q = hipEventQuery(s.r.c->fin_ev);
             if (q == hipErrorNotReady) {
                 ++k;
                 continue;
             }
             HIPCHECK(q);
+            progress = true;
             const ddafft_job& j = jobs[s.job];
             std::vector<SolveState> sf;
             TRY(fin_state_complete(s.r, s.f, nullptr, j.info, sf));
             obs_complete(s.r.c, s.f, j.einc, j.esc, j.cabs, j.cext);
             act.erase(act.begin() + (long)k);
             if (next < n) TRY(start(next++));
+        }
+        if (progress || act.empty()) continue;
+        // nothing could move: wait for the oldest outstanding event of the
+        // first instance (its next stopping decision, or its results)
+        Slot& s = act.front();
+        if (s.finishing) {
+            HIPCHECK(hipEventSynchronize(s.r.c->fin_ev));
+        } else if (s.next_read < s.r.o) {
+            const int got = read_states(s, true);
+            if (got < 0) return got;
         }
     }
     return DDAFFT_OK;
