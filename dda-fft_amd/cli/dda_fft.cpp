@@ -676,6 +676,7 @@ int main(int argc, char** argv) {
                 j.cabs = cabs.data();
                 j.cext = cext.data();
                 j.info = info.data();
+                j.exact_phases = 1;   // the reference's glibc phases for Monte-Carlo incidences too
                 if (ddafft_solve_ex(ctx, &j, o.solver, (float)o.tau, o.max_steps, (float)o.tolerance, nullptr) !=
                     DDAFFT_OK) {
                     std::fprintf(stderr, "%s\n", ddafft_last_error());

@@ -151,6 +151,10 @@ struct ddafft_ctx {
     float2* inc_tab = nullptr;         // [nrhs][2][max(ni, nj, nl)] tabulated incident phases
     std::vector<int> inc_axis_h;       // host staging of the two (kept alive for async copies)
     std::vector<float2> inc_tab_h;
+    float2* inc_cell = nullptr;        // [nrhs][2][n] host-evaluated phases of any incidence (exact_phases)
+    size_t inc_cell_cap = 0;
+    std::vector<float2> inc_cell_h;
+    std::vector<int8_t> dipf_h;        // host copy of the dipole flags
     SolveState* st = nullptr;          // [nrhs]
     double* hist = nullptr;
     int hist_cap = 0;
@@ -551,9 +555,29 @@ Geometry make_geometry(const ddafft_ctx* c, const float* kinc) {
 // (dda_func.cpp:663-675, 1694-1700) and the C library's cosf/sinf.
 int inc_tab_len(const ddafft_ctx* c) { return std::max(c->d.ni, std::max(c->d.nj, c->d.nl)); }
 
-IncPhase inc_phase(const ddafft_ctx* c) { return IncPhase{c->inc_axis, c->inc_tab, inc_tab_len(c)}; }
+IncPhase inc_phase(const ddafft_ctx* c) { return IncPhase{c->inc_axis, c->inc_tab, inc_tab_len(c), c->inc_cell}; }
 
-int upload_incidence(ddafft_ctx* c, int nrhs, const float* kinc) {
+// Per-cell phases of right-hand side r (exact_phases): the reference's float
+// arguments (dda_func.cpp:663-675 for FirstApproximation, 1694-1700 for
+// CabsCext) through the C library's cosf/sinf, on the dipole cells.
+void cell_phases(const ddafft_ctx* c, const Geometry& g, const float* k, float2* t1, float2* t2) {
+    const int ni = c->d.ni, nj = c->d.nj, nl = c->d.nl;
+    const float kx = k[0], ky = k[1], kz = k[2];
+    size_t q = 0;
+    for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j)
+            for (int l = 0; l < nl; ++l, ++q) {
+                if (!c->dipf_h[q]) continue;
+                const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
+                const float dx = rx - g.cx, dy = ry - g.cy, dz = rz - g.cz;
+                const float a1 = kx * dx + ky * dy + kz * dz;
+                const float a2 = -(kx * (rx - g.ex) + ky * (ry - g.ey) + kz * (rz - g.ez));
+                t1[q] = make_float2(std::cos(a1), std::sin(a1));
+                t2[q] = make_float2(std::cos(a2), std::sin(a2));
+            }
+}
+
+int upload_incidence(ddafft_ctx* c, int nrhs, const float* kinc, bool exact = false) {
     if (!c->up) HIPCHECK(hipStreamSynchronize(c->stream));   // unstaged copies read the host vectors below
     const Geometry g = make_geometry(c, nullptr);
     c->inc_axis_h.assign(nrhs, -1);
@@ -561,10 +585,21 @@ int upload_incidence(ddafft_ctx* c, int nrhs, const float* kinc) {
     c->inc_tab_h.assign((size_t)nrhs * 2 * tn, make_float2(1.f, 0.f));
     const int n[3] = {c->d.ni, c->d.nj, c->d.nl};
     const float cen1[3] = {g.cx, g.cy, g.cz}, cen2[3] = {g.ex, g.ey, g.ez};
+    const long long ncell = c->d.n;
+    bool cells = false;
     for (int r = 0; r < nrhs; ++r) {
         const float* k = kinc + 3 * r;
         const int nz = (k[0] != 0.f) + (k[1] != 0.f) + (k[2] != 0.f);
-        if (nz != 1) continue;
+        if (nz != 1) {
+            if (!exact) continue;
+            if (!cells) {
+                c->inc_cell_h.assign((size_t)nrhs * 2 * ncell, make_float2(0.f, 0.f));
+                cells = true;
+            }
+            c->inc_axis_h[r] = -2;
+            cell_phases(c, g, k, &c->inc_cell_h[(size_t)r * 2 * ncell], &c->inc_cell_h[((size_t)r * 2 + 1) * ncell]);
+            continue;
+        }
         const int a = k[0] != 0.f ? 0 : (k[1] != 0.f ? 1 : 2);
         c->inc_axis_h[r] = a;
         float2* t1 = &c->inc_tab_h[(size_t)r * 2 * tn];
@@ -580,6 +615,17 @@ int upload_incidence(ddafft_ctx* c, int nrhs, const float* kinc) {
     TRY(upload(c, c->kinc, kinc, (size_t)nrhs * 3 * sizeof(float)));
     TRY(upload(c, c->inc_axis, c->inc_axis_h.data(), (size_t)nrhs * sizeof(int)));
     TRY(upload(c, c->inc_tab, c->inc_tab_h.data(), c->inc_tab_h.size() * sizeof(float2)));
+    if (cells) {
+        if (c->inc_cell_cap < c->inc_cell_h.size()) {
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            if (c->inc_cell) hipFree(c->inc_cell);
+            c->inc_cell = nullptr;
+            c->inc_cell_cap = 0;
+            TRY(dev_alloc(&c->inc_cell, c->inc_cell_h.size()));
+            c->inc_cell_cap = c->inc_cell_h.size();
+        }
+        TRY(upload(c, c->inc_cell, c->inc_cell_h.data(), c->inc_cell_h.size() * sizeof(float2)));
+    }
     return DDAFFT_OK;
 }
 
@@ -631,7 +677,7 @@ int ddafft_destroy(ddafft_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->ahat, c->tables, c->occ, c->dipf, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
                     c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
-                    c->dirs, c->esc, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab};
+                    c->dirs, c->esc, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab, c->inc_cell};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->state_ring) hipHostFree(c->state_ring);
@@ -782,9 +828,9 @@ int ddafft_set_target(ddafft_ctx* c, const signed char* occ, int nmat, const int
     c->ncells = (int)cells.size();
     c->ndip = (long long)cells.size();
     if ((rc = upload(c, c->occ, occ, (size_t)n))) return rc;
-    std::vector<int8_t> dipf((size_t)n);
-    for (long long k = 0; k < n; ++k) dipf[k] = mv[occ[k]] ? 1 : 0;
-    if ((rc = upload(c, c->dipf, dipf.data(), (size_t)n))) return rc;
+    c->dipf_h.assign((size_t)n, 0);
+    for (long long k = 0; k < n; ++k) c->dipf_h[k] = mv[occ[k]] ? 1 : 0;
+    if ((rc = upload(c, c->dipf, c->dipf_h.data(), (size_t)n))) return rc;
     if ((rc = upload(c, c->mv, mv, (size_t)nmat * sizeof(int)))) return rc;
     if ((rc = upload(c, c->cells, cells.data(), cells.size() * sizeof(int)))) return rc;
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1249,9 +1295,11 @@ int job_prepare(ddafft_ctx* c, const ddafft_job* j, int solver, int max_step, co
     if (ndirs < 0 || (ndirs > 0 && !j->esc)) return fail(DDAFFT_EINVAL, "bad direction arguments");
     {
         const size_t dir_bytes = (ndirs > 0 && j->dirs) ? (size_t)ndirs * 3 * sizeof(float) * (j->dirs_per_rhs ? nrhs : 1) : 0;
-        const size_t bytes = dir_bytes + (size_t)nrhs * (c->nmat * sizeof(float2) + 3 * sizeof(float2) +
-                                                         3 * sizeof(float) + sizeof(int) +
-                                                         2 * (size_t)inc_tab_len(c) * sizeof(float2) + sizeof(SolveState));
+        const size_t cell_bytes = j->exact_phases ? (size_t)nrhs * 2 * c->d.n * sizeof(float2) : 0;
+        const size_t bytes = dir_bytes + cell_bytes +
+                             (size_t)nrhs * (c->nmat * sizeof(float2) + 3 * sizeof(float2) + 3 * sizeof(float) +
+                                             sizeof(int) + 2 * (size_t)inc_tab_len(c) * sizeof(float2) +
+                                             sizeof(SolveState));
         TRY(upload_begin(c, bytes + 16 * 256));
     }
     *ddirs = nullptr;
@@ -1286,7 +1334,7 @@ int job_prepare(ddafft_ctx* c, const ddafft_job* j, int solver, int max_step, co
     d.nrhs = nrhs;
     TRY(upload(c, c->alpha, j->alpha, (size_t)nrhs * c->nmat * sizeof(float2)));
     TRY(upload(c, c->einc, j->einc, (size_t)nrhs * 3 * sizeof(float2)));
-    TRY(upload_incidence(c, nrhs, j->kinc));
+    TRY(upload_incidence(c, nrhs, j->kinc, j->exact_phases != 0));
     const Geometry g = make_geometry(c, nullptr);
     dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), nrhs);
     hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->kinc, inc_phase(c),

@@ -37,11 +37,13 @@ struct Geometry {
 // (complex.cpp:176-181), so they match it bit for bit.  axis[r] = -1 marks a
 // general incidence, whose phases are FP64 sincos rounded once to float.
 // tab: [nrhs][2][tn], tn = max(ni, nj, nl): FirstApproximation phases, then
-// CabsCext phases.
+// CabsCext phases.  axis[r] = -2 marks a right-hand side whose phases the
+// host evaluated per cell: cell [nrhs][2][n].
 struct IncPhase {
     const int* axis;
     const float2* tab;
     int tn;
+    const float2* cell;
 };
 
 __device__ __forceinline__ int axis_coord(int ax, int i, int j, int l) { return ax == 0 ? i : (ax == 1 ? j : l); }
@@ -69,7 +71,9 @@ __global__ __launch_bounds__(kBlock) void k_first_approx(Dims d, Geometry g, con
             const float dx = rx - g.cx, dy = ry - g.cy, dz = rz - g.cz;
             const float ph = kx * dx + ky * dy + kz * dz;
             const int ax = ip.axis[rhs];
-            const float2 c0 = ax >= 0 ? ip.tab[(size_t)rhs * 2 * ip.tn + axis_coord(ax, i, j, l)] : exp_i(ph);
+            const float2 c0 = ax >= 0    ? ip.tab[(size_t)rhs * 2 * ip.tn + axis_coord(ax, i, j, l)]
+                              : ax == -2 ? ip.cell[(size_t)rhs * 2 * d.n + k]
+                                         : exp_i(ph);
             const float2 c1 = cmul(c0, alpha[(long long)rhs * nmat + m]);
             o0 = cmul(c1, E0);
             o1 = cmul(c1, E1);
@@ -205,7 +209,9 @@ __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const 
         const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
         const float r3 = -(kx * (rx - g.ex) + ky * (ry - g.ey) + kz * (rz - g.ez));
         const int ax = ip.axis[rhs];
-        const float2 c0 = ax >= 0 ? ip.tab[((size_t)rhs * 2 + 1) * ip.tn + axis_coord(ax, i, j, l)] : exp_i(r3);
+        const float2 c0 = ax >= 0    ? ip.tab[((size_t)rhs * 2 + 1) * ip.tn + axis_coord(ax, i, j, l)]
+                          : ax == -2 ? ip.cell[((size_t)rhs * 2 + 1) * d.n + k]
+                                     : exp_i(r3);
         const float2 px = P[rb + k], py = P[rb + d.n + k], pz = P[rb + 2 * d.n + k];
         const float2 c5 = cadd(cmul(pz, e2), cadd(cmul(px, e0), cmul(py, e1)));
         acc[0] += (double)cmul(c0, c5).y;

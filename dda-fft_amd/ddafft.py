@@ -37,7 +37,7 @@ class Job(C.Structure):
     """ddafft_job of include/ddafft.h."""
     _fields_ = [("nrhs", C.c_int), ("alpha", C.c_void_p), ("kinc", C.c_void_p), ("einc", C.c_void_p),
                 ("ndirs", C.c_int), ("dirs", C.c_void_p), ("dirs_per_rhs", C.c_int), ("esc", C.c_void_p),
-                ("cabs", C.c_void_p), ("cext", C.c_void_p), ("info", C.c_void_p)]
+                ("cabs", C.c_void_p), ("cext", C.c_void_p), ("info", C.c_void_p), ("exact_phases", C.c_int)]
 
 _lib = None
 
@@ -252,7 +252,7 @@ class Context:
         return dict(f=f, grad_m=gm, grad_chi=gchi, esc=esc[:, :nd], cabs=cabs, cext=cext, s11=s11[:nd],
                     info=info)
 
-    def _job(self, alpha, kinc, einc, dirs, ndirs):
+    def _job(self, alpha, kinc, einc, dirs, ndirs, exact_phases=False):
         """Builds a Job and its output arrays; keeps the inputs alive in the result."""
         alpha = np.ascontiguousarray(np.atleast_2d(alpha), np.complex64)
         einc = np.ascontiguousarray(np.atleast_2d(einc), np.complex64)
@@ -276,7 +276,7 @@ class Context:
                    cext=np.zeros(nrhs, np.float32), info=np.zeros((nrhs, 4), np.int32), nd=nd,
                    keep=(alpha, kinc, einc, dirs))
         j = Job(nrhs, _p(alpha), _p(kinc), _p(einc), nd, _p(dirs), per_rhs, _p(out["esc"]), _p(out["cabs"]),
-                _p(out["cext"]), _p(out["info"]))
+                _p(out["cext"]), _p(out["info"]), int(bool(exact_phases)))
         return j, out
 
     @staticmethod
@@ -288,11 +288,14 @@ class Context:
                             restored=bool(out["info"][r, 2]), matvecs=int(out["info"][r, 3])))
         return res
 
-    def solve_ex(self, solver, alpha, kinc, einc, dirs=None, max_step=20, err_max=1e-3, tau=0.0, ndirs=None):
+    def solve_ex(self, solver, alpha, kinc, einc, dirs=None, max_step=20, err_max=1e-3, tau=0.0, ndirs=None,
+                 exact_phases=False):
         """Like solve, with kinc [nrhs, 3] per right-hand side and dirs either
-        [ndirs, 3] (shared) or [nrhs, ndirs, 3] (one set per right-hand side)."""
+        [ndirs, 3] (shared) or [nrhs, ndirs, 3] (one set per right-hand side).
+        exact_phases: incident phases of every incidence from the C library's
+        cosf/sinf on the host, as the reference evaluates them."""
         solver = SOLVERS.get(solver, solver)
-        j, out = self._job(alpha, kinc, einc, dirs, ndirs)
+        j, out = self._job(alpha, kinc, einc, dirs, ndirs, exact_phases)
         hist = np.zeros((j.nrhs, max_step + 1), np.float64)
         _check(_lib.ddafft_solve_ex(self._h, C.byref(j), int(solver), C.c_float(tau), int(max_step),
                                     C.c_float(err_max), _p(hist)))
@@ -456,8 +459,8 @@ def solve_batch(ctxs, jobs, solver, max_step=20, err_max=1e-3, tau=0.0, width=4)
     list of per-right-hand-side dicts per context."""
     load()
     solver = SOLVERS.get(solver, solver)
-    built = [ctx._job(jb["alpha"], jb["kinc"], jb["einc"], jb.get("dirs"), jb.get("ndirs")) for ctx, jb in
-             zip(ctxs, jobs)]
+    built = [ctx._job(jb["alpha"], jb["kinc"], jb["einc"], jb.get("dirs"), jb.get("ndirs"),
+                      jb.get("exact_phases", False)) for ctx, jb in zip(ctxs, jobs)]
     arr = (Job * len(built))(*[b[0] for b in built])
     hs = (C.c_void_p * len(ctxs))(*[c._h.value for c in ctxs])
     _check(_lib.ddafft_solve_batch(len(ctxs), hs, arr, int(solver), C.c_float(tau), int(max_step),

@@ -127,6 +127,12 @@ typedef struct {
     float* cabs;          /* [nrhs] (may be NULL) */
     float* cext;          /* [nrhs] (may be NULL) */
     int* info;            /* [nrhs][4] (may be NULL) */
+    int exact_phases;     /* 1: every right-hand side's incident phases exp(i k.(r - c)) are evaluated on
+                             the host with the C library's cosf/sinf on the reference's float argument
+                             (dda_func.cpp:663-675, 1694-1700), so the first approximation and the cross
+                             sections match the reference bit for bit for any incidence, at a host cost of
+                             two sincos per dipole and right-hand side; 0: only axis-aligned incidences
+                             are tabulated so, others use FP64 sincos rounded to float */
 } ddafft_job;
 
 /* ddafft_solve with per-right-hand-side incidence and directions. */

@@ -91,10 +91,34 @@ def main():
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
+        ok &= check_exact_phases(orc)
     print("ALL OK" if ok else "SOME CHECKS FAILED", flush=True)
     return 0 if ok else 1
 
 
+
+
+def check_exact_phases(orc):
+    """A rotated incidence with exact_phases: the same dipoles as the oracle
+    with the reference's (C library) phases, bit for bit."""
+    ok = True
+    dims, kd = (8, 8, 16), np.float32(0.5)
+    occ, _ = orc.target_ellipsoid(dims, 1.0, 1.0)
+    alpha = np.array([0, orc.polarizability(1, kd, 1.33 + 0.05j)], np.complex64)
+    st = np.array([99], np.uint64)
+    R = ddafft.orientation_mc(st)
+    k = ddafft.rotate(R, np.array([0, 0, 1], np.float32))
+    e = ddafft.rotate(R, np.eye(3, dtype=np.float32)[:2]).astype(np.complex64)
+    A = orc.build_ahat(dims, kd)
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, [0, 1])
+        res = ctx.solve_ex("bicgstab", alpha, np.stack([k, k]), e, max_step=8, err_max=1e-9, exact_phases=True)
+        for p in range(2):
+            o = orc.dda_fft(2, occ, [0, 1], alpha, kd, k, e[p], A, np.zeros((0, 3), np.float32), 8, 1e-9,
+                            return_p=True)
+            ok &= check(res[p]["steps"] == o["steps"] and np.array_equal(bits(ctx.dipoles(p)), bits(o["p"])),
+                        "exact phases, rotated incidence, pol %d" % p)
+    return ok
 
 def check_long_axes(orc):
     """256- and 128-point axes (the transform sizes of the 256^3 workload, 512

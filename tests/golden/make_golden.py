@@ -161,11 +161,21 @@ CLI_CASES = {
     "ldr16": "-t ellipsoid -g 16 16 32 --aspect 1 2 -x 3 -m 1.33 0.001 -P ldr -M bicgstab -e 1e-4 -S 40 -p 3 -s 20",
     "ildr16": "-t parallelepiped -g 16 --aspect 0.5 0.75 -x 2 -m 1.7 0.1 -P ildr -M bicgstab -e 1e-4 -S 40 "
               "-p 2 -s 20 --psi 30 --iota 40 --kappa 50",
+    # the sample loop: random targets regenerated per sample from the stream,
+    # Monte-Carlo orientations, worm bodies
+    "rand16": "-t random-ellipsoid -g 16 --aspect 1 0.75 -x 3 -m 1.5 0.01 -m 1.7 0.05 --seeds 3 2 -P cm "
+              "-M bicgstab -e 1e-4 -S 60 -p 2 -s 20 -n 3",
+    "mc16": "-t ellipsoid -g 16 --aspect 1 0.5 -x 3 -m 1.5 0.01 -P cm -M bicgstab -e 1e-4 -S 60 -O mc -p 2 -s 20 "
+            "-n 4",
+    "worms16": "-t anthrax -g 16 -x 2 -m 1.5 0.01 --monomer-radius 2 --monomer-length-ratio 1 --packing 0.05 "
+               "-M bicgstab -e 1e-4 -S 60 -p 1 -s 30 -n 2",
 }
 
 
-def cli_fixtures():
+def cli_fixtures(names=None):
     for name, args in CLI_CASES.items():
+        if names and name not in names:
+            continue
         with tempfile.TemporaryDirectory() as d:
             cmd = [CLI] + args.split() + ["-j", "1", "--seed", "1", "-o", "out.dat", "-D", "out.dst"]
             res = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=600)
@@ -188,6 +198,9 @@ def cli_fixtures():
 
 def main():
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s", "ref"], check=True)
+    if "--cli" in sys.argv:   # only the named command-line runs
+        cli_fixtures(sys.argv[sys.argv.index("--cli") + 1:])
+        return
     ref = orc.Reference()
     fft_fixtures(ref)
     lattice_fixtures(ref)

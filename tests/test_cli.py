@@ -18,10 +18,15 @@ ROOT = os.path.dirname(HERE)
 CLI = os.path.join(ROOT, "dda-fft_amd", "lib", "dda_fft")
 GOLDEN = os.path.join(HERE, "golden")
 
-CASES = {}
-for name in ("sphere32", "spheroid64", "cg16", "aa16", "simple16", "ldr16", "ildr16"):
+def _args(name):
     with open(os.path.join(GOLDEN, name + ".steps.txt")) as f:
-        CASES[name] = f.readline().split(":", 1)[1].strip()
+        return f.readline().split(":", 1)[1].strip()
+
+
+CASES = {name: _args(name) for name in ("sphere32", "spheroid64", "cg16", "aa16", "simple16", "ldr16", "ildr16")}
+# sample loops: (arguments, samples, tolerance on the significant Mueller elements)
+SAMPLE_CASES = {"rand16": (_args("rand16"), 3, 1e-4), "worms16": (_args("worms16"), 2, 1e-4),
+                "mc16": (_args("mc16"), 4, 1e-4)}
 
 
 def run(args, cwd, timeout=300):
@@ -146,6 +151,31 @@ def test_out_dat_matches_reference(cli, tmp_path, name):
     assert n == rn == 1
     assert abs(dca / rdca - 1) < 2e-5 and abs(dce / rdce - 1) < 2e-5
     assert dat_error(dmm.astype(np.float64), truth["mm"]) < 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(SAMPLE_CASES))
+def test_sample_loop_matches_reference(cli, tmp_path, name):
+    """Several samples of the reference's sample loop (main.cpp:774-996):
+    random targets regenerated from the stream each sample (rand16, worms16)
+    and Monte-Carlo orientations (mc16, rotated incidence, whose phases the
+    program evaluates with the C library as the reference does).  The solves
+    are bit-exact; only the reference's float far-field sums differ."""
+    args, nsamp, tol = SAMPLE_CASES[name]
+    print("case", name, flush=True)
+    r = run(args.split() + ["-j", "1", "--seed", "1", "-o", "out.dat", "-D", "out.dst"], tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    ca, ce, th, mm = read_dat(tmp_path / "out.dat")
+    rca, rce, rth, rmm = read_dat(os.path.join(GOLDEN, name + ".out.dat"))
+    assert np.array_equal(th, rth)
+    err = dat_error(mm, rmm)
+    print(name, "Mueller worst rel vs reference %.2e" % err, "Cabs", ca, rca, "Cext", ce, rce, flush=True)
+    assert err < tol
+    assert abs(ca / rca - 1) < tol and abs(ce / rce - 1) < tol
+    n, dca, dce, _ = read_dst(tmp_path / "out.dst", len(rth))
+    rn, rdca, rdce, _ = read_dst(os.path.join(GOLDEN, name + ".out.dst"), len(rth))
+    assert n == rn == nsamp
+    assert abs(dca / rdca - 1) < tol and abs(dce / rdce - 1) < tol
 
 
 @pytest.mark.gpu

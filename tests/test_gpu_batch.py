@@ -134,3 +134,25 @@ def test_batch_equals_single_solves(orc32):
         for c in ctxs:
             c.close()
 
+
+def test_exact_phases_rotated_incidence_bit_exact(orc32):
+    """exact_phases: the incident phases of Monte-Carlo incidences from the C
+    library's cosf/sinf on the host, as the reference evaluates them: the
+    solves then match the oracle's (the reference's phases) bit for bit."""
+    dims, kd, m = CASES[1]
+    inst = mc_instance(orc32, dims, kd, m, 2, 77)
+    A = orc32.build_ahat(inst["occ"].shape, inst["kd"])
+    print("case exact phases", dims, flush=True)
+    with ddafft.Context(inst["occ"].shape, inst["kd"], nrhs_max=4) as ctx:
+        ctx.set_target(inst["occ"], inst["mv"])
+        res = ctx.solve_ex("bicgstab", inst["alpha"], inst["kinc"], inst["einc"], dirs=inst["dirs"], max_step=30,
+                           err_max=1e-4, exact_phases=True)
+        for r, got in enumerate(res):
+            ref = orc32.dda_fft(2, inst["occ"], inst["mv"], inst["alpha"], inst["kd"], inst["kinc"][r],
+                                inst["einc"][r], A, inst["dirs"][r], max_step=30, err_max=1e-4, return_p=True)
+            P = ctx.dipoles(r)
+            print("rhs", r, "steps", got["steps"], ref["steps"], flush=True)
+            assert got["steps"] == ref["steps"] and got["step_min"] == ref["step_min"]
+            assert np.array_equal(P.view(np.uint32), ref["p"].astype(np.complex64).view(np.uint32))
+            assert abs(got["cext"] / ref["cext"] - 1) < 1e-5
+            assert np.max(np.abs(got["esc"] - ref["esc"])) <= 1e-5 * np.max(np.abs(ref["esc"]))
