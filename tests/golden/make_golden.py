@@ -169,6 +169,22 @@ CLI_CASES = {
             "-n 4",
     "worms16": "-t anthrax -g 16 -x 2 -m 1.5 0.01 --monomer-radius 2 --monomer-length-ratio 1 --packing 0.05 "
                "-M bicgstab -e 1e-4 -S 60 -p 1 -s 30 -n 2",
+    # the remaining target kinds, one short run each
+    "rsphere16": "-t random-sphere -g 16 -x 2 -m 1.5 0.01 -m 1.7 0.05 --seeds 3 2 -M bicgstab -e 1e-4 -S 60 "
+                 "-p 1 -s 30 -n 2",
+    "rpar16": "-t random-parallelepiped -g 16 --aspect 1 0.75 -x 2 -m 1.5 0.01 -m 1.7 0.05 --seeds 2 2 "
+              "-M bicgstab -e 1e-4 -S 60 -p 1 -s 30 -n 2",
+    "nibble16": "-t nibble-ellipsoid -g 16 --aspect 1 1 -x 2 -m 1.5 0.01 --seeds 3 --bites 2 --depth 0.2 "
+                "-M bicgstab -e 1e-4 -S 60 -p 1 -s 30 -n 2",
+    "hier16": "-t hierarchical -g 16 --aspect 1 1 -x 2 -m 1.5 0.01 -m 1.7 0.05 --seeds 2 3 --levels 2 "
+              "-M bicgstab -e 1e-4 -S 60 -p 1 -s 30 -n 2",
+    "bisph16": "-t bisphere -g 32 16 16 --radius-ratio 1.5 -x 3 -m 1.5 0.01 -m 1.7 0.05 -M bicgstab -e 1e-4 "
+               "-S 60 -p 1 -s 30",
+    "hedgehog16": "-t hedgehog -g 16 -x 2 -m 1.5 0.01 --monomer-radius 1.5 --monomer-length-ratio 2 --packing 0.05 "
+                  "-M bicgstab -e 1e-4 -S 60 -p 1 -s 30 -n 2",
+    "dead16": "-t dead-anthrax -g 16 -x 2 -m 1.5 0.01 --monomer-radius 2 --monomer-length-ratio 1 --packing 0.05 "
+              "-M bicgstab -e 1e-4 -S 60 -p 1 -s 30 -n 2",
+    "spec16": "-t sphere-spec -g 16 -x 2 -m 1.5 0.01 --packing 0.5 -M bicgstab -e 1e-4 -S 60 -p 1 -s 30",
 }
 
 
@@ -178,7 +194,7 @@ def cli_fixtures(names=None):
             continue
         with tempfile.TemporaryDirectory() as d:
             cmd = [CLI] + args.split() + ["-j", "1", "--seed", "1", "-o", "out.dat", "-D", "out.dst"]
-            res = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=600)
+            res = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=300)
             if res.returncode != 0:
                 raise RuntimeError("reference run failed: %s\n%s" % (name, res.stdout[-2000:]))
             with open(os.path.join(d, "out.dat")) as f:
