@@ -589,12 +589,21 @@ int main(int argc, char** argv) {
                 ddafft_polarizability(o.polarizability, kd, m, kinc, einc, &out[2 * (q + 1)]);
             }
         };
-        const int group = o.orientation == 1 && fixed_target ? o.group : 1;
-        ddafft_ctx* ctx = nullptr;
-        if (ddafft_create(ni, nj, nl, kd, 2 * group, &ctx) != DDAFFT_OK) {
-            std::fprintf(stderr, "%s\n", ddafft_last_error());
-            return 1;
-        }
+        // Samples go to the GPU in groups: a fixed target's Monte-Carlo
+        // orientations as right-hand sides of one solve; random targets (a new
+        // realization each sample) as instances of one batched solve, one
+        // context each (their number capped so that they fit HBM comfortably).
+        const long long ncell = (long long)ni * nj * nl;
+        const int group = fixed_target ? (o.orientation == 1 ? o.group : 1)
+                                       : std::max(1, (int)std::min<long long>(o.group, (1LL << 24) / ncell));
+        const int nctx = fixed_target ? 1 : group;
+        std::vector<ddafft_ctx*> ctxs(nctx, nullptr);
+        for (int q = 0; q < nctx; ++q)
+            if (ddafft_create(ni, nj, nl, kd, fixed_target ? 2 * group : 2, &ctxs[q]) != DDAFFT_OK) {
+                std::fprintf(stderr, "%s\n", ddafft_last_error());
+                return 1;
+            }
+        ddafft_ctx* ctx = ctxs[0];
         bool target_set = false;
         std::vector<float> esc;
         // fixed target and fixed orientation: every sample is the same solve
@@ -603,11 +612,7 @@ int main(int argc, char** argv) {
         bool have_cache = false;
         while (st.count < o.samples) {
             const int g = std::min(group, o.samples - st.count);
-            if (!fixed_target) {
-                if (generate() < 0) return fail_exit(1, "the target does not fit the lattice");
-                target_set = false;
-            }
-            if (!target_set) {
+            if (fixed_target && !target_set) {
                 if (ddafft_set_target(ctx, occ.data(), nmat, mv.data()) != DDAFFT_OK) {
                     std::fprintf(stderr, "%s\n", ddafft_last_error());
                     return 1;
@@ -631,6 +636,14 @@ int main(int argc, char** argv) {
                     float* es = &ee[12 * s];
                     float* ns = &nrm[(size_t)s * npl * 3];
                     std::vector<float> ds((size_t)nd * 3);
+                    // a random target's realization comes first in the sample (main.cpp:813-857)
+                    if (!fixed_target) {
+                        if (generate() < 0) return fail_exit(1, "the target does not fit the lattice");
+                        if (ddafft_set_target(ctxs[s], occ.data(), nmat, mv.data()) != DDAFFT_OK) {
+                            std::fprintf(stderr, "%s\n", ddafft_last_error());
+                            return 1;
+                        }
+                    }
                     if (o.orientation == 1) {
                         float rot[9];
                         ddafft_orientation_mc(&stream, rot);
@@ -677,8 +690,28 @@ int main(int argc, char** argv) {
                 j.cext = cext.data();
                 j.info = info.data();
                 j.exact_phases = 1;   // the reference's glibc phases for Monte-Carlo incidences too
-                if (ddafft_solve_ex(ctx, &j, o.solver, (float)o.tau, o.max_steps, (float)o.tolerance, nullptr) !=
-                    DDAFFT_OK) {
+                int rc = DDAFFT_OK;
+                if (fixed_target) {
+                    rc = ddafft_solve_ex(ctx, &j, o.solver, (float)o.tau, o.max_steps, (float)o.tolerance, nullptr);
+                } else {
+                    // one instance per sample: its two polarizations
+                    std::vector<ddafft_job> jobs(g, j);
+                    for (int s = 0; s < g; ++s) {
+                        ddafft_job& js = jobs[s];
+                        js.nrhs = 2;
+                        js.alpha = &alpha[(size_t)2 * s * nmat * 2];
+                        js.kinc = &kin[(size_t)2 * s * 3];
+                        js.einc = &ein[(size_t)2 * s * 6];
+                        js.dirs = &dr[(size_t)2 * s * nd * 3];
+                        js.esc = &esc[(size_t)2 * s * nd * 6];
+                        js.cabs = &cabs[2 * s];
+                        js.cext = &cext[2 * s];
+                        js.info = &info[(size_t)2 * s * 4];
+                    }
+                    rc = ddafft_solve_batch(g, ctxs.data(), jobs.data(), o.solver, (float)o.tau, o.max_steps,
+                                            (float)o.tolerance, g);
+                }
+                if (rc != DDAFFT_OK) {
                     std::fprintf(stderr, "%s\n", ddafft_last_error());
                     return 1;
                 }
@@ -714,7 +747,7 @@ int main(int argc, char** argv) {
             }
             std::fflush(stdout);
         }
-        ddafft_destroy(ctx);
+        for (ddafft_ctx* q : ctxs) ddafft_destroy(q);
     }
     std::fclose(sf);
 

@@ -231,3 +231,19 @@ def test_shape_file_target(cli, tmp_path):
     (tmp_path / "bad.txt").write_text("1\n20 0 0 1\n")
     r3 = run(["-t", "file", "-f", "bad.txt", "-o", "g.dat", "-D", "g.dst"] + common, tmp_path)
     assert r3.returncode == 3
+
+
+@pytest.mark.gpu
+def test_random_targets_batched_equal_one_by_one(cli, tmp_path):
+    """Random targets go to the GPU as instances of one batched solve
+    (--group); the results equal those of one sample at a time, bit for bit,
+    with Monte-Carlo orientations drawn after each realization."""
+    base = ["-t", "random-ellipsoid", "-g", "16", "--aspect", "1", "0.75", "-x", "3", "-m", "1.5", "0.01", "-m",
+            "1.7", "0.05", "--seeds", "3", "2", "-M", "bicgstab", "-e", "1e-4", "-S", "60", "-O", "mc", "-p", "2",
+            "-s", "30", "-n", "5", "--seed", "3"]
+    print("case random targets, group 1 vs 4", flush=True)
+    one = run(base + ["--group", "1", "-o", "a.dat", "-D", "a.dst"], tmp_path)
+    four = run(base + ["--group", "4", "-o", "b.dat", "-D", "b.dst"], tmp_path)
+    assert one.returncode == 0 and four.returncode == 0, one.stderr + four.stderr
+    assert (tmp_path / "a.dst").read_bytes() == (tmp_path / "b.dst").read_bytes()
+    assert read_dst(tmp_path / "a.dst", 7)[0] == 5
