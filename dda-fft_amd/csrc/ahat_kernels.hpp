@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_legacy_axis(int ni, int nj, int nl, 
     constexpr int TT = clamp16(pow2floor(8192 / (6 * LS)));
     constexpr int NB = ilog2c(NA);
     __shared__ float2 s[TT * 6 * LS];
-    const long long NI2 = 2 * ni, NJ2 = 2 * nj, NL2 = 2 * nl;
+    const long long NJ2 = 2 * nj, NL2 = 2 * nl;
     // tile axis extent and per-element / per-tile strides, in cmat units
     long long es, ts, base;
     int ext;
