@@ -245,10 +245,32 @@ struct StageRunner {
 // The same transforms for a line count fixed at compile time: every lane's
 // (line, butterfly) pairs are constants of its thread index, the loops unroll
 // completely and the index arithmetic reduces to shifts, masks and constant
-// multiplies.  Same lane mapping, same operations in the same order.
-template <int N, int NL, int T>
+// multiplies.  Same operations in the same order.
+//
+// Lane grouping: in stage T, consecutive lanes take G consecutive butterflies
+// of one line before moving to the next line (G = 1: one butterfly per line).
+// Which lane runs which butterfly does not change any result; G only decides
+// which LDS banks the lanes of a half-wave meet.  LaneGroup holds the choice
+// per (N, lines, line stride, stage).
+template <int N, int NL, int LS, int T>
+struct LaneGroup {
+    static constexpr int G = 1;
+};
+
+template <int G, int NL>
+__device__ __forceinline__ void lane_pos(unsigned f, unsigned& line, unsigned& b) {
+    if constexpr (G == 1) {
+        line = f % NL;
+        b = f / NL;
+    } else {
+        line = (f / G) % NL;
+        b = f % G + G * (f / (G * NL));
+    }
+}
+
+template <int N, int NL, int LS, int T>
 struct FixedStage {
-    static __device__ __forceinline__ void run(float2* s, int ls, const float2* __restrict__ tw) {
+    static __device__ __forceinline__ void run(float2* s, const float2* __restrict__ tw) {
         if constexpr (T < Plan<N>::NST) {
             constexpr int K = Plan<N>::K;
             constexpr int R = plan_radix(K, T);
@@ -257,13 +279,16 @@ struct FixedStage {
             constexpr int step = N / span;
             constexpr int W = NL * (N / R);
             constexpr int IT = (W + kBlock - 1) / kBlock;
+            constexpr int G = LaneGroup<N, NL, LS, T>::G;
+            static_assert((N / R) % G == 0, "lane group divides the butterflies of a line");
 #pragma unroll
             for (int it = 0; it < IT; ++it) {
                 const unsigned f = threadIdx.x + it * kBlock;
                 if (W % kBlock == 0 || f < (unsigned)W) {
-                    const unsigned line = f % NL, b = f / NL;
+                    unsigned line, b;
+                    lane_pos<G, NL>(f, line, b);
                     const unsigned g = b / H, i = b % H;
-                    float2* v = s + line * ls + g * span + i;
+                    float2* v = s + line * LS + g * span + i;
                     float2 z[R];
 #pragma unroll
                     for (int q = 0; q < R; ++q) z[q] = v[q * H];
@@ -285,24 +310,27 @@ struct FixedStage {
                 }
             }
             __syncthreads();
-            FixedStage<N, NL, T + 1>::run(s, ls, tw);
+            FixedStage<N, NL, LS, T + 1>::run(s, tw);
         }
     }
 };
 
-template <int N, int NL>
-__device__ __forceinline__ void fft_lds_fixed(float2* s, int ls, const float2* __restrict__ tw) {
+template <int N, int NL, int LS>
+__device__ __forceinline__ void fft_lds_fixed(float2* s, const float2* __restrict__ tw) {
     constexpr int K = Plan<N>::K;
     constexpr int R0 = plan_radix(K, 0);
     constexpr int W = NL * (N / R0);
     constexpr int IT = (W + kBlock - 1) / kBlock;
+    constexpr int G = LaneGroup<N, NL, LS, 0>::G;
+    static_assert((N / R0) % G == 0, "lane group divides the butterflies of a line");
     float2 z[IT][R0];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const unsigned f = threadIdx.x + it * kBlock;
         if (W % kBlock == 0 || f < (unsigned)W) {
-            const unsigned line = f % NL, b = f / NL;
-            const float2* v = s + line * ls + rest_natural<N>(b);
+            unsigned line, b;
+            lane_pos<G, NL>(f, line, b);
+            const float2* v = s + line * LS + rest_natural<N>(b);
 #pragma unroll
             for (int q = 0; q < R0; ++q) z[it][q] = v[q * (N / R0)];
             butterfly<R0>(z[it]);
@@ -313,22 +341,25 @@ __device__ __forceinline__ void fft_lds_fixed(float2* s, int ls, const float2* _
     for (int it = 0; it < IT; ++it) {
         const unsigned f = threadIdx.x + it * kBlock;
         if (W % kBlock == 0 || f < (unsigned)W) {
-            const unsigned line = f % NL, b = f / NL;
-            float2* v = s + line * ls + b * R0;
+            unsigned line, b;
+            lane_pos<G, NL>(f, line, b);
+            float2* v = s + line * LS + b * R0;
 #pragma unroll
             for (int q = 0; q < R0; ++q) v[q] = z[it][q];
         }
     }
     __syncthreads();
-    FixedStage<N, NL, 1>::run(s, ls, tw);
+    FixedStage<N, NL, LS, 1>::run(s, tw);
 }
 
 // MAXL: the most lines any call of this instance passes (sizes the registers
-// that carry stage 0 across its barrier).  Ends with a __syncthreads.
-template <int N, int MAXL>
-__device__ __forceinline__ void fft_lds(float2* s, int nlines, int ls, const float2* __restrict__ tw) {
+// that carry stage 0 across its barrier); LS: the line stride in float2.
+// Ends with a __syncthreads.
+template <int N, int MAXL, int LS>
+__device__ __forceinline__ void fft_lds(float2* s, int nlines, const float2* __restrict__ tw) {
+    constexpr int ls = LS;
     if (nlines == MAXL) {   // the usual case: full tiles
-        fft_lds_fixed<N, MAXL>(s, ls, tw);
+        fft_lds_fixed<N, MAXL, LS>(s, tw);
         return;
     }
     constexpr int K = Plan<N>::K;

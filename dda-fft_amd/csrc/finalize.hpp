@@ -71,9 +71,23 @@ __device__ __forceinline__ void finalize_block(SolveState* __restrict__ st, int 
                                                double* red) {
     double v[kSlots] = {0, 0, 0, 0, 0, 0};
     const double* p = fa.partials + (long long)rhs * fa.nblk * kSlots;
-    for (int b = threadIdx.x; b < fa.nblk; b += blockDim.x)
+    // each thread sums partials b = tid, tid + 256, ... in that order; the
+    // loads of four of them are in flight together
+    constexpr int kBatch = 4;
+    for (int b0 = threadIdx.x; b0 < fa.nblk; b0 += kBatch * kBlock) {
+        double t[kBatch][kSlots];
 #pragma unroll
-        for (int q = 0; q < kSlots; ++q) v[q] += p[(long long)b * kSlots + q];
+        for (int u = 0; u < kBatch; ++u) {
+            const int b = b0 + u * kBlock;
+#pragma unroll
+            for (int q = 0; q < kSlots; ++q) t[u][q] = b < fa.nblk ? p[(long long)b * kSlots + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (b0 + u * kBlock < fa.nblk)
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) v[q] += t[u][q];
+    }
     block_sum<kSlots>(v, red);
     if (threadIdx.x != 0) return;
     SolveState& s = st[rhs];

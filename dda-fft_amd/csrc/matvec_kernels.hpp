@@ -19,6 +19,11 @@
 //   YI  y inverse (fft_func.cpp:605-616), j < nj       B3 -> B4
 //   ZI  z inverse (fft_func.cpp:632-643), l < nl, fused with the solver
 //       epilogue and its FP64 partial sums             B4 -> out
+//
+// Each inverse line transform is conj(FFT(conj(v))) (ifft_line,
+// fft_kernel.hpp:301-322).  The conjugation after one axis and the one
+// before the next cancel exactly, so X conjugates on entry only, ZI on exit
+// only, and B3 and B4 hold the conjugated partial inverse transforms.
 #pragma once
 
 #include "device_types.hpp"
@@ -38,6 +43,20 @@ template <int N> struct TileZF { static constexpr int T = clamp16(pow2floor(6144
 template <int N, int RB> struct TileX {
     static constexpr int T = N >= 128 ? 2 : clamp16(pow2floor(2048 / (6 * RB * (N + kLpad))));
 };
+// Line strides (float2) of the X pass's forward (n_i points) and inverse
+// (2 n_i points) lines.  With the lane grouping of each stage (LaneGroup),
+// they decide how the lanes of a half-wave spread over the LDS banks.
+template <int NI, int RB> struct XStride {
+    static constexpr int S1 = NI + kLpad, S2 = 2 * NI + kLpad;
+};
+// n_i = 64, two right-hand sides (24 forward lines, 12 inverse lines per
+// block): strides 66 / 131 with the stage groupings below halve the modelled
+// bank conflicts (a 64-bank, 32-lanes-per-cycle model: 776 -> 420 extra
+// cycles per block) and took the pass from 52.2 to 51.0 us on MI355X.
+template <> struct XStride<64, 2> { static constexpr int S1 = 66, S2 = 131; };
+template <> struct LaneGroup<64, 24, 66, 0> { static constexpr int G = 2; };
+template <> struct LaneGroup<64, 24, 66, 1> { static constexpr int G = 2; };
+template <> struct LaneGroup<128, 12, 131, 2> { static constexpr int G = 32; };
 template <int N> struct TileYI { static constexpr int T = clamp16(pow2floor(6144 / (N + kLpad))); };
 // The Z-inverse pass runs the solver epilogue as well; at most four (i, j)
 // rows per block keeps more blocks in flight (64^3: 16.0 vs 17.1 us at 2 RHS,
@@ -150,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
     }
     for (int f = threadIdx.x; f < NL; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NL, 2 * TJ>(s, 2 * tj, LS, stw);
+    fft_lds<NL, 2 * TJ, LS>(s, 2 * tj, stw);
     const int ltj = __builtin_ctz(tj);
     float2* dst = b1 + ((long long)c * d.ni + i) * (2 * NL) * d.nj + j0;
     const int lnj = __builtin_ctz(d.nj);
@@ -206,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
     }
     for (int f = threadIdx.x; f < NJ; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NJ, 2 * TI>(s, 2 * ti, LS, stw);
+    fft_lds<NJ, 2 * TI, LS>(s, 2 * ti, stw);
     float2* dst = b2 + ((long long)c * 2 * NJ * NL2 + L) * d.ni + i0;
     constexpr int NST = (2 * NJ * TI + kBlock - 1) / kBlock;
 #pragma unroll
@@ -243,7 +262,8 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     constexpr int CB = 3 * RB;                       // component lines per (J, L)
     constexpr int TJ = TileX<NI, RB>::T;
     constexpr int NI2 = 2 * NI;
-    constexpr int LS1 = NI + kLpad, LS2 = NI2 + kLpad;   // forward / inverse line strides
+    constexpr int LS1 = XStride<NI, RB>::S1, LS2 = XStride<NI, RB>::S2;   // forward / inverse line strides
+    static_assert(LS2 <= 2 * LS1, "the inverse lines fit the forward tile's LDS");
     constexpr int ITEMS = (TJ * NI2 + kBlock - 1) / kBlock;
     __shared__ float2 s[CB * TJ * 2 * LS1];
     __shared__ float2 stw1[NI];
@@ -323,10 +343,10 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     for (int f = threadIdx.x; f < NI; f += blockDim.x) stw1[f] = tw1[f];
     for (int f = threadIdx.x; f < NI2; f += blockDim.x) stw2[f] = tw2[f];
     __syncthreads();
-    fft_lds<NI, 2 * CB * TJ>(s, 2 * CB * tj, LS1, stw1);
+    fft_lds<NI, 2 * CB * TJ, LS1>(s, 2 * CB * tj, stw1);
 
     // Â multiply in natural order of I, one Â read for the RB right-hand
-    // sides; results land digit-reversed and conjugated for the inverse
+    // sides; results land conjugated, in natural order, for the inverse
     // transform (ifft_line, fft_kernel.hpp:301-322).
     float2 y[ITEMS][CB];
 #pragma unroll
@@ -355,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
         }
     }
     __syncthreads();
-    fft_lds<NI2, CB * TJ>(s, CB * tj, LS2, stw2);
+    fft_lds<NI2, CB * TJ, LS2>(s, CB * tj, stw2);
     {
         float2* dst = b3 + (long long)(rhs0 * 3) * NI * ((long long)NL2 * NJ2) + (long long)L * NJ2 + J0;
         const int lrow = __builtin_ctz(NL2) + __builtin_ctz(NJ2);
@@ -366,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
             const int f = threadIdx.x + q * kBlock;
             if (f < nst) {
                 const int t = f & (tj - 1), r = f >> ltj, i = r & (NI - 1), cc = r >> LNI;
-                if (act[RB == 1 ? 0 : (cc >= 3)]) dst[(r << lrow) + t] = cconj(s[(cc * tj + t) * LS2 + i]);
+                if (act[RB == 1 ? 0 : (cc >= 3)]) dst[(r << lrow) + t] = s[(cc * tj + t) * LS2 + i];
             }
         }
     }
@@ -402,12 +422,12 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int f = threadIdx.x + q * kBlock;
-            if (f < tl * NJ2) s[(f >> LNJ2) * LS + (f & (NJ2 - 1))] = cconj(v[q]);
+            if (f < tl * NJ2) s[(f >> LNJ2) * LS + (f & (NJ2 - 1))] = v[q];
         }
     }
     for (int f = threadIdx.x; f < NJ2; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NJ2, TL>(s, tl, LS, stw);
+    fft_lds<NJ2, TL, LS>(s, tl, stw);
     constexpr int NJ = NJ2 / 2;
     const int ltl = __builtin_ctz(tl), lnl2 = __builtin_ctz(NL2);
     float2* dst = b4 + (((long long)c * d.ni + i) * NJ) * NL2 + L0;
@@ -417,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
         const int f = threadIdx.x + q * kBlock;
         if (f < tl * NJ) {
             const int j = f >> ltl, t = f - (j << ltl);
-            dst[(j << lnl2) + t] = cconj(s[t * LS + j]);
+            dst[(j << lnl2) + t] = s[t * LS + j];
         }
     }
 }
@@ -505,12 +525,12 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ 
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int f = threadIdx.x + q * kBlock;
-            if (f < nld) s[(f >> LNL2) * LS + (f & (NL2 - 1))] = cconj(v[q]);   // line comp * tj + t
+            if (f < nld) s[(f >> LNL2) * LS + (f & (NL2 - 1))] = v[q];   // line comp * tj + t
         }
     }
     for (int f = threadIdx.x; f < NL2; f += blockDim.x) stw[f] = tw[f];
     __syncthreads();
-    fft_lds<NL2, 3 * TJ>(s, 3 * tj, LS, stw);
+    fft_lds<NL2, 3 * TJ, LS>(s, 3 * tj, stw);
 
     // epilogue operands of every cell of this thread are loaded before the
     // first is used (each cell's outputs overwrite only its own operands)
