@@ -173,6 +173,8 @@ struct ddafft_ctx {
     size_t gaux_cap = 0;
     float2* esc = nullptr;
     int esc_cap = 0;
+    float2* fftab = nullptr;           // far-field per-axis phases [ndirs][ni + nj + nl]
+    size_t fftab_cap = 0;
     SolveState* state_ring = nullptr;  // pinned [4][nrhs_max]: solver states after each queued iteration
     hipEvent_t fin_ev = nullptr;       // marks the results of a batched solve as copied back
     // pinned staging of the per-solve transfers, so that they are truly
@@ -677,7 +679,8 @@ int ddafft_destroy(ddafft_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->ahat, c->tables, c->occ, c->dipf, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
                     c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
-                    c->dirs, c->esc, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab, c->inc_cell};
+                    c->dirs, c->esc, c->fftab, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab,
+                    c->inc_cell};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->state_ring) hipHostFree(c->state_ring);
@@ -1060,6 +1063,15 @@ int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int
         TRY(dev_alloc(&c->esc, (size_t)nrhs * ndirs * 3));
         c->esc_cap = nrhs * ndirs * 3;
     }
+    const size_t ntab = (size_t)ndirs * (c->d.ni + c->d.nj + c->d.nl);
+    if (ndirs > 0 && ntab > c->fftab_cap) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->fftab) hipFree(c->fftab);
+        c->fftab = nullptr;
+        c->fftab_cap = 0;
+        TRY(dev_alloc(&c->fftab, ntab));
+        c->fftab_cap = ntab;
+    }
     return pinned_reserve(c, &c->dn, &c->dn_cap, off);
 }
 
@@ -1080,17 +1092,29 @@ int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_s
     HIPCHECK(hipMemcpyAsync(c->dn + f.o_part, c->partials, (size_t)nrhs * f.cb * 2 * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
     if (ndirs > 0) {
-        // right-hand sides in pairs that share one direction set share the phases
+        // right-hand sides in pairs that share one direction set share the
+        // phases; the phase table is rebuilt (in stream order) only when the
+        // direction set changes
+        if (!c->fftab || (size_t)ndirs * (d.ni + d.nj + d.nl) > c->fftab_cap)
+            return fail(DDAFFT_ESTATE, "internal: far-field phase table not allocated");
+        const int ntab = ndirs * (d.ni + d.nj + d.nl);
+        const float* tab_dirs = nullptr;
         for (int r0 = 0; r0 < nrhs;) {
             const bool pair = r0 + 1 < nrhs && (dir_stride == 0 || c->dir_pairs[r0 / 2]);
             dim3 gf(f.fchunk, (ndirs + kFD - 1) / kFD);
             const float* dd = ddirs + r0 * dir_stride;
+            if (dd != tab_dirs) {
+                hipLaunchKernelGGL(k_ff_phase, std::min((ntab + kBlock - 1) / kBlock, 1024), kBlock, 0, c->stream, d,
+                                   g, dd, ndirs, c->fftab);
+                HIPCHECK(hipGetLastError());
+                tab_dirs = dd;
+            }
             if (pair)
-                hipLaunchKernelGGL((k_far_field<2>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, dd, ndirs,
-                                   r0, c->P, c->partials);
+                hipLaunchKernelGGL((k_far_field<2>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells, c->ncells,
+                                   ndirs, r0, c->P, c->partials);
             else
-                hipLaunchKernelGGL((k_far_field<1>), gf, kBlock, 0, c->stream, d, g, c->cells, c->ncells, dd, ndirs,
-                                   r0, c->P, c->partials);
+                hipLaunchKernelGGL((k_far_field<1>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells, c->ncells,
+                                   ndirs, r0, c->P, c->partials);
             HIPCHECK(hipGetLastError());
             r0 += pair ? 2 : 1;
         }

@@ -236,11 +236,12 @@ __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const 
 //   Esc(n) = S(n) - n (n . S(n)),   S(n) = sum_j exp(i n.(r_j - c)) P_j.
 // The transverse projection is linear, so it is applied once per direction
 // after the sum (k_far_field_sum).  The phase factorizes per axis,
-// exp(i n_x (i d - c_x)) exp(i n_y (j d - c_y)) exp(i n_z (l d - c_z)); each
-// block tabulates the three factors of its kFD directions in LDS (FP64
-// sincos rounded to float, arguments formed in float as the reference forms
-// its coordinates), so no transcendental is evaluated per dipole, and one
-// phase serves all R right-hand sides.
+// exp(i n_x (i d - c_x)) exp(i n_y (j d - c_y)) exp(i n_z (l d - c_z)).
+// k_ff_phase tabulates the three factors of every direction once per
+// direction set (FP64 sincos rounded to float, arguments formed in float as
+// the reference forms its coordinates); each block of k_far_field copies the
+// rows of its kFD directions into LDS, so no transcendental is evaluated per
+// dipole, and one phase serves all R right-hand sides.
 // partials: [nrhs][nchunk][ndirs][6] (x, y, z as re, im)
 // ---------------------------------------------------------------------------
 
@@ -261,9 +262,27 @@ __device__ __forceinline__ void cmac_fma(float2& acc, float2 a, float2 b) {
 // The far field does not reproduce the reference's sequential float sum in
 // any case (its block sums are FP64), so its products use fused
 // multiply-adds: a complex multiply-accumulate is four FMAs.
+// ptab: [ndirs][ni + nj + nl]
+__global__ __launch_bounds__(kBlock) void k_ff_phase(Dims d, Geometry g, const float* __restrict__ dirs, int ndirs,
+                                                     float2* __restrict__ ptab) {
+    const int naxis = d.ni + d.nj + d.nl;
+    const int total = ndirs * naxis;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < total; f += gridDim.x * blockDim.x) {
+        const int dd = f / naxis, a = f - dd * naxis;
+        float arg;
+        if (a < d.ni)
+            arg = dirs[3 * dd] * ((float)a * g.d0 - g.cx);
+        else if (a < d.ni + d.nj)
+            arg = dirs[3 * dd + 1] * ((float)(a - d.ni) * g.d0 - g.cy);
+        else
+            arg = dirs[3 * dd + 2] * ((float)(a - d.ni - d.nj) * g.d0 - g.cz);
+        ptab[f] = exp_i(arg);
+    }
+}
+
 template <int R>
-__global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const int* __restrict__ cells, int ncells,
-                                                      const float* __restrict__ dirs, int ndirs, int rhs0,
+__global__ __launch_bounds__(kBlock) void k_far_field(Dims d, const float2* __restrict__ ptab,
+                                                      const int* __restrict__ cells, int ncells, int ndirs, int rhs0,
                                                       const float2* __restrict__ P, double* __restrict__ partials) {
     constexpr int NV = R * 6;
     __shared__ float2 tab[kFD * kFFAxis];
@@ -274,14 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const 
     for (int f = threadIdx.x; f < kFD * naxis; f += blockDim.x) {
         const int q = f / naxis, a = f - q * naxis;
         const int dd = min(d0 + q, ndirs - 1);
-        float arg;
-        if (a < d.ni)
-            arg = dirs[3 * dd] * ((float)a * g.d0 - g.cx);
-        else if (a < d.ni + d.nj)
-            arg = dirs[3 * dd + 1] * ((float)(a - d.ni) * g.d0 - g.cy);
-        else
-            arg = dirs[3 * dd + 2] * ((float)(a - d.ni - d.nj) * g.d0 - g.cz);
-        tab[q * kFFAxis + a] = exp_i(arg);
+        tab[q * kFFAxis + a] = ptab[dd * naxis + a];
     }
     __syncthreads();
     float2 acc[kFD][R][3];
@@ -314,26 +326,41 @@ __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, Geometry g, const 
                 for (int c = 0; c < 3; ++c) cmac_fma(acc[q][r][c], ph, p[r][c]);
         }
     }
-    // one direction at a time through the FP64 block sum keeps few values live
+    // FP64 block sums, one direction at a time, through the phase table's LDS:
+    // value k of thread t goes to sums[k * kBlock + t]; thread (k, g) adds the
+    // copies of threads g, g + 16, ..., then thread k adds the 16 group sums,
+    // both in a fixed order.
+    static_assert(NV * kBlock * sizeof(double) <= sizeof(tab), "block sums fit the phase table");
+    static_assert(NV * 16 * sizeof(double) <= sizeof(red), "group sums fit");
+    double* sums = reinterpret_cast<double*>(tab);
+    const int tid = threadIdx.x;
+    __syncthreads();   // every phase read of the loop above is done
 #pragma unroll
     for (int q = 0; q < kFD; ++q) {
-        double v[NV];
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                v[(r * 3 + c) * 2] = acc[q][r][c].x;
-                v[(r * 3 + c) * 2 + 1] = acc[q][r][c].y;
+                sums[((r * 3 + c) * 2) * kBlock + tid] = acc[q][r][c].x;
+                sums[((r * 3 + c) * 2 + 1) * kBlock + tid] = acc[q][r][c].y;
             }
-        block_sum<NV>(v, red);
-        if (threadIdx.x == 0 && d0 + q < ndirs) {
+        __syncthreads();
+        if (tid < NV * 16) {
+            const int k = tid >> 4, g = tid & 15;
+            double t = 0.0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                double* o = partials + (((long long)(rhs0 + r) * nchunk + blockIdx.x) * ndirs + d0 + q) * 6;
-#pragma unroll
-                for (int e = 0; e < 6; ++e) o[e] = v[r * 6 + e];
-            }
+            for (int u = 0; u < kBlock / 16; ++u) t += sums[k * kBlock + u * 16 + g];
+            red[k * 16 + g] = t;
         }
+        __syncthreads();
+        if (tid < NV && d0 + q < ndirs) {
+            double t = 0.0;
+#pragma unroll
+            for (int g = 0; g < 16; ++g) t += red[tid * 16 + g];
+            const int r = tid / 6, e = tid - r * 6;
+            partials[(((long long)(rhs0 + r) * nchunk + blockIdx.x) * ndirs + d0 + q) * 6 + e] = t;
+        }
+        __syncthreads();   // sums and red are reused by the next direction
     }
 }
 
