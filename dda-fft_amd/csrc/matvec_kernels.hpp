@@ -26,6 +26,8 @@
 // only, and B3 and B4 hold the conjugated partial inverse transforms.
 #pragma once
 
+#include <type_traits>
+
 #include "device_types.hpp"
 #include "fft_common.hpp"
 #include "finalize.hpp"
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
             s[(tj + t) * LS + l] = cmul(wpre[l], v);
         }
     }
-    for (int f = threadIdx.x; f < NL; f += blockDim.x) stw[f] = tw[f];
+    for (int f = threadIdx.x; f < NL; f += kBlock) stw[f] = tw[f];
     __syncthreads();
     fft_lds<NL, 2 * TJ, LS>(s, 2 * tj, stw);
     const int ltj = __builtin_ctz(tj);
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
             }
         }
     }
-    for (int f = threadIdx.x; f < NJ; f += blockDim.x) stw[f] = tw[f];
+    for (int f = threadIdx.x; f < NJ; f += kBlock) stw[f] = tw[f];
     __syncthreads();
     fft_lds<NJ, 2 * TI, LS>(s, 2 * ti, stw);
     float2* dst = b2 + ((long long)c * 2 * NJ * NL2 + L) * d.ni + i0;
@@ -257,33 +259,23 @@ __device__ __forceinline__ void ahat_apply(const float2* a, float2 x, float2 y, 
     oz = cadd(cmul(a[5], z), cadd(cmul(a[2], x), cmul(a[4], y)));
 }
 
-template <int NI, int RB>
-__global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st, int iter,
-                                              const float2* __restrict__ b2, float2* __restrict__ b3,
-                                              const float2* __restrict__ ahat, int conj,
-                                              const float2* __restrict__ wpre, const float2* __restrict__ tw1,
-                                              const float2* __restrict__ tw2) {
-    static_assert(RB == 1 || RB == 2, "one or two right-hand sides per X tile");
+// One block's tile of the X pass.  FULL: the tile holds TJ columns (every
+// lattice with 2 n_j >= TJ), so the tile width is a compile-time constant and
+// the index arithmetic reduces to shifts and masks; otherwise tj = 2 n_j.
+template <int NI, int RB, bool FULL>
+__device__ __forceinline__ void x_tile(const Dims& d, const bool (&act)[RB], int tj_small,
+                                       const float2* __restrict__ b2, float2* __restrict__ b3,
+                                       const float2* __restrict__ ahat, int conj, const float2* __restrict__ wpre,
+                                       const float2* __restrict__ tw1, const float2* __restrict__ tw2, float2* s,
+                                       float2* stw1, float2* stw2) {
     constexpr int CB = 3 * RB;                       // component lines per (J, L)
     constexpr int TJ = TileX<NI, RB>::T;
     constexpr int NI2 = 2 * NI;
     constexpr int LS1 = XStride<NI, RB>::S1, LS2 = XStride<NI, RB>::S2;   // forward / inverse line strides
-    static_assert(LS2 <= 2 * LS1, "the inverse lines fit the forward tile's LDS");
     constexpr int ITEMS = (TJ * NI2 + kBlock - 1) / kBlock;
-    __shared__ float2 s[CB * TJ * 2 * LS1];
-    __shared__ float2 stw1[NI];
-    __shared__ float2 stw2[NI2];
     const int rhs0 = blockIdx.y * RB;
-    bool act[RB];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < RB; ++r) {
-        act[r] = rhs0 + r < d.nrhs && !(st && iter >= st[rhs0 + r].stop_at);
-        any = any || act[r];
-    }
-    if (!any) return;
     const int NJ2 = 2 * d.nj, NL2 = 2 * d.nl;
-    const int tj = min(TJ, NJ2), njt = NJ2 / tj;
+    const int tj = FULL ? TJ : tj_small, njt = NJ2 / tj;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int L = tile / njt, J0 = (tile - L * njt) * tj;
 
@@ -345,8 +337,10 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
             }
         }
     }
-    for (int f = threadIdx.x; f < NI; f += blockDim.x) stw1[f] = tw1[f];
-    for (int f = threadIdx.x; f < NI2; f += blockDim.x) stw2[f] = tw2[f];
+    // strides of the compile-time block size: reading blockDim here made the
+    // compiler wait for every load in flight (Â and the field) first
+    for (int f = threadIdx.x; f < NI; f += kBlock) stw1[f] = tw1[f];
+    for (int f = threadIdx.x; f < NI2; f += kBlock) stw2[f] = tw2[f];
     __syncthreads();
     fft_lds<NI, 2 * CB * TJ, LS1>(s, 2 * CB * tj, stw1);
 
@@ -397,6 +391,38 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     }
 }
 
+template <int NI, int RB>
+__global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st, int iter,
+                                              const float2* __restrict__ b2, float2* __restrict__ b3,
+                                              const float2* __restrict__ ahat, int conj,
+                                              const float2* __restrict__ wpre, const float2* __restrict__ tw1,
+                                              const float2* __restrict__ tw2) {
+    static_assert(RB == 1 || RB == 2, "one or two right-hand sides per X tile");
+    constexpr int TJ = TileX<NI, RB>::T;
+    constexpr int LS1 = XStride<NI, RB>::S1, LS2 = XStride<NI, RB>::S2;
+    static_assert(LS2 <= 2 * LS1, "the inverse lines fit the forward tile's LDS");
+    __shared__ float2 s[3 * RB * TJ * 2 * LS1];
+    __shared__ float2 stw1[NI];
+    __shared__ float2 stw2[2 * NI];
+    const int rhs0 = blockIdx.y * RB;
+    bool act[RB];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        act[r] = rhs0 + r < d.nrhs && !(st && iter >= st[rhs0 + r].stop_at);
+        any = any || act[r];
+    }
+    if (!any) return;
+    // Measured on MI355X (2 RHS), against a run-time tile width: n_i = 64
+    // 57.8 -> 56.3 us per launch, n_i = 128 612 -> 534 us, n_i = 256
+    // 6571 -> 5688 us.
+    const int tj = min(TJ, 2 * d.nj);
+    if (tj == TJ)
+        x_tile<NI, RB, true>(d, act, TJ, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
+    else
+        x_tile<NI, RB, false>(d, act, tj, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
+}
+
 // ---------------------------------------------------------------------------
 // YI: lines (c, i, L) along J; 2 n_j -> j < n_j.
 // ---------------------------------------------------------------------------
@@ -430,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
             if (f < tl * NJ2) s[(f >> LNJ2) * LS + (f & (NJ2 - 1))] = v[q];
         }
     }
-    for (int f = threadIdx.x; f < NJ2; f += blockDim.x) stw[f] = tw[f];
+    for (int f = threadIdx.x; f < NJ2; f += kBlock) stw[f] = tw[f];
     __syncthreads();
     fft_lds<NJ2, TL, LS>(s, tl, stw);
     constexpr int NJ = NJ2 / 2;
@@ -511,175 +537,183 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ 
         if (iter >= st[rhs].stop_at) return;
         rB = st[rhs].rB;
     }
-    const int tj = min(TJ, d.nj), njt = d.nj / tj;
-    const int i = blockIdx.x / njt, j0 = (blockIdx.x - i * njt) * tj;
-    const int ltj = __builtin_ctz(tj), lnj = __builtin_ctz(d.nj), lni = __builtin_ctz(d.ni);
-    {
-        const float2* src = b4 + (((long long)rhs * 3 * d.ni + i) * d.nj + j0) * NL2;
-        const int lcs = lni + lnj + LNL2;   // one component plane of B4
-        const int nld = 3 * tj * NL2;
-        float2 v[NLD];
+    // the tile body, with the row count a compile-time constant for full tiles
+    // (nj >= TJ) and a run-time one otherwise
+    auto tile = [&](auto tj_arg) {
+        const int tj = tj_arg, njt = d.nj / tj;
+        const int i = blockIdx.x / njt, j0 = (blockIdx.x - i * njt) * tj;
+        const int ltj = __builtin_ctz(tj), lnj = __builtin_ctz(d.nj), lni = __builtin_ctz(d.ni);
+        {
+            const float2* src = b4 + (((long long)rhs * 3 * d.ni + i) * d.nj + j0) * NL2;
+            const int lcs = lni + lnj + LNL2;   // one component plane of B4
+            const int nld = 3 * tj * NL2;
+            float2 v[NLD];
 #pragma unroll
-        for (int q = 0; q < NLD; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < nld) {
-                const int LL = f & (NL2 - 1), r = f >> LNL2, t = r & (tj - 1), comp = r >> ltj;
-                v[q] = src[(comp << lcs) + (t << LNL2) + LL];
+            for (int q = 0; q < NLD; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < nld) {
+                    const int LL = f & (NL2 - 1), r = f >> LNL2, t = r & (tj - 1), comp = r >> ltj;
+                    v[q] = src[(comp << lcs) + (t << LNL2) + LL];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NLD; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < nld) s[(f >> LNL2) * LS + (f & (NL2 - 1))] = v[q];   // line comp * tj + t
             }
         }
-#pragma unroll
-        for (int q = 0; q < NLD; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < nld) s[(f >> LNL2) * LS + (f & (NL2 - 1))] = v[q];   // line comp * tj + t
-        }
-    }
-    for (int f = threadIdx.x; f < NL2; f += blockDim.x) stw[f] = tw[f];
-    __syncthreads();
-    fft_lds<NL2, 3 * TJ, LS>(s, 3 * tj, stw);
+        for (int f = threadIdx.x; f < NL2; f += kBlock) stw[f] = tw[f];
+        __syncthreads();
+        fft_lds<NL2, 3 * TJ, LS>(s, 3 * tj, stw);
 
-    // epilogue operands of every cell of this thread are loaded before the
-    // first is used (each cell's outputs overwrite only its own operands)
-    const int kind = ea.kind;
-    const bool need_x = kind != EK_CONV && kind != EK_BICG_R0 && kind != EK_CG_R0 && kind != EK_AA_R0;
-    const bool need_y = kind == EK_BICG_H || kind == EK_CG_W || kind == EK_SIM;
-    const bool need_o2 = kind == EK_AA_SUPD;
-    const float2* al = ea.alpha + (long long)rhs * ea.nmat;
-    const long long rb = (long long)rhs * 3 * d.n;
-    const long long k0 = ((long long)i * d.nj + j0) * NLH;   // first cell of the block
-    const int nep = tj * NLH;
-    int kq[NEP];
-    bool dip[NEP];
-    float2 a[NEP], xr[NEP][3], yr[NEP][3], o2[NEP][3];
+        // epilogue operands of every cell of this thread are loaded before the
+        // first is used (each cell's outputs overwrite only its own operands)
+        const int kind = ea.kind;
+        const bool need_x = kind != EK_CONV && kind != EK_BICG_R0 && kind != EK_CG_R0 && kind != EK_AA_R0;
+        const bool need_y = kind == EK_BICG_H || kind == EK_CG_W || kind == EK_SIM;
+        const bool need_o2 = kind == EK_AA_SUPD;
+        const float2* al = ea.alpha + (long long)rhs * ea.nmat;
+        const long long rb = (long long)rhs * 3 * d.n;
+        const long long k0 = ((long long)i * d.nj + j0) * NLH;   // first cell of the block
+        const int nep = tj * NLH;
+        int kq[NEP];
+        bool dip[NEP];
+        float2 a[NEP], xr[NEP][3], yr[NEP][3], o2[NEP][3];
 #pragma unroll
-    for (int q = 0; q < NEP; ++q) {
-        const int f = threadIdx.x + q * kBlock;
-        kq[q] = f;   // cell k0 + f: row t = f / nl, l = f % nl
-        dip[q] = false;
-        a[q] = make_float2(0.f, 0.f);
-        if (f < nep) {
-            const int m = occ[k0 + f];
-            dip[q] = dipf[k0 + f] != 0;
-            a[q] = al[m];
+        for (int q = 0; q < NEP; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            kq[q] = f;   // cell k0 + f: row t = f / nl, l = f % nl
+            dip[q] = false;
+            a[q] = make_float2(0.f, 0.f);
+            if (f < nep) {
+                const int m = occ[k0 + f];
+                dip[q] = dipf[k0 + f] != 0;
+                a[q] = al[m];
 #pragma unroll
-            for (int c3 = 0; c3 < 3; ++c3) {
-                const long long ck = rb + c3 * d.n + k0 + f;
-                if (need_x) xr[q][c3] = ea.x[ck];
-                if (need_y) yr[q][c3] = ea.y[ck];
-                if (need_o2) o2[q][c3] = ea.out2[ck];
+                for (int c3 = 0; c3 < 3; ++c3) {
+                    const long long ck = rb + c3 * d.n + k0 + f;
+                    if (need_x) xr[q][c3] = ea.x[ck];
+                    if (need_y) yr[q][c3] = ea.y[ck];
+                    if (need_o2) o2[q][c3] = ea.out2[ck];
+                }
             }
         }
-    }
-    double acc[kSlots] = {0, 0, 0, 0, 0, 0};
+        double acc[kSlots] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int q = 0; q < NEP; ++q) {
-        const int f = kq[q];
-        if (f >= nep) continue;
-        const int t = f >> LNLH, l = f & (NLH - 1);
-        const long long k = k0 + f;
-        const bool dp = dip[q];
-        float2 av = a[q];
-        if (ea.conj) av.y = -av.y;
-        float2 cv[3], o[3];
+        for (int q = 0; q < NEP; ++q) {
+            const int f = kq[q];
+            if (f >= nep) continue;
+            const int t = f >> LNLH, l = f & (NLH - 1);
+            const long long k = k0 + f;
+            const bool dp = dip[q];
+            float2 av = a[q];
+            if (ea.conj) av.y = -av.y;
+            float2 cv[3], o[3];
 #pragma unroll
-        for (int comp = 0; comp < 3; ++comp) cv[comp] = cconj(s[(comp * tj + t) * LS + l]);
-        double nrm = 0;
-        switch (kind) {
-            case EK_CONV:
+            for (int comp = 0; comp < 3; ++comp) cv[comp] = cconj(s[(comp * tj + t) * LS + l]);
+            double nrm = 0;
+            switch (kind) {
+                case EK_CONV:
 #pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) ea.out[rb + c3 * d.n + k] = cv[c3];
-                break;
-            case EK_MV:
-            case EK_BICG_H:
-            case EK_BICG_W:
-            case EK_CG_W:
-            case EK_AA_S0:
-            case EK_AA_W:
+                    for (int c3 = 0; c3 < 3; ++c3) ea.out[rb + c3 * d.n + k] = cv[c3];
+                    break;
+                case EK_MV:
+                case EK_BICG_H:
+                case EK_BICG_W:
+                case EK_CG_W:
+                case EK_AA_S0:
+                case EK_AA_W:
 #pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) {
-                    const long long ck = rb + c3 * d.n + k;
-                    const float2 x = xr[q][c3];
-                    o[c3] = dp ? csub(x, cmul(av, cv[c3])) : make_float2(0.f, 0.f);
-                    ea.out[ck] = o[c3];
-                    if (!dp) continue;
-                    if (kind == EK_BICG_H) dot_acc(yr[q][c3], o[c3], acc[0], acc[1]);
-                    if (kind == EK_BICG_W) {
-                        dot_acc(o[c3], x, acc[0], acc[1]);
-                        dot_acc(o[c3], o[c3], acc[2], acc[3]);
-                    }
-                    if (kind == EK_CG_W) {
-                        dot_acc(x, o[c3], acc[0], acc[1]);
-                        dot_acc(yr[q][c3], x, acc[2], acc[3]);
-                    }
-                    if (kind == EK_AA_S0) nrm += norm_term(x);
-                    if (kind == EK_AA_W) acc[4] += norm_term(x);
-                }
-                if (kind == EK_AA_S0 && dp) acc[5] += sqrt(nrm);
-                break;
-            case EK_BICG_R0:
-            case EK_CG_R0:
-            case EK_AA_R0:
-            case EK_SIM_R0:
-#pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) {
-                    const long long ck = rb + c3 * d.n + k;
-                    o[c3] = dp ? cneg(cmul(av, cv[c3])) : make_float2(0.f, 0.f);
-                    ea.out[ck] = o[c3];
-                    if (kind == EK_BICG_R0) {
-                        ea.out2[ck] = o[c3];
-                        ea.out3[ck] = o[c3];
-                        if (dp) dot_acc(o[c3], o[c3], acc[0], acc[1]);
-                    }
-                    if (kind == EK_CG_R0) ea.out2[ck] = o[c3];
-                    if (kind == EK_AA_R0 && dp) acc[4] += norm_term(o[c3]);
-                    if (kind == EK_SIM_R0) {
+                    for (int c3 = 0; c3 < 3; ++c3) {
+                        const long long ck = rb + c3 * d.n + k;
                         const float2 x = xr[q][c3];
-                        ea.out2[ck] = x;                       // S = P0 (Copying)
-                        ea.out3[ck] = csub(x, o[c3]);          // P = S - R (all cells)
+                        o[c3] = dp ? csub(x, cmul(av, cv[c3])) : make_float2(0.f, 0.f);
+                        ea.out[ck] = o[c3];
+                        if (!dp) continue;
+                        if (kind == EK_BICG_H) dot_acc(yr[q][c3], o[c3], acc[0], acc[1]);
+                        if (kind == EK_BICG_W) {
+                            dot_acc(o[c3], x, acc[0], acc[1]);
+                            dot_acc(o[c3], o[c3], acc[2], acc[3]);
+                        }
+                        if (kind == EK_CG_W) {
+                            dot_acc(x, o[c3], acc[0], acc[1]);
+                            dot_acc(yr[q][c3], x, acc[2], acc[3]);
+                        }
+                        if (kind == EK_AA_S0) nrm += norm_term(x);
+                        if (kind == EK_AA_W) acc[4] += norm_term(x);
                     }
-                    if (dp) nrm += norm_term(o[c3]);
-                }
-                if ((kind == EK_CG_R0 || kind == EK_SIM_R0) && dp) acc[5] += sqrt(nrm);
-                break;
-            case EK_AA_SUPD:
+                    if (kind == EK_AA_S0 && dp) acc[5] += sqrt(nrm);
+                    break;
+                case EK_BICG_R0:
+                case EK_CG_R0:
+                case EK_AA_R0:
+                case EK_SIM_R0:
 #pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) {
-                    const long long ck = rb + c3 * d.n + k;
-                    if (dp) {
-                        const float2 w = csub(xr[q][c3], cmul(av, cv[c3]));
-                        ea.out2[ck] = cadd(w, rmul(rB, o2[q][c3]));
+                    for (int c3 = 0; c3 < 3; ++c3) {
+                        const long long ck = rb + c3 * d.n + k;
+                        o[c3] = dp ? cneg(cmul(av, cv[c3])) : make_float2(0.f, 0.f);
+                        ea.out[ck] = o[c3];
+                        if (kind == EK_BICG_R0) {
+                            ea.out2[ck] = o[c3];
+                            ea.out3[ck] = o[c3];
+                            if (dp) dot_acc(o[c3], o[c3], acc[0], acc[1]);
+                        }
+                        if (kind == EK_CG_R0) ea.out2[ck] = o[c3];
+                        if (kind == EK_AA_R0 && dp) acc[4] += norm_term(o[c3]);
+                        if (kind == EK_SIM_R0) {
+                            const float2 x = xr[q][c3];
+                            ea.out2[ck] = x;                       // S = P0 (Copying)
+                            ea.out3[ck] = csub(x, o[c3]);          // P = S - R (all cells)
+                        }
+                        if (dp) nrm += norm_term(o[c3]);
                     }
-                }
-                break;
-            case EK_SIM:
+                    if ((kind == EK_CG_R0 || kind == EK_SIM_R0) && dp) acc[5] += sqrt(nrm);
+                    break;
+                case EK_AA_SUPD:
 #pragma unroll
-                for (int c3 = 0; c3 < 3; ++c3) {
-                    const long long ck = rb + c3 * d.n + k;
-                    if (dp) {
-                        const float2 x = xr[q][c3];
-                        const float2 w = csub(x, cmul(av, cv[c3]));
-                        const float2 rr = csub(w, yr[q][c3]);
-                        ea.out[ck] = rr;
-                        ea.out2[ck] = csub(x, rmul(ea.tau, rr));
-                        nrm += norm_term(rr);
-                    } else {
-                        ea.out[ck] = make_float2(0.f, 0.f);
-                        ea.out2[ck] = make_float2(0.f, 0.f);
+                    for (int c3 = 0; c3 < 3; ++c3) {
+                        const long long ck = rb + c3 * d.n + k;
+                        if (dp) {
+                            const float2 w = csub(xr[q][c3], cmul(av, cv[c3]));
+                            ea.out2[ck] = cadd(w, rmul(rB, o2[q][c3]));
+                        }
                     }
-                }
-                if (dp) acc[5] += sqrt(nrm);
-                break;
-            default:
-                break;
+                    break;
+                case EK_SIM:
+#pragma unroll
+                    for (int c3 = 0; c3 < 3; ++c3) {
+                        const long long ck = rb + c3 * d.n + k;
+                        if (dp) {
+                            const float2 x = xr[q][c3];
+                            const float2 w = csub(x, cmul(av, cv[c3]));
+                            const float2 rr = csub(w, yr[q][c3]);
+                            ea.out[ck] = rr;
+                            ea.out2[ck] = csub(x, rmul(ea.tau, rr));
+                            nrm += norm_term(rr);
+                        } else {
+                            ea.out[ck] = make_float2(0.f, 0.f);
+                            ea.out2[ck] = make_float2(0.f, 0.f);
+                        }
+                    }
+                    if (dp) acc[5] += sqrt(nrm);
+                    break;
+                default:
+                    break;
+            }
         }
-    }
-    if (ea.partials) {
-        block_sum<kSlots>(acc, red);
-        if (threadIdx.x == 0) {
-            double* p = ea.partials + ((long long)rhs * gridDim.x + blockIdx.x) * kSlots;
+        if (ea.partials) {
+            block_sum<kSlots>(acc, red);
+            if (threadIdx.x == 0) {
+                double* p = ea.partials + ((long long)rhs * gridDim.x + blockIdx.x) * kSlots;
 #pragma unroll
-            for (int q = 0; q < kSlots; ++q) p[q] = acc[q];
+                for (int q = 0; q < kSlots; ++q) p[q] = acc[q];
+            }
         }
-    }
+    };
+    if (d.nj >= TJ)
+        tile(std::integral_constant<int, TJ>{});
+    else
+        tile(d.nj);
 }
 
 }  // namespace ddafft
