@@ -118,78 +118,86 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
         // only the first matvec of an iteration carries the P -> P_min copy
         snap = (pa.Pmin != nullptr && pa.P != nullptr) ? st[rhs].snapshot : 0;
     }
-    const int tj = min(TJ, d.nj), ntile = d.nj / tj;
-    const int i = blockIdx.x / ntile, j0 = (blockIdx.x - i * ntile) * tj;
-    const long long cell0 = ((long long)i * d.nj + j0) * NL;
-    const long long cb = (long long)c * d.n + cell0;   // this block's cells of component c
-    const int nld = tj * NL;
-    // every load of the thread is issued before the first is used: flags and
-    // the prologue's operands (the prologue writes only cells it has read)
-    bool dip[NLD];
-    float2 va[NLD], vb[NLD], vd[NLD], vp[NLD];
-    const int mode = pa.mode;
+    // the tile body, with the tile width a compile-time constant for full
+    // tiles and a run-time one otherwise
+    auto tile = [&](auto tile_arg) {
+        const int tj = tile_arg, ntile = d.nj / tj;
+        const int i = blockIdx.x / ntile, j0 = (blockIdx.x - i * ntile) * tj;
+        const long long cell0 = ((long long)i * d.nj + j0) * NL;
+        const long long cb = (long long)c * d.n + cell0;   // this block's cells of component c
+        const int nld = tj * NL;
+        // every load of the thread is issued before the first is used: flags and
+        // the prologue's operands (the prologue writes only cells it has read)
+        bool dip[NLD];
+        float2 va[NLD], vb[NLD], vd[NLD], vp[NLD];
+        const int mode = pa.mode;
 #pragma unroll
-    for (int q = 0; q < NLD; ++q) {
-        const int f = threadIdx.x + q * kBlock;
-        dip[q] = false;
-        va[q] = vb[q] = vd[q] = vp[q] = make_float2(0.f, 0.f);
-        if (f < nld) {
-            dip[q] = dipf[cell0 + f] != 0;
-            va[q] = pa.A[cb + f];
-            if (mode != PRO_COPY) vb[q] = pa.B[cb + f];
-            if (mode == PRO_LC4) vd[q] = pa.D[cb + f];
-            if (snap) vp[q] = pa.P[cb + f];   // P is never a prologue output
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < NLD; ++q) {
-        const int f = threadIdx.x + q * kBlock;
-        if (f < nld) {
-            float2 v;
-            switch (mode) {
-                case PRO_LC1:   // dda_func.cpp:884-905
-                    v = va[q];
-                    if (dip[q]) {
-                        v = cadd(va[q], cmul(c1, vb[q]));
-                        pa.A[cb + f] = v;
-                    }
-                    break;
-                case PRO_LC4:   // :843-882
-                    v = dip[q] ? cadd(cmul(c2, vd[q]), cadd(va[q], cmul(c1, vb[q]))) : make_float2(0.f, 0.f);
-                    pa.B[cb + f] = v;
-                    break;
-                case PRO_LC2:   // :906-924
-                    v = vb[q];
-                    if (dip[q]) {
-                        v = cadd(va[q], cmul(c1, vb[q]));
-                        pa.B[cb + f] = v;
-                    }
-                    break;
-                default:
-                    v = va[q];
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            dip[q] = false;
+            va[q] = vb[q] = vd[q] = vp[q] = make_float2(0.f, 0.f);
+            if (f < nld) {
+                dip[q] = dipf[cell0 + f] != 0;
+                va[q] = pa.A[cb + f];
+                if (mode != PRO_COPY) vb[q] = pa.B[cb + f];
+                if (mode == PRO_LC4) vd[q] = pa.D[cb + f];
+                if (snap) vp[q] = pa.P[cb + f];   // P is never a prologue output
             }
-            if (snap) pa.Pmin[cb + f] = vp[q];
-            const int t = f >> LNL, l = f & (NL - 1);
-            s[t * LS + l] = v;
-            s[(tj + t) * LS + l] = cmul(wpre[l], v);
         }
-    }
-    for (int f = threadIdx.x; f < NL; f += kBlock) stw[f] = tw[f];
-    __syncthreads();
-    fft_lds<NL, 2 * TJ, LS>(s, 2 * tj, stw);
-    const int ltj = __builtin_ctz(tj);
-    float2* dst = b1 + ((long long)c * d.ni + i) * (2 * NL) * d.nj + j0;
-    const int lnj = __builtin_ctz(d.nj);
-    constexpr int NST = (2 * NL * TJ + kBlock - 1) / kBlock;
 #pragma unroll
-    for (int q = 0; q < NST; ++q) {
-        const int f = threadIdx.x + q * kBlock;
-        if (f < 2 * NL * tj) {
-            const int L = f >> ltj, t = f - (L << ltj);
-            const float2 v = (L & 1) ? s[(tj + t) * LS + (L >> 1)] : s[t * LS + (L >> 1)];
-            dst[(L << lnj) + t] = v;
+        for (int q = 0; q < NLD; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < nld) {
+                float2 v;
+                switch (mode) {
+                    case PRO_LC1:   // dda_func.cpp:884-905
+                        v = va[q];
+                        if (dip[q]) {
+                            v = cadd(va[q], cmul(c1, vb[q]));
+                            pa.A[cb + f] = v;
+                        }
+                        break;
+                    case PRO_LC4:   // :843-882
+                        v = dip[q] ? cadd(cmul(c2, vd[q]), cadd(va[q], cmul(c1, vb[q]))) : make_float2(0.f, 0.f);
+                        pa.B[cb + f] = v;
+                        break;
+                    case PRO_LC2:   // :906-924
+                        v = vb[q];
+                        if (dip[q]) {
+                            v = cadd(va[q], cmul(c1, vb[q]));
+                            pa.B[cb + f] = v;
+                        }
+                        break;
+                    default:
+                        v = va[q];
+                }
+                if (snap) pa.Pmin[cb + f] = vp[q];
+                const int t = f >> LNL, l = f & (NL - 1);
+                s[t * LS + l] = v;
+                s[(tj + t) * LS + l] = cmul(wpre[l], v);
+            }
         }
-    }
+        for (int f = threadIdx.x; f < NL; f += kBlock) stw[f] = tw[f];
+        __syncthreads();
+        fft_lds<NL, 2 * TJ, LS>(s, 2 * tj, stw);
+        const int ltj = __builtin_ctz(tj);
+        float2* dst = b1 + ((long long)c * d.ni + i) * (2 * NL) * d.nj + j0;
+        const int lnj = __builtin_ctz(d.nj);
+        constexpr int NST = (2 * NL * TJ + kBlock - 1) / kBlock;
+#pragma unroll
+        for (int q = 0; q < NST; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < 2 * NL * tj) {
+                const int L = f >> ltj, t = f - (L << ltj);
+                const float2 v = (L & 1) ? s[(tj + t) * LS + (L >> 1)] : s[t * LS + (L >> 1)];
+                dst[(L << lnj) + t] = v;
+            }
+        }
+    };
+    if (d.nj >= TJ)
+        tile(std::integral_constant<int, TJ>{});
+    else
+        tile(d.nj);
 }
 
 // ---------------------------------------------------------------------------
@@ -208,42 +216,50 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
     if (st && iter >= st[rhs].stop_at) return;
     constexpr int LNJ = ilog2c(NJ);
     const int NL2 = 2 * d.nl;
-    const int ti = min(TI, d.ni), nit = d.ni / ti;
-    const int L = blockIdx.x / nit, i0 = (blockIdx.x - L * nit) * ti;
-    const int lnl2 = __builtin_ctz(NL2), lni = __builtin_ctz(d.ni), lti = __builtin_ctz(ti);
-    {
-        constexpr int NLD = (TI * NJ + kBlock - 1) / kBlock;
-        const float2* src = b1 + (((long long)c * d.ni + i0) * NL2 + L) * NJ;
-        float2 v[NLD];
+    // the tile body, with the tile width a compile-time constant for full
+    // tiles and a run-time one otherwise
+    auto tile = [&](auto tile_arg) {
+        const int ti = tile_arg, nit = d.ni / ti;
+        const int L = blockIdx.x / nit, i0 = (blockIdx.x - L * nit) * ti;
+        const int lnl2 = __builtin_ctz(NL2), lni = __builtin_ctz(d.ni), lti = __builtin_ctz(ti);
+        {
+            constexpr int NLD = (TI * NJ + kBlock - 1) / kBlock;
+            const float2* src = b1 + (((long long)c * d.ni + i0) * NL2 + L) * NJ;
+            float2 v[NLD];
 #pragma unroll
-        for (int q = 0; q < NLD; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < ti * NJ) v[q] = src[((f >> LNJ) << (lnl2 + LNJ)) + (f & (NJ - 1))];
-        }
+            for (int q = 0; q < NLD; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < ti * NJ) v[q] = src[((f >> LNJ) << (lnl2 + LNJ)) + (f & (NJ - 1))];
+            }
 #pragma unroll
-        for (int q = 0; q < NLD; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < ti * NJ) {
-                const int t = f >> LNJ, jj = f & (NJ - 1);
-                s[t * LS + jj] = v[q];
-                s[(ti + t) * LS + jj] = cmul(wpre[jj], v[q]);
+            for (int q = 0; q < NLD; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < ti * NJ) {
+                    const int t = f >> LNJ, jj = f & (NJ - 1);
+                    s[t * LS + jj] = v[q];
+                    s[(ti + t) * LS + jj] = cmul(wpre[jj], v[q]);
+                }
             }
         }
-    }
-    for (int f = threadIdx.x; f < NJ; f += kBlock) stw[f] = tw[f];
-    __syncthreads();
-    fft_lds<NJ, 2 * TI, LS>(s, 2 * ti, stw);
-    float2* dst = b2 + ((long long)c * 2 * NJ * NL2 + L) * d.ni + i0;
-    constexpr int NST = (2 * NJ * TI + kBlock - 1) / kBlock;
+        for (int f = threadIdx.x; f < NJ; f += kBlock) stw[f] = tw[f];
+        __syncthreads();
+        fft_lds<NJ, 2 * TI, LS>(s, 2 * ti, stw);
+        float2* dst = b2 + ((long long)c * 2 * NJ * NL2 + L) * d.ni + i0;
+        constexpr int NST = (2 * NJ * TI + kBlock - 1) / kBlock;
 #pragma unroll
-    for (int q = 0; q < NST; ++q) {
-        const int f = threadIdx.x + q * kBlock;
-        if (f < 2 * NJ * ti) {
-            const int J = f >> lti, t = f - (J << lti);
-            const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
-            dst[(J << (lnl2 + lni)) + t] = v;
+        for (int q = 0; q < NST; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (f < 2 * NJ * ti) {
+                const int J = f >> lti, t = f - (J << lti);
+                const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
+                dst[(J << (lnl2 + lni)) + t] = v;
+            }
         }
-    }
+    };
+    if (d.ni >= TI)
+        tile(std::integral_constant<int, TI>{});
+    else
+        tile(d.ni);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,39 +454,47 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
     const int c = blockIdx.y, rhs = c / 3;
     if (st && iter >= st[rhs].stop_at) return;
     const int NL2 = 2 * d.nl;
-    const int tl = min(TL, NL2), nlt = NL2 / tl;
-    const int i = blockIdx.x / nlt, L0 = (blockIdx.x - i * nlt) * tl;
-    constexpr int LNJ2 = ilog2c(NJ2);
-    const float2* src = b3 + (((long long)c * d.ni + i) * NL2 + L0) * NJ2;
-    {
-        constexpr int NLD = (TL * NJ2 + kBlock - 1) / kBlock;
-        float2 v[NLD];
+    // the tile body, with the tile width a compile-time constant for full
+    // tiles and a run-time one otherwise
+    auto tile = [&](auto tile_arg) {
+        const int tl = tile_arg, nlt = NL2 / tl;
+        const int i = blockIdx.x / nlt, L0 = (blockIdx.x - i * nlt) * tl;
+        constexpr int LNJ2 = ilog2c(NJ2);
+        const float2* src = b3 + (((long long)c * d.ni + i) * NL2 + L0) * NJ2;
+        {
+            constexpr int NLD = (TL * NJ2 + kBlock - 1) / kBlock;
+            float2 v[NLD];
 #pragma unroll
-        for (int q = 0; q < NLD; ++q) {
+            for (int q = 0; q < NLD; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < tl * NJ2) v[q] = src[f];
+            }
+#pragma unroll
+            for (int q = 0; q < NLD; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < tl * NJ2) s[(f >> LNJ2) * LS + (f & (NJ2 - 1))] = v[q];
+            }
+        }
+        for (int f = threadIdx.x; f < NJ2; f += kBlock) stw[f] = tw[f];
+        __syncthreads();
+        fft_lds<NJ2, TL, LS>(s, tl, stw);
+        constexpr int NJ = NJ2 / 2;
+        const int ltl = __builtin_ctz(tl), lnl2 = __builtin_ctz(NL2);
+        float2* dst = b4 + (((long long)c * d.ni + i) * NJ) * NL2 + L0;
+        constexpr int NST = (TL * NJ + kBlock - 1) / kBlock;
+#pragma unroll
+        for (int q = 0; q < NST; ++q) {
             const int f = threadIdx.x + q * kBlock;
-            if (f < tl * NJ2) v[q] = src[f];
+            if (f < tl * NJ) {
+                const int j = f >> ltl, t = f - (j << ltl);
+                dst[(j << lnl2) + t] = s[t * LS + j];
+            }
         }
-#pragma unroll
-        for (int q = 0; q < NLD; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < tl * NJ2) s[(f >> LNJ2) * LS + (f & (NJ2 - 1))] = v[q];
-        }
-    }
-    for (int f = threadIdx.x; f < NJ2; f += kBlock) stw[f] = tw[f];
-    __syncthreads();
-    fft_lds<NJ2, TL, LS>(s, tl, stw);
-    constexpr int NJ = NJ2 / 2;
-    const int ltl = __builtin_ctz(tl), lnl2 = __builtin_ctz(NL2);
-    float2* dst = b4 + (((long long)c * d.ni + i) * NJ) * NL2 + L0;
-    constexpr int NST = (TL * NJ + kBlock - 1) / kBlock;
-#pragma unroll
-    for (int q = 0; q < NST; ++q) {
-        const int f = threadIdx.x + q * kBlock;
-        if (f < tl * NJ) {
-            const int j = f >> ltl, t = f - (j << ltl);
-            dst[(j << lnl2) + t] = s[t * LS + j];
-        }
-    }
+    };
+    if (NL2 >= TL)
+        tile(std::integral_constant<int, TL>{});
+    else
+        tile(NL2);
 }
 
 // ---------------------------------------------------------------------------
