@@ -422,10 +422,14 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sums NV values over the block; the result is valid in thread 0.
+// Sums NV values over a block of kBlock threads; the result is valid in
+// thread 0.  The wave count is the compile-time block size: reading blockDim
+// would make the compiler wait for every memory operation in flight, the
+// caller's stores included, before the sum.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch /* >= NV*16 doubles */) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    constexpr int nw = kBlock >> 6;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
     __syncthreads();
