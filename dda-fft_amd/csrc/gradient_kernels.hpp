@@ -80,10 +80,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_source(Dims d, Geometry g, const
         int i = 0, j = 0, l = 0, m = 0;
         bool dip = false;
         if (in) {
-            l = (int)(k % d.nl);
-            const long long ij = k / d.nl;
-            j = (int)(ij % d.nj);
-            i = (int)(ij / d.nj);
+            // lattice dimensions are powers of two: (i, j, l) by shifts
+            l = (int)(k & (d.nl - 1));
+            const long long ij = k >> __builtin_ctz(d.nl);
+            j = (int)(ij & (d.nj - 1));
+            i = (int)(ij >> __builtin_ctz(d.nj));
             m = occ[k];
             dip = mv[m] != 0;
         }

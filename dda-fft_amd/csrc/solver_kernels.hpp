@@ -61,9 +61,10 @@ __global__ __launch_bounds__(kBlock) void k_first_approx(Dims d, Geometry g, con
     const float2 E0 = einc[rhs * 3 + 0], E1 = einc[rhs * 3 + 1], E2 = einc[rhs * 3 + 2];
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < d.n;
          k += (long long)gridDim.x * blockDim.x) {
-        const int l = (int)(k % d.nl);
-        const long long ij = k / d.nl;
-        const int j = (int)(ij % d.nj), i = (int)(ij / d.nj);
+        // lattice dimensions are powers of two: cell index -> (i, j, l) by shifts
+        const int l = (int)(k & (d.nl - 1));
+        const long long ij = k >> __builtin_ctz(d.nl);
+        const int j = (int)(ij & (d.nj - 1)), i = (int)(ij >> __builtin_ctz(d.nj));
         const int m = occ[k];
         float2 o0 = make_float2(0.f, 0.f), o1 = o0, o2 = o0;
         if (mv[m] != 0) {
@@ -203,9 +204,10 @@ __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const 
         const int m = occ[k];
         const float2 a = alpha[(long long)rhs * nmat + m];
         if (a.x == 0.f && a.y == 0.f) continue;
-        const int l = (int)(k % d.nl);
-        const long long ij = k / d.nl;
-        const int j = (int)(ij % d.nj), i = (int)(ij / d.nj);
+        // lattice dimensions are powers of two: cell index -> (i, j, l) by shifts
+        const int l = (int)(k & (d.nl - 1));
+        const long long ij = k >> __builtin_ctz(d.nl);
+        const int j = (int)(ij & (d.nj - 1)), i = (int)(ij >> __builtin_ctz(d.nj));
         const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
         const float r3 = -(kx * (rx - g.ex) + ky * (ry - g.ey) + kz * (rz - g.ez));
         const int ax = ip.axis[rhs];
