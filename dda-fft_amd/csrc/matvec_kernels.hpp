@@ -36,14 +36,18 @@ namespace ddafft {
 
 // Tile widths (lines per block).  The z and y passes keep the LDS footprint
 // <= 48 KB.  The fused X pass holds 3 components of RB right-hand sides at 2ni
-// points; its tile stays near 16 KB (two (J, L) columns at ni = 64) so that
-// several blocks per CU hide the latency of its two global phases (measured
-// on MI355X: 73 us at 8 columns, 57 at 4, 52 at 2, 69 at 1, 64^3, 2 RHS).
-// Longer lines keep two columns, so that the 384 butterflies of a stage fill
-// the block's threads evenly (256^3, 2 RHS: 7.16 ms at one column, 6.66 at two).
+// points; its tile stays small so that several blocks per CU hide the latency
+// of its two global phases.  Two right-hand sides at n_i = 64 and 128 take
+// four (J, L) columns (measured on MI355X with compile-time tile widths,
+// 2 RHS: 64^3 50.3 us at 4 columns, 56.5 at 2, 58.5 at 8; 128^3 496 us at 4,
+// 530 at 2); 32^3 keeps 4 (13.2 us, 13.8 at 8); 256-point lines keep two,
+// so that the 384 butterflies of a stage fill the block's threads evenly
+// (256^3, 2 RHS: 7.16 ms at one column, 6.66 at two).
 template <int N> struct TileZF { static constexpr int T = clamp16(pow2floor(6144 / (2 * (N + kLpad)))); };
 template <int N, int RB> struct TileX {
-    static constexpr int T = N >= 128 ? 2 : clamp16(pow2floor(2048 / (6 * RB * (N + kLpad))));
+    static constexpr int T = (RB == 2 && (N == 64 || N == 128)) ? 4
+                             : N >= 128                           ? 2
+                                                                  : clamp16(pow2floor(2048 / (6 * RB * (N + kLpad))));
 };
 // Line strides (float2) of the X pass's forward (n_i points) and inverse
 // (2 n_i points) lines.  With the lane grouping of each stage (LaneGroup),
@@ -51,17 +55,18 @@ template <int N, int RB> struct TileX {
 template <int NI, int RB> struct XStride {
     static constexpr int S1 = NI + kLpad, S2 = 2 * NI + kLpad;
 };
-// n_i = 64, two right-hand sides (24 forward lines, 12 inverse lines per
-// block): strides 66 / 131 with the stage groupings below halve the modelled
-// bank conflicts (a 64-bank, 32-lanes-per-cycle model: 776 -> 420 extra
-// cycles per block) and took the pass from 52.2 to 51.0 us on MI355X.
+// n_i = 64, two right-hand sides: strides 66 / 131.  Chosen for a two-column
+// tile (24 forward, 12 inverse lines), where with the stage groupings of a
+// 64-bank conflict model they took the pass from 52.2 to 51.0 us; the
+// four-column tile (48 / 24 lines, one butterfly per line per lane group)
+// was measured with the same strides.
 template <> struct XStride<64, 2> { static constexpr int S1 = 66, S2 = 131; };
-template <> struct LaneGroup<64, 24, 66, 0> { static constexpr int G = 2; };
-template <> struct LaneGroup<64, 24, 66, 1> { static constexpr int G = 2; };
+// 12 lines of 128 points at stride 131 (the Z-inverse tile below): a half-wave
+// takes 32 consecutive butterflies of one line in the last stage.
 template <> struct LaneGroup<128, 12, 131, 2> { static constexpr int G = 32; };
-// The Z-inverse pass at 2 n_l = 128 also transforms 12 lines per block (three
-// components of four rows): the same stride and grouping take it from 23.1
-// to 22.7 us per launch (64^3, 2 RHS).
+// The Z-inverse pass at 2 n_l = 128 transforms 12 lines per block (three
+// components of four rows): that stride and grouping take it from 23.1 to
+// 22.7 us per launch (64^3, 2 RHS).
 template <int NL2, int T> struct ZStride { static constexpr int S = NL2 + kLpad; };
 template <> struct ZStride<128, 4> { static constexpr int S = 131; };
 template <int N> struct TileYI { static constexpr int T = clamp16(pow2floor(6144 / (N + kLpad))); };
