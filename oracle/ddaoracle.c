@@ -357,32 +357,31 @@ void orc_build_ahat(int ni, int nj, int nl, real d0, real *out) {
     cmat *A = (cmat *)out;
     cplx *wi = (cplx *)malloc(sizeof(cplx) * ni), *wj = (cplx *)malloc(sizeof(cplx) * nj),
          *wl = (cplx *)malloc(sizeof(cplx) * nl);
-    cmat *ba = (cmat *)malloc(sizeof(cmat) * Nm), *bb = (cmat *)malloc(sizeof(cmat) * Nm);
     orc_amatrix(ni, nj, nl, d0, out);
     orc_fourier_phase(ni, (real *)wi); orc_fourier_phase(nj, (real *)wj); orc_fourier_phase(nl, (real *)wl);
-    for (int i = 0; i < Ni; i++)
-        for (int j = 0; j < Nj; j++) {
-            for (int l = 0; l < Nl; l++) ba[l] = A[idx3(i, j, l, Nj, Nl)];
-            legacy_fft(Nl, wl, (cplx *)ba, (cplx *)bb, 6);
-            for (int l = 0; l < Nl; l++) A[idx3(i, j, l, Nj, Nl)] = bb[l];
-        }
-    for (int i = 0; i < Ni; i++)
-        for (int l = 0; l < Nl; l++) {
-            for (int j = 0; j < Nj; j++) ba[j] = A[idx3(i, j, l, Nj, Nl)];
-            legacy_fft(Nj, wj, (cplx *)ba, (cplx *)bb, 6);
-            for (int j = 0; j < Nj; j++) A[idx3(i, j, l, Nj, Nl)] = bb[j];
-        }
-    for (int j = 0; j < Nj; j++)
-        for (int l = 0; l < Nl; l++) {
-            for (int i = 0; i < Ni; i++) ba[i] = A[idx3(i, j, l, Nj, Nl)];
-            legacy_fft(Ni, wi, (cplx *)ba, (cplx *)bb, 6);
-            for (int i = 0; i < Ni; i++) A[idx3(i, j, l, Nj, Nl)] = bb[i];
-        }
+    /* independent lines per axis, spread over threads (per-thread scratch) */
+#define ORC_ALINES(n0, n1, len, w, at)                                              \
+    _Pragma("omp parallel")                                                         \
+    {                                                                               \
+        cmat *ba = (cmat *)malloc(sizeof(cmat) * Nm), *bb = (cmat *)malloc(sizeof(cmat) * Nm); \
+        _Pragma("omp for collapse(2) schedule(static)")                             \
+        for (int u = 0; u < (n0); u++)                                              \
+            for (int v = 0; v < (n1); v++) {                                        \
+                for (int t = 0; t < (len); t++) ba[t] = A[at];                      \
+                legacy_fft((len), (w), (cplx *)ba, (cplx *)bb, 6);                  \
+                for (int t = 0; t < (len); t++) A[at] = bb[t];                      \
+            }                                                                       \
+        free(ba); free(bb);                                                         \
+    }
+    ORC_ALINES(Ni, Nj, Nl, wl, idx3(u, v, t, Nj, Nl))
+    ORC_ALINES(Ni, Nl, Nj, wj, idx3(u, t, v, Nj, Nl))
+    ORC_ALINES(Nj, Nl, Ni, wi, idx3(t, u, v, Nj, Nl))
+#undef ORC_ALINES
     real norma = (real)(1. / (Ni * Nj * Nl));
     size_t P = (size_t)Ni * Nj * Nl;
     cplx *c = (cplx *)out;
     for (size_t k = 0; k < 6 * P; k++) c[k] = rmul(norma, c[k]);
-    free(wi); free(wj); free(wl); free(ba); free(bb);
+    free(wi); free(wj); free(wl);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -393,7 +392,7 @@ typedef struct {
     int ni, nj, nl;
     orc_plan pi, pj, pl, Pi, Pj, Pl;  /* n-point (forward halves) and 2n-point (inverse) */
     cplx *wi, *wj, *wl;               /* FourierPhase(n): W_2n^k, k < n */
-    cvec *b0, *b1;
+    int m;                            /* longest line (points) */
 } orc_fft3;
 
 static int fft3_init(orc_fft3 *f, int ni, int nj, int nl) {
@@ -405,59 +404,66 @@ static int fft3_init(orc_fft3 *f, int ni, int nj, int nl) {
     f->wi = (cplx *)malloc(sizeof(cplx) * ni); orc_fourier_phase(ni, (real *)f->wi);
     f->wj = (cplx *)malloc(sizeof(cplx) * nj); orc_fourier_phase(nj, (real *)f->wj);
     f->wl = (cplx *)malloc(sizeof(cplx) * nl); orc_fourier_phase(nl, (real *)f->wl);
-    int m = 2 * (ni > nj ? (ni > nl ? ni : nl) : (nj > nl ? nj : nl));
-    f->b0 = (cvec *)malloc(sizeof(cvec) * m);
-    f->b1 = (cvec *)malloc(sizeof(cvec) * m);
+    f->m = 2 * (ni > nj ? (ni > nl ? ni : nl) : (nj > nl ? nj : nl));
     return 0;
 }
 
 static void fft3_free(orc_fft3 *f) {
     plan_free(&f->pi); plan_free(&f->pj); plan_free(&f->pl);
     plan_free(&f->Pi); plan_free(&f->Pj); plan_free(&f->Pl);
-    free(f->wi); free(f->wj); free(f->wl); free(f->b0); free(f->b1);
+    free(f->wi); free(f->wj); free(f->wl);
 }
 
 static inline cvec cvtw(cplx w, cvec v) { cvec o; o.x = cmul(w, v.x); o.y = cmul(w, v.y); o.z = cmul(w, v.z); return o; }
 
 /* One pruned forward line: n inputs at stride st -> 2n outputs at stride st,
-   X[2k] = DFT_n(x)[k], X[2k+1] = DFT_n(W_2n^j x_j)[k]  (fft_func.cpp:447-462). */
-static void fwd_line(orc_fft3 *f, const orc_plan *p, const cplx *w, cvec *d, size_t st) {
+   X[2k] = DFT_n(x)[k], X[2k+1] = DFT_n(W_2n^j x_j)[k]  (fft_func.cpp:447-462).
+   b0, b1: scratch of n points. */
+static void fwd_line(const orc_plan *p, const cplx *w, cvec *d, size_t st, cvec *b0, cvec *b1) {
     int n = p->n;
-    for (int k = 0; k < n; k++) f->b0[k] = d[k * st];
-    line_fwd(p, f->b1, f->b0);
-    for (int k = 0; k < n; k++) f->b0[k] = cvtw(w[k], d[k * st]);
-    for (int k = 0; k < n; k++) d[2 * k * st] = f->b1[k];
-    line_fwd(p, f->b1, f->b0);
-    for (int k = 0; k < n; k++) d[(2 * k + 1) * st] = f->b1[k];
+    for (int k = 0; k < n; k++) b0[k] = d[k * st];
+    line_fwd(p, b1, b0);
+    for (int k = 0; k < n; k++) b0[k] = cvtw(w[k], d[k * st]);
+    for (int k = 0; k < n; k++) d[2 * k * st] = b1[k];
+    line_fwd(p, b1, b0);
+    for (int k = 0; k < n; k++) d[(2 * k + 1) * st] = b1[k];
 }
 
-static void inv_line(orc_fft3 *f, const orc_plan *p, cvec *d, size_t st) {
+static void inv_line(const orc_plan *p, cvec *d, size_t st, cvec *b0, cvec *b1) {
     int n = p->n;
-    for (int k = 0; k < n; k++) f->b0[k] = d[k * st];
-    line_inv(p, f->b1, f->b0);
-    for (int k = 0; k < n; k++) d[k * st] = f->b1[k];
+    for (int k = 0; k < n; k++) b0[k] = d[k * st];
+    line_inv(p, b1, b0);
+    for (int k = 0; k < n; k++) d[k * st] = b1[k];
 }
+
+/* The lines of one axis are independent: with OpenMP they are spread over
+   threads, each with its own scratch; every line's float operations are the
+   same as in a sequential run, so the results do not depend on the thread
+   count. */
+#define ORC_LINES(n0, n1, body)                                                   \
+    _Pragma("omp parallel")                                                       \
+    {                                                                             \
+        cvec *b0 = (cvec *)malloc(sizeof(cvec) * f->m), *b1 = (cvec *)malloc(sizeof(cvec) * f->m); \
+        _Pragma("omp for collapse(2) schedule(static)")                           \
+        for (int u = 0; u < (n0); u++)                                            \
+            for (int v = 0; v < (n1); v++) { body; }                              \
+        free(b0); free(b1);                                                       \
+    }
 
 /* fft_func.cpp:418-550 on the padded buffer w [2ni][2nj][2nl] */
 static void fft3_forward(orc_fft3 *f, cvec *w) {
     int ni = f->ni, nj = f->nj, nl = f->nl, Nj = 2 * nj, Nl = 2 * nl;
-    for (int i = 0; i < ni; i++)
-        for (int j = 0; j < nj; j++) fwd_line(f, &f->pl, f->wl, &w[idx3(i, j, 0, Nj, Nl)], 1);
-    for (int i = 0; i < ni; i++)
-        for (int l = 0; l < Nl; l++) fwd_line(f, &f->pj, f->wj, &w[idx3(i, 0, l, Nj, Nl)], (size_t)Nl);
-    for (int j = 0; j < Nj; j++)
-        for (int l = 0; l < Nl; l++) fwd_line(f, &f->pi, f->wi, &w[idx3(0, j, l, Nj, Nl)], (size_t)Nj * Nl);
+    ORC_LINES(ni, nj, fwd_line(&f->pl, f->wl, &w[idx3(u, v, 0, Nj, Nl)], 1, b0, b1))
+    ORC_LINES(ni, Nl, fwd_line(&f->pj, f->wj, &w[idx3(u, 0, v, Nj, Nl)], (size_t)Nl, b0, b1))
+    ORC_LINES(Nj, Nl, fwd_line(&f->pi, f->wi, &w[idx3(0, u, v, Nj, Nl)], (size_t)Nj * Nl, b0, b1))
 }
 
 /* fft_func.cpp:556-653 */
 static void fft3_inverse(orc_fft3 *f, cvec *w) {
     int ni = f->ni, nj = f->nj, Nj = 2 * nj, Nl = 2 * f->nl;
-    for (int j = 0; j < Nj; j++)
-        for (int l = 0; l < Nl; l++) inv_line(f, &f->Pi, &w[idx3(0, j, l, Nj, Nl)], (size_t)Nj * Nl);
-    for (int i = 0; i < ni; i++)
-        for (int l = 0; l < Nl; l++) inv_line(f, &f->Pj, &w[idx3(i, 0, l, Nj, Nl)], (size_t)Nl);
-    for (int i = 0; i < ni; i++)
-        for (int j = 0; j < nj; j++) inv_line(f, &f->Pl, &w[idx3(i, j, 0, Nj, Nl)], 1);
+    ORC_LINES(Nj, Nl, inv_line(&f->Pi, &w[idx3(0, u, v, Nj, Nl)], (size_t)Nj * Nl, b0, b1))
+    ORC_LINES(ni, Nl, inv_line(&f->Pj, &w[idx3(u, 0, v, Nj, Nl)], (size_t)Nl, b0, b1))
+    ORC_LINES(ni, nj, inv_line(&f->Pl, &w[idx3(u, v, 0, Nj, Nl)], 1, b0, b1))
 }
 
 /* dda_func.cpp:702-718 CopyingBack */
@@ -485,6 +491,7 @@ static inline int cvec_nonzero(const cvec *v) {
 }
 
 static void pointwise(int Ni, int Nj, int Nl, const cmat *A, cvec *w, int conj) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < Ni; i++)
         for (int j = 0; j < Nj; j++)
             for (int l = 0; l < Nl; l++) {
@@ -946,6 +953,7 @@ static int dda_core(int solver, real tau, int ni, int nj, int nl, const real *al
     if (Error > Error_min) { memcpy(P, Pmin, sizeof(cvec) * n); restored = 1; }   /* :1537-1548 */
 
     orc_cabs_cext(d0, kinc, einc, ni, nj, nl, alpha, occ, (const real *)P, cabs, cext);
+#pragma omp parallel for schedule(dynamic)
     for (int d = 0; d < ndirs; d++) orc_far_field(ni, nj, nl, mv, occ, d0, (const real *)P, dirs + 3 * d, esc + 6 * d);
     if (info) { info[0] = last; info[1] = step_min; info[2] = restored; info[3] = nmv; }
     if (p_out) memcpy(p_out, P, sizeof(cvec) * n);

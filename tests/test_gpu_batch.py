@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import ddafft
-from conftest import directions
+from conftest import bits, directions
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +37,7 @@ def mc_instance(orc, dims, kd, m, norient, seed, theta=np.arange(0, 181, 30.0)):
                 einc=np.array(einc), dirs=np.array(dirs), kd=np.float32(kd))
 
 
-def check_instance(orc, inst, res, solver, max_step, err_max, label):
+def check_instance(orc, inst, res, solver, max_step, err_max, label, P=None):
     """Each right-hand side against the oracle's single solve.  The oracle runs
     with correctly rounded phases (the device's): glibc's cosf/sinf, which the
     reference calls, differ from them by one ulp for about 1% of arguments, and
@@ -49,11 +49,14 @@ def check_instance(orc, inst, res, solver, max_step, err_max, label):
     try:
         for r, got in enumerate(res):
             ref = orc.dda_fft(solver, inst["occ"], inst["mv"], inst["alpha"], inst["kd"], inst["kinc"][r],
-                              inst["einc"][r], A, inst["dirs"][r], max_step=max_step, err_max=err_max)
+                              inst["einc"][r], A, inst["dirs"][r], max_step=max_step, err_max=err_max,
+                              return_p=True)
             erel = np.max(np.abs(got["esc"] - ref["esc"])) / np.max(np.abs(ref["esc"]))
             print(label, "rhs", r, "steps", got["steps"], ref["steps"], "esc", erel, flush=True)
             assert got["steps"] == ref["steps"] and got["step_min"] == ref["step_min"]
             assert got["restored"] == ref["restored"]
+            if P is not None:   # same phases, same float operations: the same dipoles
+                assert np.array_equal(bits(P[r]), bits(ref["p"])), (label, r)
             assert erel < 1e-5
             assert abs(got["cext"] / ref["cext"] - 1) < 1e-5
             assert abs(got["cabs"] / ref["cabs"] - 1) < 1e-4
@@ -70,7 +73,8 @@ def test_solve_ex_per_rhs_incidence(orc32):
         ctx.set_target(inst["occ"], inst["mv"])
         res = ctx.solve_ex("bicgstab", inst["alpha"], inst["kinc"], inst["einc"], dirs=inst["dirs"], max_step=30,
                            err_max=1e-4)
-    check_instance(orc32, inst, res, 2, 30, 1e-4, "solve_ex")
+        P = [ctx.dipoles(r) for r in range(len(res))]
+    check_instance(orc32, inst, res, 2, 30, 1e-4, "solve_ex", P)
 
 
 @pytest.mark.parametrize("width", [1, 2, 4])
@@ -84,11 +88,12 @@ def test_solve_batch_uneven_lattices(orc32, width):
             ctxs.append(ctx)
         jobs = [dict(alpha=i["alpha"], kinc=i["kinc"], einc=i["einc"], dirs=i["dirs"]) for i in insts]
         out = ddafft.solve_batch(ctxs, jobs, "bicgstab", max_step=30, err_max=1e-4, width=width)
+        P = [[c.dipoles(r) for r in range(len(res))] for c, res in zip(ctxs, out)]
     finally:
         for c in ctxs:
             c.close()
     for k, (inst, res) in enumerate(zip(insts, out)):
-        check_instance(orc32, inst, res, 2, 30, 1e-4, "batch w%d inst %d" % (width, k))
+        check_instance(orc32, inst, res, 2, 30, 1e-4, "batch w%d inst %d" % (width, k), P[k])
 
 
 def test_rotated_incidence_against_reference_phases(orc32):
