@@ -52,6 +52,7 @@ struct Options {
     bool has_seed = false;
     unsigned long long seed = 0;
     int group = 8;   // samples per batched solve (this program's own option)
+    bool reference_sums = false;   // observables summed as the reference sums them (own option)
 };
 
 struct Name {
@@ -147,7 +148,9 @@ void usage(const char* prog) {
                 "  --tau <factor>                   weight factor, required by solver 3\n"
                 "  -j, --threads <N>                accepted for compatibility (samples follow -j 1)\n"
                 "  --seed <u64>                     random seed for reproducible runs\n"
-                "  --group <N>                      samples solved together on the GPU (default 8)\n",
+                "  --group <N>                      samples solved together on the GPU, 1..32 (default 8)\n"
+                "  --reference-sums                 sum Cext, Cabs and the far field sequentially in float, in the\n"
+                "                                   reference's order (default: FP64 sums of the same terms)\n",
                 prog);
 }
 
@@ -219,6 +222,8 @@ int parse(int argc, char** argv, Options* o) {
             errors.push_back("--interactive: prompts are not provided; give every value as an option");
         } else if (key == "--restart") {
             o->restart = true;
+        } else if (key == "--reference-sums") {
+            o->reference_sums = true;
         } else if (key == "-o" || key == "--output") {
             if (need(k, key, &v, inl)) o->output_file = v;
         } else if (key == "-D" || key == "--state") {
@@ -409,7 +414,7 @@ int parse(int argc, char** argv, Options* o) {
     if (!(o->tolerance > 0.)) errors.push_back("--tolerance must be positive");
     if (o->max_steps < 1) errors.push_back("--max-steps must be at least 1");
     if (o->threads < 0) errors.push_back("--threads must be 0 (auto) or positive");
-    if (o->group < 1) errors.push_back("--group must be at least 1");
+    if (o->group < 1 || o->group > 32) errors.push_back("--group must be between 1 and 32");
     if (o->output_file == o->state_file) errors.push_back("--output and --state must name different files");
     if (o->output_file.size() > 1023) errors.push_back("--output is too long (1024 bytes at most)");
     if (o->state_file.size() > 1023) errors.push_back("--state is too long (1024 bytes at most)");
@@ -599,7 +604,8 @@ int main(int argc, char** argv) {
         const int nctx = fixed_target ? 1 : group;
         std::vector<ddafft_ctx*> ctxs(nctx, nullptr);
         for (int q = 0; q < nctx; ++q)
-            if (ddafft_create(ni, nj, nl, kd, fixed_target ? 2 * group : 2, &ctxs[q]) != DDAFFT_OK) {
+            if (ddafft_create(ni, nj, nl, kd, fixed_target ? 2 * group : 2, &ctxs[q]) != DDAFFT_OK ||
+                (o.reference_sums && ddafft_set_sum_mode(ctxs[q], 1) != DDAFFT_OK)) {
                 std::fprintf(stderr, "%s\n", ddafft_last_error());
                 return 1;
             }

@@ -39,6 +39,12 @@ int fail(int code, const std::string& msg) {
             return fail(DDAFFT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
     } while (0)
 
+#define TRY(x)                  \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_) return rc_;    \
+    } while (0)
+
 const double kPi = 3.1415926535897932384626433832795;
 
 bool pow2_ok(int n) { return n >= 4 && n <= 256 && (n & (n - 1)) == 0; }
@@ -175,7 +181,21 @@ struct ddafft_ctx {
     int esc_cap = 0;
     float2* fftab = nullptr;           // far-field per-axis phases [ndirs][ni + nj + nl]
     size_t fftab_cap = 0;
+    int sum_mode = 0;                  // observables: 0 FP64 sums, 1 the reference's sequential float sums
+    int x_blocks = 1 << 30;            // X pass: blocks per launch up to which right-hand-side groups are spread
+    int x_rb = 0;                      // X pass: right-hand sides per tile (0: 4 when they come in fours, else 2)
+    float* seqbuf = nullptr;           // terms of the sequential float sums (sum_mode 1)
+    size_t seqbuf_cap = 0;
     SolveState* state_ring = nullptr;  // pinned [4][nrhs_max]: solver states after each queued iteration
+    SolveState* ring_dev = nullptr;    // its device address: the iteration's last scalar step writes it
+    // per-solve inputs in one device block with a page-locked mirror of the
+    // same layout (st | alpha | einc | kinc | inc_axis | inc_tab): staged on
+    // the host, then one copy of the range written
+    char* small = nullptr;
+    char* small_h = nullptr;
+    size_t small_bytes = 0, dirty_lo = 0, dirty_hi = 0;
+    bool pending_first_approx = false; // job staged: the first approximation runs after the copy
+    char* dn_dev = nullptr;            // device address of the pinned result buffer dn
     hipEvent_t fin_ev = nullptr;       // marks the results of a batched solve as copied back
     // pinned staging of the per-solve transfers, so that they are truly
     // asynchronous: uploads are packed at up_off (reset when a solve starts,
@@ -196,6 +216,9 @@ struct ddafft_ctx {
 };
 
 namespace {
+
+int flush_small(ddafft_ctx* c);
+int inc_tab_len(const ddafft_ctx* c);
 
 int ensure_partials(ddafft_ctx* c, size_t doubles) {
     if (doubles <= c->partials_cap) return DDAFFT_OK;
@@ -222,25 +245,20 @@ int tile_zf(int n) {
     }
     return 1;
 }
-int tile_x(int n, int rb) {
-    switch (n * 4 + rb) {
-        case 4 * 4 + 1: return TileX<4, 1>::T;
-        case 8 * 4 + 1: return TileX<8, 1>::T;
-        case 16 * 4 + 1: return TileX<16, 1>::T;
-        case 32 * 4 + 1: return TileX<32, 1>::T;
-        case 64 * 4 + 1: return TileX<64, 1>::T;
-        case 128 * 4 + 1: return TileX<128, 1>::T;
-        case 256 * 4 + 1: return TileX<256, 1>::T;
-        case 4 * 4 + 2: return TileX<4, 2>::T;
-        case 8 * 4 + 2: return TileX<8, 2>::T;
-        case 16 * 4 + 2: return TileX<16, 2>::T;
-        case 32 * 4 + 2: return TileX<32, 2>::T;
-        case 64 * 4 + 2: return TileX<64, 2>::T;
-        case 128 * 4 + 2: return TileX<128, 2>::T;
-        case 256 * 4 + 2: return TileX<256, 2>::T;
+template <int RB>
+int tile_x_rb(int n) {
+    switch (n) {
+        case 4: return TileX<4, RB>::T;
+        case 8: return TileX<8, RB>::T;
+        case 16: return TileX<16, RB>::T;
+        case 32: return TileX<32, RB>::T;
+        case 64: return TileX<64, RB>::T;
+        case 128: return TileX<128, RB>::T;
+        case 256: return TileX<256, RB>::T;
     }
     return 1;
 }
+int tile_x(int n, int rb) { return rb == 4 ? tile_x_rb<4>(n) : rb == 2 ? tile_x_rb<2>(n) : tile_x_rb<1>(n); }
 int tile_yi(int n2) {
     switch (n2) {
         case 8: return TileYI<8>::T;
@@ -343,6 +361,7 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
         if (!c->occ || !c->mv || !c->b1 || !c->b2 || !c->b3 || !c->ahat || !c->tables)
             return fail(DDAFFT_ESTATE, "internal: context buffers missing");
     }
+    TRY(flush_small(c));
     Dims d = c->d;
     d.nrhs = nrhs;
     const int C = 3 * nrhs;
@@ -362,17 +381,29 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
     }
     if (ev) HIPCHECK(hipEventRecord(ev[2], s));
     {
-        // right-hand sides go through the X pass in pairs that share each Â read
-        const int rb = nrhs >= 2 ? 2 : 1;
+        // right-hand sides go through the X pass in groups of rb that share
+        // each Â value (4 when they come in fours on lines of at most 32
+        // points -- measured: 32^3, 16 RHS 41 us against 44.5 in pairs; 64^3
+        // 425 against 315 -- else 2, or the tuned choice); a block takes
+        // `gpb` consecutive groups and reads its
+        // tile's Â once for all of them, as many groups as keep about
+        // x_blocks blocks in the launch
+        int rb = c->x_rb > 0 ? c->x_rb : ((nrhs % 4 == 0 && d.ni <= 32) ? 4 : 2);
+        while (rb > 1 && rb > nrhs) rb >>= 1;
         const int tj = std::min(tile_x(d.ni, rb), 2 * d.nj);
-        dim3 grid(2 * d.nl * (2 * d.nj / tj), (nrhs + rb - 1) / rb);
-        if (rb == 2) {
-            DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN, 2>), grid, kBlock, 0, s, d, st, iter, c->b2, c->b3, c->ahat, conj,
-                                                      c->wpre_i, c->tw_i, c->tw_2i));
+        const int ntile = 2 * d.nl * (2 * d.nj / tj), ngroups = (nrhs + rb - 1) / rb;
+        const int ysets = std::max(1, std::min(ngroups, (c->x_blocks + ntile - 1) / ntile));
+        const int gpb = (ngroups + ysets - 1) / ysets;
+        dim3 grid(ntile, (ngroups + gpb - 1) / gpb);
+#define DDA_KX(RB, RUN)                                                                                   \
+    DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN, RB, RUN>), grid, kBlock, 0, s, d, st, iter, c->b2, c->b3,   \
+                                              c->ahat, conj, c->wpre_i, c->tw_i, c->tw_2i, gpb))
+        if (gpb > 1) {
+            if (rb == 4) DDA_KX(4, true) else if (rb == 2) DDA_KX(2, true) else DDA_KX(1, true)
         } else {
-            DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN, 1>), grid, kBlock, 0, s, d, st, iter, c->b2, c->b3, c->ahat, conj,
-                                                      c->wpre_i, c->tw_i, c->tw_2i));
+            if (rb == 4) DDA_KX(4, false) else if (rb == 2) DDA_KX(2, false) else DDA_KX(1, false)
         }
+#undef DDA_KX
     }
     if (ev) HIPCHECK(hipEventRecord(ev[3], s));
     {
@@ -471,6 +502,19 @@ int prof_collect(ddafft_ctx* c, int last_iter = 1 << 30) {
 // staging buffer while it has room (a pinned source keeps the copy
 // asynchronous), otherwise from the caller's memory.
 int upload(ddafft_ctx* c, void* dst, const void* src, size_t bytes) {
+    char* d = static_cast<char*>(dst);
+    if (c->small && d >= c->small && d + bytes <= c->small + c->small_bytes) {
+        const size_t o = (size_t)(d - c->small);
+        std::memcpy(c->small_h + o, src, bytes);
+        if (c->dirty_hi <= c->dirty_lo) {
+            c->dirty_lo = o;
+            c->dirty_hi = o + bytes;
+        } else {
+            c->dirty_lo = std::min(c->dirty_lo, o);
+            c->dirty_hi = std::max(c->dirty_hi, o + bytes);
+        }
+        return DDAFFT_OK;
+    }
     const size_t need = (bytes + 255) & ~(size_t)255;
     if (c->up && c->up_off + need <= c->up_cap) {
         char* q = c->up + c->up_off;
@@ -485,9 +529,22 @@ int upload(ddafft_ctx* c, void* dst, const void* src, size_t bytes) {
     return DDAFFT_OK;
 }
 
+// Copies what was staged into the small-input mirror since the last copy
+// (one copy of the range written).  Called before every launch that may read
+// the small inputs; a range is never restaged while its copy is pending
+// (each solve stages once, after the stream has drained).
+int flush_small(ddafft_ctx* c) {
+    if (c->dirty_hi <= c->dirty_lo) return DDAFFT_OK;
+    HIPCHECK(hipMemcpyAsync(c->small + c->dirty_lo, c->small_h + c->dirty_lo, c->dirty_hi - c->dirty_lo,
+                            hipMemcpyHostToDevice, c->stream));
+    c->dirty_lo = c->dirty_hi = 0;
+    return DDAFFT_OK;
+}
+
 // (Re)allocates pinned host memory of at least `bytes`; waits for the stream
-// first, since pending copies may use the old buffer.
-int pinned_reserve(ddafft_ctx* c, char** p, size_t* cap, size_t bytes) {
+// first, since pending copies may use the old buffer.  dev (optional)
+// receives the device address of the buffer.
+int pinned_reserve(ddafft_ctx* c, char** p, size_t* cap, size_t bytes, char** dev = nullptr) {
     if (*p && *cap >= bytes) return DDAFFT_OK;
     HIPCHECK(hipStreamSynchronize(c->stream));
     if (*p) hipHostFree(*p);
@@ -499,6 +556,43 @@ int pinned_reserve(ddafft_ctx* c, char** p, size_t* cap, size_t bytes) {
         return fail(DDAFFT_ENOMEM, "hipHostMalloc of pinned staging failed");
     }
     *cap = n;
+    if (dev && hipHostGetDevicePointer((void**)dev, *p, 0) != hipSuccess) {
+        hipHostFree(*p);
+        *p = nullptr;
+        *cap = 0;
+        return fail(DDAFFT_EHIP, "hipHostGetDevicePointer of pinned staging failed");
+    }
+    return DDAFFT_OK;
+}
+
+// Carves the small per-solve inputs out of one device block and allocates
+// its page-locked mirror.
+int small_layout(ddafft_ctx* c) {
+    const int R = c->nrhs_max;
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        const size_t o = off;
+        off += (b + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_st = take((size_t)R * sizeof(SolveState));
+    const size_t o_al = take((size_t)R * 128 * sizeof(float2));   // set_target allows 128 materials
+    const size_t o_ei = take((size_t)R * 3 * sizeof(float2));
+    const size_t o_ki = take((size_t)R * 3 * sizeof(float));
+    const size_t o_ax = take((size_t)R * sizeof(int));
+    const size_t o_tb = take((size_t)R * 2 * inc_tab_len(c) * sizeof(float2));
+    TRY(dev_alloc(&c->small, off));
+    if (hipHostMalloc((void**)&c->small_h, off, hipHostMallocDefault) != hipSuccess) {
+        c->small_h = nullptr;
+        return fail(DDAFFT_ENOMEM, "hipHostMalloc of the input mirror failed");
+    }
+    c->small_bytes = off;
+    c->st = reinterpret_cast<SolveState*>(c->small + o_st);
+    c->alpha = reinterpret_cast<float2*>(c->small + o_al);
+    c->einc = reinterpret_cast<float2*>(c->small + o_ei);
+    c->kinc = reinterpret_cast<float*>(c->small + o_ki);
+    c->inc_axis = reinterpret_cast<int*>(c->small + o_ax);
+    c->inc_tab = reinterpret_cast<float2*>(c->small + o_tb);
     return DDAFFT_OK;
 }
 
@@ -509,12 +603,6 @@ int upload_begin(ddafft_ctx* c, size_t bytes) {
     c->up_off = 0;
     return pinned_reserve(c, &c->up, &c->up_cap, bytes);
 }
-
-#define TRY(x)                  \
-    do {                        \
-        int rc_ = (x);          \
-        if (rc_) return rc_;    \
-    } while (0)
 
 int field_in(ddafft_ctx* c, const float* host_aos, float2* dev_soa, float2* staging) {
     const long long n = c->d.n;
@@ -678,12 +766,12 @@ int ddafft_destroy(ddafft_ctx* c) {
     if (!c) return DDAFFT_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->ahat, c->tables, c->occ, c->dipf, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
-                    c->S, c->G, c->H, c->W, c->X, c->Y, c->alpha, c->einc, c->st, c->hist, c->partials,
-                    c->dirs, c->esc, c->fftab, c->Pf, c->phase, c->gaux, c->kinc, c->inc_axis, c->inc_tab,
-                    c->inc_cell};
+                    c->S, c->G, c->H, c->W, c->X, c->Y, c->small, c->hist, c->partials,
+                    c->dirs, c->esc, c->fftab, c->Pf, c->phase, c->gaux, c->inc_cell, c->seqbuf};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->state_ring) hipHostFree(c->state_ring);
+    if (c->small_h) hipHostFree(c->small_h);
     if (c->up) hipHostFree(c->up);
     if (c->dn) hipHostFree(c->dn);
     for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
@@ -735,16 +823,14 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
     float2** fields[] = {&c->P, &c->Pmin, &c->R, &c->S, &c->G, &c->H, &c->W, &c->X, &c->Y};
     for (float2** f : fields)
         if (!rc) rc = dev_alloc(f, (size_t)(n * C));
-    if (!rc) rc = dev_alloc(&c->einc, (size_t)3 * nrhs_max);
-    if (!rc) rc = dev_alloc(&c->kinc, (size_t)3 * nrhs_max);
-    if (!rc) rc = dev_alloc(&c->inc_axis, (size_t)nrhs_max);
-    if (!rc) rc = dev_alloc(&c->inc_tab, (size_t)nrhs_max * 2 * inc_tab_len(c));
-    if (!rc) rc = dev_alloc(&c->st, (size_t)nrhs_max);
+    if (!rc) rc = small_layout(c);
     if (!rc) rc = dev_alloc(&c->occ, (size_t)n);
     if (!rc) rc = dev_alloc(&c->dipf, (size_t)n);
     if (!rc && hipHostMalloc((void**)&c->state_ring, (size_t)4 * nrhs_max * sizeof(SolveState),
                              hipHostMallocDefault) != hipSuccess)
         rc = fail(DDAFFT_ENOMEM, "hipHostMalloc");
+    if (!rc && hipHostGetDevicePointer((void**)&c->ring_dev, c->state_ring, 0) != hipSuccess)
+        rc = fail(DDAFFT_EHIP, "hipHostGetDevicePointer of the state ring");
     if (!rc) rc = ensure_partials(c, (size_t)nrhs_max * std::max<long long>(zi_blocks(c->d), 1024) * kSlots);
     if (rc) {
         ddafft_destroy(c);
@@ -817,13 +903,10 @@ int ddafft_set_target(ddafft_ctx* c, const signed char* occ, int nmat, const int
     HIPCHECK(hipStreamSynchronize(c->stream));
     if (c->mv) hipFree(c->mv);
     if (c->cells) hipFree(c->cells);
-    if (c->alpha) hipFree(c->alpha);
     c->mv = nullptr;
     c->cells = nullptr;
-    c->alpha = nullptr;
     int rc = dev_alloc(&c->mv, (size_t)nmat);
     if (!rc) rc = dev_alloc(&c->cells, cells.size());
-    if (!rc) rc = dev_alloc(&c->alpha, (size_t)nmat * c->nrhs_max);
     if (rc) return rc;
     c->nmat = nmat;
     c->mat_count.assign(nmat, 0);
@@ -838,6 +921,29 @@ int ddafft_set_target(ddafft_ctx* c, const signed char* occ, int nmat, const int
     if ((rc = upload(c, c->cells, cells.data(), cells.size() * sizeof(int)))) return rc;
     HIPCHECK(hipStreamSynchronize(c->stream));
     c->have_target = true;
+    return DDAFFT_OK;
+}
+
+int ddafft_set_tuning(ddafft_ctx* c, int key, int value) {
+    if (!c) return fail(DDAFFT_EINVAL, "NULL context");
+    switch (key) {
+        case DDAFFT_TUNE_X_BLOCKS:
+            if (value < 1) return fail(DDAFFT_EINVAL, "x_blocks must be positive");
+            c->x_blocks = value;
+            return DDAFFT_OK;
+        case DDAFFT_TUNE_X_RB:
+            if (value != 0 && value != 1 && value != 2 && value != 4)
+                return fail(DDAFFT_EINVAL, "x_rb must be 0 (automatic), 1, 2 or 4");
+            c->x_rb = value;
+            return DDAFFT_OK;
+        default:
+            return fail(DDAFFT_EINVAL, "unknown tuning key");
+    }
+}
+
+int ddafft_set_sum_mode(ddafft_ctx* c, int mode) {
+    if (!c || mode < 0 || mode > 1) return fail(DDAFFT_EINVAL, "sum mode must be 0 or 1");
+    c->sum_mode = mode;
     return DDAFFT_OK;
 }
 
@@ -910,7 +1016,10 @@ struct Seq {
         f.nblk = nblk;
         f.partials = c->partials;
         f.hist = c->hist;
-        if (!f.partials || !f.hist) return fail(DDAFFT_ESTATE, "solver scratch not allocated");
+        // the step that closes iteration `iter` publishes the states to ring slot iter & 3
+        f.ring = ends_step(kind) ? c->ring_dev + (size_t)(iter & 3) * c->nrhs_max : nullptr;
+        if (!f.partials || !f.hist || (ends_step(kind) && !c->ring_dev))
+            return fail(DDAFFT_ESTATE, "solver scratch not allocated");
         hipLaunchKernelGGL(k_finalize, nrhs, kBlock, 0, c->stream, c->st, iter, f);
         HIPCHECK(hipGetLastError());
         return DDAFFT_OK;
@@ -1013,14 +1122,6 @@ int iteration(Seq& q, int solver, int o, float tau) {
     return DDAFFT_OK;
 }
 
-// Copies the solver states into slot `slot` of the pinned ring; the host
-// reads the stopping decisions from there once the slot's event has passed.
-int copy_states(ddafft_ctx* c, int nrhs, int slot) {
-    HIPCHECK(hipMemcpyAsync(c->state_ring + (size_t)slot * c->nrhs_max, c->st, (size_t)nrhs * sizeof(SolveState),
-                            hipMemcpyDeviceToHost, c->stream));
-    return DDAFFT_OK;
-}
-
 // Results of a solve on their way back to the host: offsets into the
 // context's pinned download buffer, filled by stream-ordered copies, read
 // after the stream (or an event after the copies) has completed.
@@ -1039,7 +1140,9 @@ int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int
     f.state = state;
     f.hist = state && hist;
     f.max_step = max_step;
-    f.cb = (int)std::min<long long>((c->d.n + kBlock - 1) / kBlock, 512);
+    // cross-section partials per right-hand side: block sums, or the one
+    // sequential float sum of the reference-order mode
+    f.cb = c->sum_mode ? 1 : (int)std::min<long long>((c->d.n + kBlock - 1) / kBlock, 512);
     const int dtiles = (ndirs + kFD - 1) / kFD;
     // dipole chunks per direction tile: enough blocks to fill the chip
     f.fchunk = ndirs > 0 ? (int)std::max(1, std::min({64, (c->ncells + 4095) / 4096,
@@ -1072,25 +1175,28 @@ int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int
         TRY(dev_alloc(&c->fftab, ntab));
         c->fftab_cap = ntab;
     }
-    return pinned_reserve(c, &c->dn, &c->dn_cap, off);
+    return pinned_reserve(c, &c->dn, &c->dn_cap, off, &c->dn_dev);
 }
 
 // Cross sections and far fields of the nrhs solutions in c->P; c->kinc and
 // c->einc hold each right-hand side's incidence.  Right-hand side r uses the
 // directions ddirs + r * dir_stride.  Queues the kernels and the copies of
 // their results into the pinned buffer.
+int obs_enqueue_seq(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_stride);
+
 int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_stride) {
+    if (c->sum_mode) return obs_enqueue_seq(c, f, ddirs, dir_stride);
     const int nrhs = f.nrhs, ndirs = f.ndirs;
     Dims d = c->d;
     d.nrhs = nrhs;
     const Geometry g = make_geometry(c, nullptr);
+    if (!c->dn_dev) return fail(DDAFFT_ESTATE, "internal: result buffer not reserved");
+    TRY(flush_small(c));
     dim3 gc(f.cb, nrhs);
+    // the block partials go straight to the pinned result buffer
     hipLaunchKernelGGL(k_cabs_cext, gc, kBlock, 0, c->stream, d, g, c->kinc, inc_phase(c), c->occ,
-                       c->alpha, c->nmat, c->einc, c->P, c->partials);
+                       c->alpha, c->nmat, c->einc, c->P, reinterpret_cast<double*>(c->dn_dev + f.o_part));
     HIPCHECK(hipGetLastError());
-    // copied before the far field reuses the partials (stream order)
-    HIPCHECK(hipMemcpyAsync(c->dn + f.o_part, c->partials, (size_t)nrhs * f.cb * 2 * sizeof(double),
-                            hipMemcpyDeviceToHost, c->stream));
     if (ndirs > 0) {
         // right-hand sides in pairs that share one direction set share the
         // phases; the phase table is rebuilt (in stream order) only when the
@@ -1119,10 +1225,62 @@ int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_s
             r0 += pair ? 2 : 1;
         }
         hipLaunchKernelGGL(k_far_field_sum, (nrhs * ndirs + kBlock - 1) / kBlock, kBlock, 0, c->stream, c->partials,
-                           f.fchunk, ndirs, nrhs, ddirs, dir_stride, c->esc);
+                           f.fchunk, ndirs, nrhs, ddirs, dir_stride, reinterpret_cast<float2*>(c->dn_dev + f.o_esc));
         HIPCHECK(hipGetLastError());
-        HIPCHECK(hipMemcpyAsync(c->dn + f.o_esc, c->esc, (size_t)nrhs * ndirs * 3 * sizeof(float2),
-                                hipMemcpyDeviceToHost, c->stream));
+    }
+    return DDAFFT_OK;
+}
+
+// The same observables with the reference's sequential float sums
+// (sum_mode 1): the cross-section terms of every dipole, then one thread per
+// sum adds them in the reference's cell order; the far field goes through
+// the dipole list in chunks (terms of a chunk formed in parallel, then each
+// of the nrhs * ndirs * 6 float sums continued over the chunk in order).
+// Results land where obs_enqueue puts them: partials [nrhs][1][2] (the float
+// sums, widened exactly) and esc.
+int obs_enqueue_seq(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_stride) {
+    const int nrhs = f.nrhs, ndirs = f.ndirs, nc = c->ncells;
+    Dims d = c->d;
+    d.nrhs = nrhs;
+    const Geometry g = make_geometry(c, nullptr);
+    // chunk of dipoles whose far-field terms fit 256 MB (at least one dipole)
+    const long long per = (long long)nrhs * std::max(ndirs, 1) * 6;
+    const int chunk = (int)std::max<long long>(1, std::min<long long>(nc, (64LL << 20) / per));
+    const size_t need = std::max<size_t>((size_t)2 * nrhs * nc, ndirs > 0 ? (size_t)chunk * per : 0);
+    if (need > c->seqbuf_cap) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->seqbuf) hipFree(c->seqbuf);
+        c->seqbuf = nullptr;
+        c->seqbuf_cap = 0;
+        TRY(dev_alloc(&c->seqbuf, need));
+        c->seqbuf_cap = need;
+    }
+    if (f.cb != 1) return fail(DDAFFT_ESTATE, "internal: sequential sums need one partial per right-hand side");
+    if (!c->dn_dev) return fail(DDAFFT_ESTATE, "internal: result buffer not reserved");
+    TRY(flush_small(c));
+    {
+        dim3 gt((int)std::min<long long>((nc + kBlock - 1) / kBlock, 1024), nrhs);
+        hipLaunchKernelGGL(k_cc_terms, gt, kBlock, 0, c->stream, d, g, c->kinc, inc_phase(c), c->cells, nc, c->occ,
+                           c->alpha, c->nmat, c->einc, c->P, c->seqbuf);
+        HIPCHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_seq_sum, (2 * nrhs + 63) / 64, 64, 0, c->stream, c->seqbuf, nc, 2 * nrhs,
+                           reinterpret_cast<double*>(c->dn_dev + f.o_part));
+        HIPCHECK(hipGetLastError());
+    }
+    if (ndirs > 0) {
+        float* acc = reinterpret_cast<float*>(c->esc);   // [nrhs][ndirs][6]
+        const int nsums = (int)per;
+        for (int e0 = 0; e0 < nc; e0 += chunk) {
+            const int cnt = std::min(chunk, nc - e0);
+            const long long tot = (long long)cnt * nrhs * ndirs;
+            hipLaunchKernelGGL(k_ff_terms, (int)std::min<long long>((tot + kBlock - 1) / kBlock, 8192), kBlock, 0,
+                               c->stream, d, g, c->cells, e0, cnt, ndirs, nrhs, ddirs, dir_stride, c->P, c->seqbuf);
+            HIPCHECK(hipGetLastError());
+            float* out = e0 + cnt >= nc ? reinterpret_cast<float*>(c->dn_dev + f.o_esc) : nullptr;
+            hipLaunchKernelGGL(k_ff_seq_sum, (nsums + kBlock - 1) / kBlock, kBlock, 0, c->stream, c->seqbuf, cnt, nsums,
+                               e0 == 0 ? 1 : 0, acc, out);
+            HIPCHECK(hipGetLastError());
+        }
     }
     return DDAFFT_OK;
 }
@@ -1190,13 +1348,21 @@ int run_begin(Run& r) {
     }
     TRY(upload(c, c->st, s0.data(), r.nrhs * sizeof(SolveState)));
     TRY(ensure_partials(c, (size_t)r.nrhs * std::max(zi_blocks(d), update_blocks(d)) * kSlots));
+    TRY(flush_small(c));   // every staged input of the solve, in one copy
+    if (c->pending_first_approx) {
+        c->pending_first_approx = false;
+        const Geometry g = make_geometry(c, nullptr);
+        dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), r.nrhs);
+        hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->kinc, inc_phase(c), c->occ, c->mv,
+                           c->alpha, c->nmat, c->einc, c->P);
+        HIPCHECK(hipGetLastError());
+    }
     r.q = Seq{c, r.nrhs, FinArgs{}};
     r.q.fa.hist = c->hist;
     r.q.fa.max_step = r.max_step;
     r.q.fa.err_max = (double)r.err_max;
     r.q.fa.inv_ndip = 1.0 / (double)c->ndip;
-    TRY(step0(r.q, r.solver));
-    TRY(copy_states(c, r.nrhs, 0));
+    TRY(step0(r.q, r.solver));   // its last scalar step publishes the states to ring slot 0
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
     r.o = 1;
     r.known_stop = r.max_step + 1;
@@ -1205,12 +1371,12 @@ int run_begin(Run& r) {
 
 bool run_more(const Run& r) { return r.o <= r.max_step && r.o < r.known_stop; }
 
-// enqueues iteration r.o and its stop flag
+// enqueues iteration r.o; its last scalar step publishes the states (the
+// stopping decisions) to ring slot o & 3, which the event marks as written
 int run_enqueue(Run& r) {
     ddafft_ctx* c = r.c;
     const int o = r.o;
     TRY(iteration(r.q, r.solver, o, r.tau));
-    TRY(copy_states(c, r.nrhs, o & 3));
     HIPCHECK(hipEventRecord(c->ev[o & 3], c->stream));
     r.o = o + 1;
     return DDAFFT_OK;
@@ -1228,17 +1394,19 @@ int run_wait(Run& r) {
     return DDAFFT_OK;
 }
 
-// best-iterate restore, then the copies of the final states (and history)
+// best-iterate restore, which also writes the final states into the pinned
+// result buffer; then the copy of the history, when asked for
 int fin_state_enqueue(Run& r, const Fin& f) {
     ddafft_ctx* c = r.c;
+    if (!c->dn_dev) return fail(DDAFFT_ESTATE, "internal: result buffer not reserved");
     Dims d = c->d;
     d.nrhs = r.nrhs;
     {
         dim3 grid((int)std::min<long long>((3 * d.n + kBlock - 1) / kBlock, 2048), r.nrhs);
-        hipLaunchKernelGGL(k_restore, grid, kBlock, 0, c->stream, d, c->st, c->P, c->Pmin);
+        hipLaunchKernelGGL(k_restore, grid, kBlock, 0, c->stream, d, c->st, c->P, c->Pmin,
+                           reinterpret_cast<SolveState*>(c->dn_dev + f.o_st));
         HIPCHECK(hipGetLastError());
     }
-    HIPCHECK(hipMemcpyAsync(c->dn + f.o_st, c->st, r.nrhs * sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
     if (f.hist)
         HIPCHECK(hipMemcpyAsync(c->dn + f.o_hist, c->hist, (size_t)r.nrhs * (r.max_step + 1) * sizeof(double),
                                 hipMemcpyDeviceToHost, c->stream));
@@ -1307,7 +1475,8 @@ int solve_core(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, flo
 }
 
 // Checks one job against its context and stages its inputs: polarizabilities,
-// incidence, directions, and the first approximation in c->P.
+// incidence, directions; run_begin then copies them and computes the first
+// approximation in c->P.
 int job_prepare(ddafft_ctx* c, const ddafft_job* j, int solver, int max_step, const float** ddirs,
                 long long* dir_stride) {
     if (!c || !j || !j->alpha || !j->kinc || !j->einc) return fail(DDAFFT_EINVAL, "NULL argument");
@@ -1359,12 +1528,9 @@ int job_prepare(ddafft_ctx* c, const ddafft_job* j, int solver, int max_step, co
     TRY(upload(c, c->alpha, j->alpha, (size_t)nrhs * c->nmat * sizeof(float2)));
     TRY(upload(c, c->einc, j->einc, (size_t)nrhs * 3 * sizeof(float2)));
     TRY(upload_incidence(c, nrhs, j->kinc, j->exact_phases != 0));
-    const Geometry g = make_geometry(c, nullptr);
-    dim3 grid((int)std::min<long long>((d.n + kBlock - 1) / kBlock, 2048), nrhs);
-    hipLaunchKernelGGL(k_first_approx, grid, kBlock, 0, c->stream, d, g, c->kinc, inc_phase(c),
-                       c->occ, c->mv, c->alpha, c->nmat,
-                       c->einc, c->P);
-    HIPCHECK(hipGetLastError());
+    // the first approximation (the sources in c->P) runs once the staged
+    // inputs have been copied, at the start of the run
+    c->pending_first_approx = true;
     return DDAFFT_OK;
 }
 
@@ -1686,6 +1852,7 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
     }
     // the weights and coefficients (the scales are computed on the device)
     TRY(upload(c, c->gaux, aux.data(), aux_bytes));
+    TRY(flush_small(c));   // the adjoint polarizabilities, before the kernels below read them
 
     // adjoint sources b = diag(alpha) g and their mean norms
     d.nrhs = nadj;

@@ -33,7 +33,22 @@ struct FinArgs {
     int max_step;
     double err_max;
     double inv_ndip;       // 1 / number of dipoles (ldError's n)
+    SolveState* ring;      // steps that end an iteration: the host-visible copy of the states
+                           // ([nrhs], page-locked host memory), else nullptr
 };
+
+// Publishes one right-hand side's state to the host-visible ring.  The host
+// reads it only after the event recorded behind the iteration has completed,
+// which orders the kernel's stores to host memory before it (no fence here:
+// a system-scope fence writes back the whole L2).
+__device__ __forceinline__ void publish_state(const FinArgs& fa, int rhs, const SolveState& s) {
+    if (fa.ring) fa.ring[rhs] = s;
+}
+
+// End-of-step kinds: the scalar step that closes an iteration (or step 0).
+__host__ __device__ constexpr bool ends_step(int kind) {
+    return kind == 3 || kind == 4 || kind == 6 || kind == 8 || kind == 10 || kind == 11;
+}
 
 __device__ __forceinline__ void ld_div(double ar, double ai, double br, double bi, double& cr, double& ci) {
     const double n = br * br + bi * bi;
@@ -162,6 +177,7 @@ __device__ __forceinline__ void finalize_block(SolveState* __restrict__ st, int 
         default:
             break;
     }
+    publish_state(fa, rhs, s);
 }
 
 }  // namespace ddafft

@@ -44,8 +44,12 @@ namespace ddafft {
 // so that the 384 butterflies of a stage fill the block's threads evenly
 // (256^3, 2 RHS: 7.16 ms at one column, 6.66 at two).
 template <int N> struct TileZF { static constexpr int T = clamp16(pow2floor(6144 / (2 * (N + kLpad)))); };
+// Four right-hand sides per tile take half as many columns as two, so the
+// tile's LDS and its transforms stay those of the two-right-hand-side tile
+// while each Â value serves twice the right-hand sides.
 template <int N, int RB> struct TileX {
-    static constexpr int T = (RB == 2 && (N == 64 || N == 128)) ? 4
+    static constexpr int T = RB == 4 ? (N <= 16 ? clamp16(pow2floor(2048 / (12 * (N + kLpad)))) : N <= 64 ? 2 : 1)
+                             : (RB == 2 && (N == 64 || N == 128)) ? 4
                              : N >= 128                           ? 2
                                                                   : clamp16(pow2floor(2048 / (6 * RB * (N + kLpad))));
 };
@@ -61,6 +65,7 @@ template <int NI, int RB> struct XStride {
 // four-column tile (48 / 24 lines, one butterfly per line per lane group)
 // was measured with the same strides.
 template <> struct XStride<64, 2> { static constexpr int S1 = 66, S2 = 131; };
+template <> struct XStride<64, 4> { static constexpr int S1 = 66, S2 = 131; };   // the same 48 / 24 lines
 // 12 lines of 128 points at stride 131 (the Z-inverse tile below): a half-wave
 // takes 32 consecutive butterflies of one line in the last stage.
 template <> struct LaneGroup<128, 12, 131, 2> { static constexpr int G = 32; };
@@ -280,12 +285,28 @@ __device__ __forceinline__ void ahat_apply(const float2* a, float2 x, float2 y, 
     oz = cadd(cmul(a[5], z), cadd(cmul(a[2], x), cmul(a[4], y)));
 }
 
-// One block's tile of the X pass.  FULL: the tile holds TJ columns (every
-// lattice with 2 n_j >= TJ), so the tile width is a compile-time constant and
-// the index arithmetic reduces to shifts and masks; otherwise tj = 2 n_j.
-template <int NI, int RB, bool FULL>
-__device__ __forceinline__ void x_tile(const Dims& d, const bool (&act)[RB], int tj_small,
-                                       const float2* __restrict__ b2, float2* __restrict__ b3,
+// Right-hand sides [rhs0, rhs1) that exist and are still running, as bits
+// from rhs0 (at most 64 right-hand sides per context).
+__device__ __forceinline__ unsigned long long x_active(const Dims& d, const SolveState* __restrict__ st, int iter,
+                                                       int rhs0, int rhs1) {
+    unsigned long long m = 0;
+    for (int r = rhs0; r < rhs1 && r < d.nrhs; ++r)
+        if (!(st && iter >= st[r].stop_at)) m |= 1ull << (r - rhs0);
+    return m;
+}
+
+// One block's tile of the X pass, for the right-hand-side groups
+// [g0, g1) of RB each (RUN; otherwise the one group g0); bit r of amask:
+// right-hand side g0 * RB + r is running.  Â of the tile is read once,
+// before the first group, and kept in registers for all of them.  The
+// single-group form is straight-line code: a loop over groups keeps the
+// per-group address arithmetic live across it (195 against 76 VGPRs at
+// n_i = 64), so it is compiled separately.  FULL: the tile holds TJ columns
+// (every lattice with 2 n_j >= TJ), so the tile width is a compile-time
+// constant and the index arithmetic reduces to shifts and masks; otherwise
+// tj = 2 n_j.
+template <int NI, int RB, bool FULL, bool RUN>
+__device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, int g0, int g1, int tj_small, const float2* __restrict__ b2, float2* __restrict__ b3,
                                        const float2* __restrict__ ahat, int conj, const float2* __restrict__ wpre,
                                        const float2* __restrict__ tw1, const float2* __restrict__ tw2, float2* s,
                                        float2* stw1, float2* stw2) {
@@ -294,7 +315,6 @@ __device__ __forceinline__ void x_tile(const Dims& d, const bool (&act)[RB], int
     constexpr int NI2 = 2 * NI;
     constexpr int LS1 = XStride<NI, RB>::S1, LS2 = XStride<NI, RB>::S2;   // forward / inverse line strides
     constexpr int ITEMS = (TJ * NI2 + kBlock - 1) / kBlock;
-    const int rhs0 = blockIdx.y * RB;
     const int NJ2 = 2 * d.nj, NL2 = 2 * d.nl;
     const int tj = FULL ? TJ : tj_small, njt = NJ2 / tj;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -328,120 +348,135 @@ __device__ __forceinline__ void x_tile(const Dims& d, const bool (&act)[RB], int
                 for (int e = 0; e < 6; ++e) a[it][e].y = -a[it][e].y;
         }
     }
-
     const int ltj = __builtin_ctz(tj);
     constexpr int LNI = ilog2c(NI);
-    {
-        // all loads of this thread are issued before the first is used; the
-        // block's lines lie at 32-bit offsets from one 64-bit base
-        constexpr int NLD = (CB * TJ * NI + kBlock - 1) / kBlock;
-        const int nld = CB * tj * NI;
-        const float2* src = b2 + ((long long)(rhs0 * 3) * NJ2 + J0) * ((long long)NL2 * NI) + (long long)L * NI;
-        const int lcs = __builtin_ctz(NJ2) + __builtin_ctz(NL2) + LNI, lts = __builtin_ctz(NL2) + LNI;
-        float2 v[NLD];
+    bool staged = false;   // twiddles in LDS
+    const int gend = RUN ? g1 : g0 + 1;
+    for (int grp = g0; grp < gend; ++grp) {
+        const int rhs0 = grp * RB;
+        const unsigned gm = (unsigned)(amask >> ((grp - g0) * RB)) & ((1u << RB) - 1u);
+        if (!gm) continue;
+        bool act[RB];
 #pragma unroll
-        for (int q = 0; q < NLD; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            v[q] = make_float2(0.f, 0.f);
-            if (f < nld) {
-                const int ii = f & (NI - 1), r = f >> LNI, t = r & (tj - 1), cc = r >> ltj;
-                if (act[RB == 1 ? 0 : (cc >= 3)]) v[q] = src[(cc << lcs) + (t << lts) + ii];
+        for (int r = 0; r < RB; ++r) act[r] = (gm >> r) & 1u;
+        if (grp > g0) __syncthreads();   // the previous group's stores have read the tile
+        {
+            // all loads of this thread are issued before the first is used; the
+            // block's lines lie at 32-bit offsets from one 64-bit base
+            constexpr int NLD = (CB * TJ * NI + kBlock - 1) / kBlock;
+            const int nld = CB * tj * NI;
+            const float2* src = b2 + ((long long)(rhs0 * 3) * NJ2 + J0) * ((long long)NL2 * NI) + (long long)L * NI;
+            const int lcs = __builtin_ctz(NJ2) + __builtin_ctz(NL2) + LNI, lts = __builtin_ctz(NL2) + LNI;
+            float2 v[NLD];
+#pragma unroll
+            for (int q = 0; q < NLD; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                v[q] = make_float2(0.f, 0.f);
+                if (f < nld) {
+                    const int ii = f & (NI - 1), r = f >> LNI, t = r & (tj - 1), cc = r >> ltj;
+                    if (act[cc / 3]) v[q] = src[(cc << lcs) + (t << lts) + ii];
+                }
+            }
+            // twiddles staged after the first field loads are issued, so that the
+            // Â and field loads are in flight together; strides of the
+            // compile-time block size (reading blockDim made the compiler wait
+            // for every load in flight first)
+            if (!staged) {
+                for (int f = threadIdx.x; f < NI; f += kBlock) stw1[f] = tw1[f];
+                for (int f = threadIdx.x; f < NI2; f += kBlock) stw2[f] = tw2[f];
+                staged = true;
+            }
+#pragma unroll
+            for (int q = 0; q < NLD; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < nld) {
+                    const int ii = f & (NI - 1), line = (f >> LNI) * 2;   // (cc * tj + t) * 2
+                    s[line * LS1 + ii] = v[q];
+                    s[(line + 1) * LS1 + ii] = cmul(wpre[ii], v[q]);
+                }
             }
         }
-#pragma unroll
-        for (int q = 0; q < NLD; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < nld) {
-                const int ii = f & (NI - 1), line = (f >> LNI) * 2;   // (cc * tj + t) * 2
-                s[line * LS1 + ii] = v[q];
-                s[(line + 1) * LS1 + ii] = cmul(wpre[ii], v[q]);
-            }
-        }
-    }
-    // strides of the compile-time block size: reading blockDim here made the
-    // compiler wait for every load in flight (Â and the field) first
-    for (int f = threadIdx.x; f < NI; f += kBlock) stw1[f] = tw1[f];
-    for (int f = threadIdx.x; f < NI2; f += kBlock) stw2[f] = tw2[f];
-    __syncthreads();
-    fft_lds<NI, 2 * CB * TJ, LS1>(s, 2 * CB * tj, stw1);
+        __syncthreads();
+        fft_lds<NI, 2 * CB * TJ, LS1>(s, 2 * CB * tj, stw1);
 
-    // Â multiply in natural order of I, one Â read for the RB right-hand
-    // sides; results land conjugated, in natural order, for the inverse
-    // transform (ifft_line, fft_kernel.hpp:301-322).
-    float2 y[ITEMS][CB];
+        // Â multiply in natural order of I, one Â read for every right-hand
+        // side; results land conjugated, in natural order, for the inverse
+        // transform (ifft_line, fft_kernel.hpp:301-322).
+        float2 y[ITEMS][CB];
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-        const int f = threadIdx.x + it * kBlock;
-        if (f < tj * NI2) {
-            const int I = f % NI2, t = f / NI2;
+        for (int it = 0; it < ITEMS; ++it) {
+            const int f = threadIdx.x + it * kBlock;
+            if (f < tj * NI2) {
+                const int I = f % NI2, t = f / NI2;
 #pragma unroll
-            for (int r = 0; r < RB; ++r) {
-                float2 xv[3];
+                for (int r = 0; r < RB; ++r) {
+                    float2 xv[3];
 #pragma unroll
-                for (int comp = 0; comp < 3; ++comp)
-                    xv[comp] = s[(((r * 3 + comp) * tj + t) * 2 + (I & 1)) * LS1 + (I >> 1)];
-                ahat_apply(a[it], xv[0], xv[1], xv[2], y[it][r * 3], y[it][r * 3 + 1], y[it][r * 3 + 2]);
+                    for (int comp = 0; comp < 3; ++comp)
+                        xv[comp] = s[(((r * 3 + comp) * tj + t) * 2 + (I & 1)) * LS1 + (I >> 1)];
+                    ahat_apply(a[it], xv[0], xv[1], xv[2], y[it][r * 3], y[it][r * 3 + 1], y[it][r * 3 + 2]);
+                }
             }
         }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-        const int f = threadIdx.x + it * kBlock;
-        if (f < tj * NI2) {
-            const int I = f % NI2, t = f / NI2;
+        for (int it = 0; it < ITEMS; ++it) {
+            const int f = threadIdx.x + it * kBlock;
+            if (f < tj * NI2) {
+                const int I = f % NI2, t = f / NI2;
 #pragma unroll
-            for (int cc = 0; cc < CB; ++cc) s[(cc * tj + t) * LS2 + I] = cconj(y[it][cc]);
+                for (int cc = 0; cc < CB; ++cc) s[(cc * tj + t) * LS2 + I] = cconj(y[it][cc]);
+            }
         }
-    }
-    __syncthreads();
-    fft_lds<NI2, CB * TJ, LS2>(s, CB * tj, stw2);
-    {
-        float2* dst = b3 + (long long)(rhs0 * 3) * NI * ((long long)NL2 * NJ2) + (long long)L * NJ2 + J0;
-        const int lrow = __builtin_ctz(NL2) + __builtin_ctz(NJ2);
-        constexpr int NST = (CB * TJ * NI + kBlock - 1) / kBlock;
-        const int nst = CB * tj * NI;
+        __syncthreads();
+        fft_lds<NI2, CB * TJ, LS2>(s, CB * tj, stw2);
+        {
+            float2* dst = b3 + (long long)(rhs0 * 3) * NI * ((long long)NL2 * NJ2) + (long long)L * NJ2 + J0;
+            const int lrow = __builtin_ctz(NL2) + __builtin_ctz(NJ2);
+            constexpr int NST = (CB * TJ * NI + kBlock - 1) / kBlock;
+            const int nst = CB * tj * NI;
 #pragma unroll
-        for (int q = 0; q < NST; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < nst) {
-                const int t = f & (tj - 1), r = f >> ltj, i = r & (NI - 1), cc = r >> LNI;
-                if (act[RB == 1 ? 0 : (cc >= 3)]) dst[(r << lrow) + t] = s[(cc * tj + t) * LS2 + i];
+            for (int q = 0; q < NST; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < nst) {
+                    const int t = f & (tj - 1), r = f >> ltj, i = r & (NI - 1), cc = r >> LNI;
+                    if (act[cc / 3]) dst[(r << lrow) + t] = s[(cc * tj + t) * LS2 + i];
+                }
             }
         }
     }
 }
 
-template <int NI, int RB>
+// Grid: x over (J, L) tiles, y over sets of `gpb` consecutive right-hand-side
+// groups of RB (RUN; otherwise gpb = 1); each block reads its tile's Â once
+// for all of its groups.
+template <int NI, int RB, bool RUN>
 __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st, int iter,
                                               const float2* __restrict__ b2, float2* __restrict__ b3,
                                               const float2* __restrict__ ahat, int conj,
                                               const float2* __restrict__ wpre, const float2* __restrict__ tw1,
-                                              const float2* __restrict__ tw2) {
-    static_assert(RB == 1 || RB == 2, "one or two right-hand sides per X tile");
+                                              const float2* __restrict__ tw2, int gpb) {
+    static_assert(RB == 1 || RB == 2 || RB == 4, "one, two or four right-hand sides per X tile");
     constexpr int TJ = TileX<NI, RB>::T;
     constexpr int LS1 = XStride<NI, RB>::S1, LS2 = XStride<NI, RB>::S2;
     static_assert(LS2 <= 2 * LS1, "the inverse lines fit the forward tile's LDS");
     __shared__ float2 s[3 * RB * TJ * 2 * LS1];
     __shared__ float2 stw1[NI];
     __shared__ float2 stw2[2 * NI];
-    const int rhs0 = blockIdx.y * RB;
-    bool act[RB];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < RB; ++r) {
-        act[r] = rhs0 + r < d.nrhs && !(st && iter >= st[rhs0 + r].stop_at);
-        any = any || act[r];
-    }
-    if (!any) return;
+    const int ngroups = (d.nrhs + RB - 1) / RB;
+    const int g0 = blockIdx.y * gpb, g1 = RUN ? min(g0 + gpb, ngroups) : g0 + 1;
+    // which right-hand sides of the run are running, read before any other
+    // load is issued (so that no later wait on them waits for Â as well)
+    const unsigned long long amask = x_active(d, st, iter, g0 * RB, g1 * RB);
+    if (!amask) return;
     // Measured on MI355X (2 RHS), against a run-time tile width: n_i = 64
     // 57.8 -> 56.3 us per launch, n_i = 128 612 -> 534 us, n_i = 256
     // 6571 -> 5688 us.
     const int tj = min(TJ, 2 * d.nj);
     if (tj == TJ)
-        x_tile<NI, RB, true>(d, act, TJ, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
+        x_tile<NI, RB, true, RUN>(d, amask, g0, g1, TJ, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
     else
-        x_tile<NI, RB, false>(d, act, tj, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
+        x_tile<NI, RB, false, RUN>(d, amask, g0, g1, tj, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
 }
 
 // ---------------------------------------------------------------------------

@@ -165,17 +165,28 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, SolveState* __restric
 __global__ __launch_bounds__(kBlock) void k_finalize(SolveState* __restrict__ st, int iter, FinArgs fa) {
     __shared__ double red[kSlots * 16];
     const int rhs = blockIdx.x;
-    if (iter >= st[rhs].stop_at) return;
+    if (iter >= st[rhs].stop_at) {   // stopped earlier: its state still goes out
+        if (threadIdx.x == 0) publish_state(fa, rhs, st[rhs]);
+        return;
+    }
     finalize_block(st, iter, fa, rhs, red);
 }
 
-// Best-iterate fallback (dda_func.cpp:1537-1548).
+// Best-iterate fallback (dda_func.cpp:1537-1548); block 0 also writes the
+// final states to `out` (host-visible, may be nullptr).
 __global__ __launch_bounds__(kBlock) void k_restore(Dims d, SolveState* __restrict__ st, float2* __restrict__ P,
-                                                    const float2* __restrict__ Pmin) {
+                                                    const float2* __restrict__ Pmin, SolveState* __restrict__ out) {
     const int rhs = blockIdx.y;
-    const SolveState s = st[rhs];
-    if (!(s.err > s.err_min)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st[rhs].restored = 1;
+    SolveState s = st[rhs];
+    const bool restore = s.err > s.err_min;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (restore) {
+            st[rhs].restored = 1;
+            s.restored = 1;
+        }
+        if (out) out[rhs] = s;   // read by the host after the solve's synchronization
+    }
+    if (!restore) return;
     const long long rb = (long long)rhs * 3 * d.n;
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < 3 * d.n;
          k += (long long)gridDim.x * blockDim.x)
@@ -388,6 +399,149 @@ __global__ __launch_bounds__(kBlock) void k_far_field_sum(const double* __restri
 #pragma unroll
     for (int c = 0; c < 3; ++c)
         out[(long long)idx * 3 + c] = make_float2((float)(s[2 * c] - n[c] * dr), (float)(s[2 * c + 1] - n[c] * di));
+}
+
+// ---------------------------------------------------------------------------
+// Reference-order float sums (opt-in, ddafft_set_sum_mode): Cext, Cabs and the
+// far field accumulated sequentially in float over the dipoles in the
+// reference's storage order (i, then j, then l: dda_func.cpp:1684-1725,
+// 1582-1618), from the same float terms.  A float sum is not associative, so
+// each sum stays one sequential chain, run by one thread; the terms of the
+// chains are formed in parallel beforehand.  Dipole cells are the compact
+// list `cells` (ascending cell index, mv != 0); every polarizability model
+// gives alpha = 0 exactly for m = 1 (the only material with mv = 0) and a
+// non-zero alpha otherwise, so the list is the set of cells CabsCext visits
+// (alpha != 0) as well.
+// ---------------------------------------------------------------------------
+
+// Cross-section terms of the dipole list: terms [nrhs][2][ncells], the Cext
+// term Im(c0 (P . conj E)) and the Cabs term Im(|P|^2 / conj(alpha)) - (2/3)|P|^2
+// exactly as k_cabs_cext forms them (dda_func.cpp:1694-1722).
+__global__ __launch_bounds__(kBlock) void k_cc_terms(Dims d, Geometry g, const float* __restrict__ kinc, IncPhase ip,
+                                                     const int* __restrict__ cells, int ncells,
+                                                     const int8_t* __restrict__ occ, const float2* __restrict__ alpha,
+                                                     int nmat, const float2* __restrict__ einc,
+                                                     const float2* __restrict__ P, float* __restrict__ terms) {
+    const int rhs = blockIdx.y;
+    const float kx = kinc[3 * rhs], ky = kinc[3 * rhs + 1], kz = kinc[3 * rhs + 2];
+    const long long rb = (long long)rhs * 3 * d.n;
+    const float2 e0 = cconj(einc[rhs * 3]), e1 = cconj(einc[rhs * 3 + 1]), e2 = cconj(einc[rhs * 3 + 2]);
+    const float r4 = (float)(2. / 3.);
+    const int sl = __builtin_ctz(d.nl), sj = __builtin_ctz(d.nj);
+    float* t1 = terms + (long long)rhs * 2 * ncells;
+    float* t2 = t1 + ncells;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ncells; e += gridDim.x * blockDim.x) {
+        const int k = cells[e];
+        const int l = k & (d.nl - 1), ij = k >> sl;
+        const int j = ij & (d.nj - 1), i = ij >> sj;
+        const float2 a = alpha[(long long)rhs * nmat + occ[k]];
+        const float rx = (float)i * g.d0, ry = (float)j * g.d0, rz = (float)l * g.d0;
+        const float r3 = -(kx * (rx - g.ex) + ky * (ry - g.ey) + kz * (rz - g.ez));
+        const int ax = ip.axis[rhs];
+        const float2 c0 = ax >= 0    ? ip.tab[((size_t)rhs * 2 + 1) * ip.tn + axis_coord(ax, i, j, l)]
+                          : ax == -2 ? ip.cell[((size_t)rhs * 2 + 1) * d.n + k]
+                                     : exp_i(r3);
+        const float2 px = P[rb + k], py = P[rb + d.n + k], pz = P[rb + 2 * d.n + k];
+        const float2 c5 = cadd(cmul(pz, e2), cadd(cmul(px, e0), cmul(py, e1)));
+        t1[e] = cmul(c0, c5).y;
+        const float nx = px.x * px.x + px.y * px.y, ny = py.x * py.x + py.y * py.y, nz = pz.x * pz.x + pz.y * pz.y;
+        const float ix = px.y * px.x - px.x * px.y, iy = py.y * py.x - py.x * py.y, iz = pz.y * pz.x - pz.x * pz.y;
+        const float2 q = make_float2(nz + (nx + ny), iz + (ix + iy));
+        const float2 ca = make_float2(a.x, -a.y);
+        const float nn = ca.x * ca.x + ca.y * ca.y;
+        const float qi = (-q.x * ca.y + q.y * ca.x) / nn;
+        t2[e] = qi - r4 * q.x;
+    }
+}
+
+// Sequential float sums: thread t < nsums adds src[t * len + e] for e = 0, 1,
+// ..., len - 1 into a float that starts at 0 (dda_func.cpp:1684-1725: r1, r2
+// start at 0 and take one += per dipole) and stores it widened to double
+// (exact) in out[t].
+__global__ __launch_bounds__(64) void k_seq_sum(const float* __restrict__ src, int len, int nsums,
+                                                double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsums) return;
+    const float* p = src + (long long)t * len;
+    float s = 0.f;
+    int e = 0;
+    for (; e + 16 <= len; e += 16) {   // loads in flight together, adds in order
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = p[e + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += v[u];
+    }
+    for (; e < len; ++e) s += p[e];
+    out[t] = (double)s;
+}
+
+// Far-field terms of the dipoles [e0, e0 + cnt) of the list, for every
+// right-hand side and direction (FarField, dda_func.cpp:1586-1614):
+//   c0 = exp(i n.(r - c)),  u = n . P,  term = c0 (P - n u)
+// with the reference's float operations.  The phase is the correctly rounded
+// float of exp(i arg) for the reference's float argument (glibc's cosf/sinf
+// differ from it by one ulp for about 1% of arguments).
+// T [cnt][nrhs][ndirs][6] floats (x, y, z as re, im); right-hand side r uses
+// the directions dirs + r * dir_stride.
+__global__ __launch_bounds__(kBlock) void k_ff_terms(Dims d, Geometry g, const int* __restrict__ cells, int e0, int cnt,
+                                                     int ndirs, int nrhs, const float* __restrict__ dirs,
+                                                     long long dir_stride, const float2* __restrict__ P,
+                                                     float* __restrict__ T) {
+    const long long total = (long long)cnt * nrhs * ndirs;
+    const int sl = __builtin_ctz(d.nl), sj = __builtin_ctz(d.nj);
+    for (long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; f < total;
+         f += (long long)gridDim.x * blockDim.x) {
+        const int dd = (int)(f % ndirs);
+        const long long er = f / ndirs;
+        const int rhs = (int)(er % nrhs), e = (int)(er / nrhs);
+        const int k = cells[e0 + e];
+        const int l = k & (d.nl - 1), ij = k >> sl;
+        const int j = ij & (d.nj - 1), i = ij >> sj;
+        const float* nv = dirs + rhs * dir_stride + 3 * dd;
+        const float kx = nv[0], ky = nv[1], kz = nv[2];
+        const float rx = (float)i * g.d0 - g.cx, ry = (float)j * g.d0 - g.cy, rz = (float)l * g.d0 - g.cz;
+        const float r0 = kx * rx + ky * ry + kz * rz;
+        const float2 c0 = exp_i(r0);
+        const long long rb = (long long)rhs * 3 * d.n;
+        const float2 px = P[rb + k], py = P[rb + d.n + k], pz = P[rb + 2 * d.n + k];
+        float2 c1 = rmul(kx, px), c2 = rmul(ky, py), c3 = rmul(kz, pz);
+        const float2 c4 = cadd(c1, c2);
+        c1 = cadd(c3, c4);
+        c2 = rmul(kx, c1);
+        c3 = rmul(ky, c1);
+        const float2 c4b = rmul(kz, c1);
+        const float2 o0 = cmul(c0, csub(px, c2)), o1 = cmul(c0, csub(py, c3)), o2 = cmul(c0, csub(pz, c4b));
+        float* o = T + f * 6;
+        o[0] = o0.x;
+        o[1] = o0.y;
+        o[2] = o1.x;
+        o[3] = o1.y;
+        o[4] = o2.x;
+        o[5] = o2.y;
+    }
+}
+
+// Continues the sequential far-field sums over one chunk of terms: thread
+// t < nsums (= nrhs * ndirs * 6) adds T[e * nsums + t], e = 0 .. cnt - 1, to
+// acc[t] (first chunk: to 0), in dipole order.  acc and out (last chunk
+// only, else nullptr) hold the far field [nrhs][ndirs][3] complex float.
+__global__ __launch_bounds__(kBlock) void k_ff_seq_sum(const float* __restrict__ T, int cnt, int nsums, int first,
+                                                       float* __restrict__ acc, float* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsums) return;
+    float s = first ? 0.f : acc[t];
+    int e = 0;
+    for (; e + 16 <= cnt; e += 16) {   // loads in flight together, adds in order
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = T[(long long)(e + u) * nsums + t];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += v[u];
+    }
+    for (; e < cnt; ++e) s += T[(long long)e * nsums + t];
+    acc[t] = s;
+    if (out) out[t] = s;   // last chunk: the host-visible result
 }
 
 // ---------------------------------------------------------------------------

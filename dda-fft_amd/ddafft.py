@@ -29,7 +29,8 @@ EXPORTS = (
     "ddafft_target_hierarchical_ellipsoid", "ddafft_target_sphere_spec", "ddafft_worm_count", "ddafft_target_worms",
     "ddafft_target_file_read", "ddafft_target_file_write", "ddafft_orientation_mc", "ddafft_rotate",
     "ddafft_target_mixture", "ddafft_agglomerate_count", "ddafft_target_agglomerate",
-    "ddafft_host_alloc", "ddafft_host_free",
+    "ddafft_host_alloc", "ddafft_host_free", "ddafft_set_sum_mode",
+    "ddafft_set_tuning",
 )
 
 
@@ -62,6 +63,8 @@ def load(path=LIB_PATH):
     lib.ddafft_get_ahat.argtypes = [P, P]
     lib.ddafft_set_target.argtypes = [P, P, I, P]
     lib.ddafft_set_directions.argtypes = [P, I, P]
+    lib.ddafft_set_sum_mode.argtypes = [P, I]
+    lib.ddafft_set_tuning.argtypes = [P, I, I]
     lib.ddafft_convolution.argtypes = [P, P, P, I]
     lib.ddafft_matvec.argtypes = [P, P, P, P, I]
     lib.ddafft_solve.argtypes = [P, I, I, F, I, F, P, P, P, I, P, P, P, P, P, P]
@@ -152,6 +155,20 @@ class Context:
         mv = np.ascontiguousarray(mv, np.int32)
         _check(_lib.ddafft_set_target(self._h, _p(occ), int(mv.size), _p(mv)))
         self.nmat = int(mv.size)
+
+    def set_sum_mode(self, mode):
+        """0: FP64 sums of the observables (default); 1: the reference's
+        sequential float sums in its cell order (ddafft_set_sum_mode)."""
+        _check(_lib.ddafft_set_sum_mode(self._h, int(mode)))
+
+    def set_tuning(self, x_blocks=None, x_rb=None):
+        """Launch schedule (ddafft_set_tuning): x_blocks, the X-pass launch
+        size up to which right-hand-side groups are spread over blocks;
+        x_rb, right-hand sides per X tile (0 automatic, 1, 2 or 4)."""
+        if x_blocks is not None:
+            _check(_lib.ddafft_set_tuning(self._h, 0, int(x_blocks)))
+        if x_rb is not None:
+            _check(_lib.ddafft_set_tuning(self._h, 1, int(x_rb)))
 
     def set_directions(self, dirs):
         dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)

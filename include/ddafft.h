@@ -111,6 +111,25 @@ int ddafft_cabs_cext(ddafft_ctx* ctx, const float* alpha, const float* kinc, con
  * (FarField, dda_func.cpp:1576-1625). */
 int ddafft_far_field(ddafft_ctx* ctx, const float* p, int ndirs, const float* dirs, float* esc);
 
+/* How the observables of every later solve on this context are summed
+ * (Cext, Cabs: CabsCext, dda_func.cpp:1679-1727; far field: FarField,
+ * dda_func.cpp:1576-1625).  mode 0 (default): the reference's per-dipole
+ * float terms summed in FP64.  mode 1: the reference's own sums, sequential
+ * float accumulation over the dipoles in its cell order from the same float
+ * terms (far-field phases: the correctly rounded float of exp(i arg) for the
+ * reference's float argument); a parity mode, slower than mode 0. */
+int ddafft_set_sum_mode(ddafft_ctx* ctx, int mode);
+/* Launch schedule of the passes (no counterpart in the reference: the
+ * knobs of the autotuner, which measures them per lattice and right-hand
+ * side count).  The fused X pass works on groups of X_RB right-hand sides
+ * that share each Â value read (1, 2 or 4; 0, the default: 4 when the
+ * right-hand sides come in fours and n_i <= 32, else 2); each block takes a tile of
+ * (J, L) columns and a run of groups, and reads the tile's Â once for the
+ * whole run.  Groups are spread over more blocks (shorter runs) until a
+ * launch has X_BLOCKS blocks (default: every group its own block). */
+#define DDAFFT_TUNE_X_BLOCKS 0
+#define DDAFFT_TUNE_X_RB 1
+int ddafft_set_tuning(ddafft_ctx* ctx, int key, int value);
 /* One instance's solve for ddafft_solve_ex / ddafft_solve_batch: nrhs
  * right-hand sides, each with its own incidence (Monte-Carlo orientations x
  * polarizations of one target, main.cpp:875-940), optionally each with its

@@ -185,6 +185,8 @@ void launch(F f, dim3 g, dim3 b, A... args) {
 
 }  // namespace emul
 
+inline void __threadfence_system() {}   // blocks run in turn: every store is already visible
+
 inline void __syncthreads() {
     emul::g_fib[emul::g_cur].state = 1;
     emul::to_main();
@@ -240,6 +242,10 @@ inline hipError_t hipFree(void* p) {
     return hipSuccess;
 }
 inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) { return hipMalloc(p, bytes); }
+inline hipError_t hipHostGetDevicePointer(void** d, void* h, unsigned) {
+    *d = h;   // one address space here, as with page-locked memory mapped for the GPU
+    return hipSuccess;
+}
 inline hipError_t hipHostFree(void* p) { return hipFree(p); }
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
     std::memcpy(d, s, n);

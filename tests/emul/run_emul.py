@@ -88,6 +88,8 @@ def main():
             k, t = ctx.time_matvec(2, 1)
             ok &= check(t >= 0, "time_matvec")
     ok &= check_batch(orc)
+    ok &= check_rhs_groups(orc)
+    ok &= check_reference_sums(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -147,6 +149,66 @@ def check_long_axes(orc):
                 ok &= check(res[p]["steps"] == o["steps"] and np.array_equal(bits(ctx.dipoles(p)), bits(o["p"])),
                             "solve %s pol %d" % (dims, p))
     return ok
+
+def check_rhs_groups(orc):
+    """Six right-hand sides with the X pass taking all its groups in one
+    block (x_blocks = 1: one Â read for all of them) or one group per block,
+    in groups of two or of four (the second group half empty): dipoles
+    bit-identical to the oracle's."""
+    ok = True
+    dims, kd = (8, 4, 8), np.float32(0.5)
+    occ, _ = orc.target_ellipsoid(dims, 0.5, 1.0)
+    mv = np.array([0, 1], np.int32)
+    ms = [1.5 + 0.01j, 1.6 + 0.02j, 1.33 + 0.001j]
+    alpha = np.array([[0, orc.polarizability(1, kd, ms[r // 2])] for r in range(6)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.array([np.eye(3)[r % 2] for r in range(6)], np.complex64)
+    A = orc.build_ahat(dims, kd)
+    for xb, rb in ((1, 0), (4096, 0), (1 << 30, 4), (1, 4)):
+        with ddafft.Context(dims, kd, nrhs_max=6) as ctx:
+            ctx.set_target(occ, mv)
+            ctx.set_tuning(x_blocks=xb, x_rb=rb)
+            res = ctx.solve("bicgstab", alpha, kinc, einc, max_step=3, err_max=1e-9)
+            for r in range(6):
+                o = orc.dda_fft(2, occ, mv, alpha[r], kd, kinc, einc[r], A, np.zeros((0, 3), np.float32), 3, 1e-9,
+                                return_p=True)
+                ok &= check(res[r]["steps"] == o["steps"] and np.array_equal(bits(ctx.dipoles(r)), bits(o["p"])),
+                            "6 right-hand sides, x_blocks %d, x_rb %d, rhs %d" % (xb, rb, r))
+    return ok
+
+
+def check_reference_sums(orc):
+    """Sum mode 1: Cext and Cabs bit-identical to the oracle's sequential
+    float sums, the far field to its FarField with correctly rounded phases,
+    for shared and for per-right-hand-side directions."""
+    ok = True
+    dims, kd = (8, 8, 4), np.float32(0.6)
+    occ, _ = orc.target_ellipsoid(dims, 1.0, 0.5)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.01j)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = np.concatenate([directions(np.arange(0, 181, 20), p) for p in (0.0, 45.0)])
+    A = orc.build_ahat(dims, kd)
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        ctx.set_sum_mode(1)
+        for per_rhs in (False, True):
+            d2 = np.stack([dirs, dirs[::-1].copy()]) if per_rhs else dirs
+            res = ctx.solve_ex("bicgstab", alpha, np.stack([kinc, kinc]), einc, dirs=d2, max_step=4, err_max=1e-6)
+            for p in range(2):
+                o = orc.dda_fft(2, occ, mv, alpha, kd, kinc, einc[p], A, np.zeros((0, 3), np.float32), 4, 1e-6,
+                                return_p=True)
+                dd = d2[p] if per_rhs else dirs
+                orc.set_phase_rounding(True)
+                esc = np.array([orc.far_field(occ, mv, kd, o["p"], d) for d in dd]).astype(np.complex64)
+                orc.set_phase_rounding(False)
+                good = (np.float32(res[p]["cext"]).tobytes() == np.float32(o["cext"]).tobytes()
+                        and np.float32(res[p]["cabs"]).tobytes() == np.float32(o["cabs"]).tobytes()
+                        and np.array_equal(bits(res[p]["esc"]), bits(esc)))
+                ok &= check(good, "reference sums, per-rhs directions %d, pol %d" % (per_rhs, p))
+    return ok
+
 
 def check_batch(orc):
     """ddafft_solve_batch over two uneven lattices (width 1: the refill path),

@@ -173,3 +173,66 @@ def test_solve_golden(orc32, tag, solvers):
                 np.testing.assert_allclose(got["cabs"], g[key + "_cabs"], rtol=1e-4, err_msg=key)
                 np.testing.assert_allclose(got["cext"], g[key + "_cext"], rtol=2e-5, err_msg=key)
 
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+@pytest.mark.parametrize("case", range(len(SOLVE_CASES)))
+def test_reference_sums_bitexact(orc32, solver, case):
+    """Sum mode 1 (the reference's sequential float sums): Cext and Cabs are
+    the oracle's (the reference's) bits; the far field equals the oracle's
+    FarField with correctly rounded phases bit for bit."""
+    dims, x, m, tol, steps = SOLVE_CASES[case]
+    kd = np.float32(2 * x / dims[0])
+    occ = target(orc32, dims)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, m)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = directions(np.arange(0, 181, 10))
+    print("reference sums", solver, dims, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        ctx.set_sum_mode(1)
+        res = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=steps, err_max=tol, tau=0.5)
+        A = ctx.ahat()
+    for pol in range(2):
+        ref = orc32.dda_fft(solver, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=steps, err_max=tol,
+                            tau=0.5, return_p=True)
+        got = res[pol]
+        assert got["steps"] == ref["steps"]
+        assert bits(np.float32(got["cext"])) == bits(np.float32(ref["cext"]))
+        assert bits(np.float32(got["cabs"])) == bits(np.float32(ref["cabs"]))
+        orc32.set_phase_rounding(True)
+        try:
+            esc = np.array([orc32.far_field(occ, mv, kd, ref["p"], d) for d in dirs])
+        finally:
+            orc32.set_phase_rounding(False)
+        assert np.array_equal(bits(got["esc"]), bits(esc.astype(np.complex64)))
+
+
+@pytest.mark.parametrize("tag,solvers", [("s16", (0, 1, 2, 3)), ("e8", (0, 1, 2, 3)), ("a32", (2,))])
+def test_reference_sums_golden(orc32, tag, solvers):
+    """Sum mode 1 against the reference's own DDA_FFT outputs: Cext and Cabs
+    bit for bit, the far field within 2e-6 of its largest value (correctly
+    rounded phases against glibc's)."""
+    g = golden("solves.npz")
+    dims = {"s16": (16, 16, 16), "e8": (8, 8, 16), "a32": (32, 32, 32)}[tag]
+    kd = g[tag + "_kd"]
+    alpha = g[tag + "_alpha"]
+    occ = target(orc32, dims)
+    dirs = directions(np.arange(0, 181, 10))
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, [0, 1])
+        ctx.set_sum_mode(1)
+        for solver in solvers:
+            tol, steps = g["%s_sv%d_p0_cfg" % (tag, solver)]
+            res = ctx.solve(solver, alpha, [0, 0, 1], np.eye(3, dtype=np.complex64)[:2], dirs=dirs,
+                            max_step=int(steps), err_max=float(tol), tau=0.5)
+            for pol in range(2):
+                key = "%s_sv%d_p%d" % (tag, solver, pol)
+                got = res[pol]
+                e = _relerr(got["esc"], g[key + "_esc"])
+                print(key, "esc", e, "cabs", got["cabs"], g[key + "_cabs"], "cext", got["cext"], g[key + "_cext"],
+                      flush=True)
+                assert bits(np.float32(got["cext"])) == bits(np.float32(g[key + "_cext"])), key
+                assert bits(np.float32(got["cabs"])) == bits(np.float32(g[key + "_cabs"])), key
+                assert e < 2e-6, key
