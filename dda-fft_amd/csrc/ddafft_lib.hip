@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -106,13 +107,56 @@ void green_octant(int ni, int nj, int nl, float d0, std::vector<float2>& oct) {
     }
 }
 
+// Set while an allocation-free call (ddafft_solve_noalloc) runs on this host
+// thread: any path that would allocate fails loudly instead.
+thread_local bool t_noalloc = false;
+
 template <class T>
 int dev_alloc(T** p, size_t count) {
     *p = nullptr;
     if (count == 0) return DDAFFT_OK;
+    if (t_noalloc) return fail(DDAFFT_ESTATE, "internal: device allocation inside the allocation-free solve");
     hipError_t e = hipMalloc((void**)p, count * sizeof(T));
     if (e != hipSuccess) return fail(DDAFFT_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     return DDAFFT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device scratch of the allocation-free solve (ddafft_solve_noalloc): a
+// file-scope array of fixed size, one copy per GPU, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel (its host-side
+// name is only a stand-in); it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device when a context is
+// created there.  Every allocation-free call on one GPU shares that GPU's
+// array: such calls must not overlap.  Nothing in it needs to start at zero
+// (every kernel that reads a part of it reads what an earlier kernel of the
+// same call wrote).
+// ---------------------------------------------------------------------------
+
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_dev[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Device address of this device's scratch (the calling thread's current
+// device), looked up once; nullptr when the lookup failed.
+char* scratch_address(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_dev[dev] = e == hipSuccess ? static_cast<char*>(p) : nullptr;
+        g_scratch_done[dev] = true;
+    }
+    return g_scratch_dev[dev];
 }
 
 }  // namespace
@@ -196,6 +240,8 @@ struct ddafft_ctx {
     size_t small_bytes = 0, dirty_lo = 0, dirty_hi = 0;
     bool pending_first_approx = false; // job staged: the first approximation runs after the copy
     char* dn_dev = nullptr;            // device address of the pinned result buffer dn
+    char* scratch = nullptr;           // this device's allocation-free scratch (hipGetSymbolAddress), or nullptr
+    alignas(256) char noalloc_dn[1 << 20];   // host side of the results of an allocation-free solve
     hipEvent_t fin_ev = nullptr;       // marks the results of a batched solve as copied back
     // pinned staging of the per-solve transfers, so that they are truly
     // asynchronous: uploads are packed at up_off (reset when a solve starts,
@@ -546,6 +592,7 @@ int flush_small(ddafft_ctx* c) {
 // receives the device address of the buffer.
 int pinned_reserve(ddafft_ctx* c, char** p, size_t* cap, size_t bytes, char** dev = nullptr) {
     if (*p && *cap >= bytes) return DDAFFT_OK;
+    if (t_noalloc) return fail(DDAFFT_ESTATE, "internal: pinned allocation inside the allocation-free solve");
     HIPCHECK(hipStreamSynchronize(c->stream));
     if (*p) hipHostFree(*p);
     *p = nullptr;
@@ -601,6 +648,9 @@ int small_layout(ddafft_ctx* c) {
 int upload_begin(ddafft_ctx* c, size_t bytes) {
     HIPCHECK(hipStreamSynchronize(c->stream));
     c->up_off = 0;
+    // allocation-free: stage only into what exists (upload() copies the rest
+    // from the caller's memory and waits)
+    if (t_noalloc) return DDAFFT_OK;
     return pinned_reserve(c, &c->up, &c->up_cap, bytes);
 }
 
@@ -804,6 +854,7 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
         if (e != hipSuccess && rc == DDAFFT_OK) rc = fail(DDAFFT_EHIP, std::string(what) + ": " + hipGetErrorString(e));
     };
     chk(hipGetDevice(&c->device), "hipGetDevice");
+    if (rc == DDAFFT_OK) c->scratch = scratch_address(c->device);   // set-up of the allocation-free solve
     chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (hipEvent_t& e : c->ev) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     chk(hipEventCreateWithFlags(&c->fin_ev, hipEventDisableTiming), "hipEventCreate");
@@ -1128,8 +1179,19 @@ int iteration(Seq& q, int solver, int o, float tau) {
 struct Fin {
     int nrhs = 0, ndirs = 0, cb = 0, fchunk = 0, max_step = 0;
     bool state = false, hist = false;
-    size_t o_st = 0, o_hist = 0, o_part = 0, o_esc = 0;
+    size_t o_st = 0, o_hist = 0, o_part = 0, o_esc = 0, total = 0;
 };
+
+// Block counts of the observables: cross-section partials per right-hand
+// side (block sums, or the one sequential float sum of the reference-order
+// mode) and dipole chunks per far-field direction tile (enough blocks to
+// fill the chip).
+void obs_blocks(const ddafft_ctx* c, int ndirs, int* cb, int* fchunk) {
+    *cb = c->sum_mode ? 1 : (int)std::min<long long>((c->d.n + kBlock - 1) / kBlock, 512);
+    const int dtiles = (ndirs + kFD - 1) / kFD;
+    *fchunk = ndirs > 0 ? (int)std::max(1, std::min({64, (c->ncells + 4095) / 4096,
+                                                     (2048 + dtiles - 1) / std::max(dtiles, 1)})) : 0;
+}
 
 // Sizes the downloads and reserves the pinned buffer (before anything of this
 // solve is queued to it).
@@ -1140,13 +1202,7 @@ int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int
     f.state = state;
     f.hist = state && hist;
     f.max_step = max_step;
-    // cross-section partials per right-hand side: block sums, or the one
-    // sequential float sum of the reference-order mode
-    f.cb = c->sum_mode ? 1 : (int)std::min<long long>((c->d.n + kBlock - 1) / kBlock, 512);
-    const int dtiles = (ndirs + kFD - 1) / kFD;
-    // dipole chunks per direction tile: enough blocks to fill the chip
-    f.fchunk = ndirs > 0 ? (int)std::max(1, std::min({64, (c->ncells + 4095) / 4096,
-                                                      (2048 + dtiles - 1) / std::max(dtiles, 1)})) : 0;
+    obs_blocks(c, ndirs, &f.cb, &f.fchunk);
     size_t off = 0;
     auto take = [&](size_t b) {
         const size_t o = off;
@@ -1157,6 +1213,7 @@ int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int
     if (f.hist) f.o_hist = take((size_t)nrhs * (max_step + 1) * sizeof(double));
     f.o_part = take((size_t)nrhs * f.cb * 2 * sizeof(double));
     if (ndirs > 0) f.o_esc = take((size_t)nrhs * ndirs * 3 * sizeof(float2));
+    f.total = off;
     TRY(ensure_partials(c, std::max<size_t>((size_t)nrhs * f.cb * 2, (size_t)nrhs * f.fchunk * std::max(ndirs, 1) * 6)));
     if (ndirs > 0 && nrhs * ndirs * 3 > c->esc_cap) {
         HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1243,9 +1300,11 @@ int obs_enqueue_seq(ddafft_ctx* c, const Fin& f, const float* ddirs, long long d
     Dims d = c->d;
     d.nrhs = nrhs;
     const Geometry g = make_geometry(c, nullptr);
-    // chunk of dipoles whose far-field terms fit 256 MB (at least one dipole)
+    // chunk of dipoles whose far-field terms fit 256 MB, or the space the
+    // allocation-free call has left (at least one dipole)
     const long long per = (long long)nrhs * std::max(ndirs, 1) * 6;
-    const int chunk = (int)std::max<long long>(1, std::min<long long>(nc, (64LL << 20) / per));
+    const long long budget = t_noalloc ? (long long)c->seqbuf_cap : (64LL << 20);
+    const int chunk = (int)std::max<long long>(1, std::min<long long>(nc, budget / per));
     const size_t need = std::max<size_t>((size_t)2 * nrhs * nc, ndirs > 0 ? (size_t)chunk * per : 0);
     if (need > c->seqbuf_cap) {
         HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1407,9 +1466,9 @@ int fin_state_enqueue(Run& r, const Fin& f) {
                            reinterpret_cast<SolveState*>(c->dn_dev + f.o_st));
         HIPCHECK(hipGetLastError());
     }
-    if (f.hist)
-        HIPCHECK(hipMemcpyAsync(c->dn + f.o_hist, c->hist, (size_t)r.nrhs * (r.max_step + 1) * sizeof(double),
-                                hipMemcpyDeviceToHost, c->stream));
+    if (f.hist)   // into the result buffer's device address (page-locked memory, or the scratch)
+        HIPCHECK(hipMemcpyAsync(c->dn_dev + f.o_hist, c->hist, (size_t)r.nrhs * (r.max_step + 1) * sizeof(double),
+                                hipMemcpyDefault, c->stream));
     return DDAFFT_OK;
 }
 
@@ -1558,6 +1617,8 @@ int ddafft_solve_ex(ddafft_ctx* c, const ddafft_job* job, int solver, float tau,
     TRY(fin_state_enqueue(r, f));
     TRY(obs_enqueue(c, f, ddirs, stride));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    if (t_noalloc)   // the results sit in the device scratch: one copy to the context's host buffer
+        HIPCHECK(hipMemcpy(c->dn, c->dn_dev, f.total, hipMemcpyDeviceToHost));
     std::vector<SolveState> sf;
     TRY(fin_state_complete(r, f, err_hist, job->info, sf));
     obs_complete(c, f, job->einc, job->esc, job->cabs, job->cext);
@@ -1584,6 +1645,112 @@ int ddafft_solve(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, f
     j.cext = cext;
     j.info = info;
     return ddafft_solve_ex(c, &j, solver, tau, max_step, err_max, err_hist);
+}
+
+int ddafft_solve_noalloc(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, float err_max,
+                         const float* alpha, const float* kinc, const float* einc, int ndirs, const float* dirs,
+                         float* esc, float* cabs, float* cext, double* err_hist, int* info) {
+    if (!c) return fail(DDAFFT_EINVAL, "NULL context");
+    if (!c->scratch) return fail(DDAFFT_ESTATE, "the allocation-free scratch is not available on this device");
+    if (!c->have_target) return fail(DDAFFT_ESTATE, "set_target first");
+    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDAFFT_EINVAL, "nrhs outside [1, nrhs_max]");
+    if (max_step < 0 || ndirs < 0) return fail(DDAFFT_EINVAL, "bad max_step or ndirs");
+    if (ndirs > 0 && !dirs && ndirs != c->ndirs)
+        return fail(DDAFFT_EINVAL, "dirs == NULL but ndirs differs from the resident directions");
+    if (c->prof) return fail(DDAFFT_ESTATE, "profiling records events: disable it for the allocation-free solve");
+    const Dims& d = c->d;
+    // this call's pieces of the scratch
+    int cb = 0, fchunk = 0;
+    obs_blocks(c, ndirs, &cb, &fchunk);
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        const size_t o = off;
+        off += (b + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t n_part = std::max({(size_t)nrhs * std::max(zi_blocks(d), update_blocks(d)) * kSlots,
+                                    (size_t)nrhs * cb * 2, (size_t)nrhs * fchunk * std::max(ndirs, 1) * 6});
+    const size_t n_hist = (size_t)c->nrhs_max * (max_step + 1);
+    const size_t n_esc = (size_t)nrhs * ndirs * 3, n_tab = (size_t)ndirs * (d.ni + d.nj + d.nl);
+    const size_t n_dirs = dirs ? (size_t)ndirs * 3 : 0;
+    size_t res = 0;   // results, laid out as fin_layout lays them out
+    {
+        auto t = [&](size_t b) { res += (b + 255) & ~(size_t)255; };
+        t((size_t)nrhs * sizeof(SolveState));
+        if (err_hist) t((size_t)nrhs * (max_step + 1) * sizeof(double));
+        t((size_t)nrhs * cb * 2 * sizeof(double));
+        if (ndirs > 0) t((size_t)nrhs * ndirs * 3 * sizeof(float2));
+    }
+    const size_t o_part = take(n_part * sizeof(double)), o_hist = take(n_hist * sizeof(double));
+    const size_t o_esc = take(n_esc * sizeof(float2)), o_tab = take(n_tab * sizeof(float2));
+    const size_t o_dirs = take(n_dirs * sizeof(float)), o_res = take(res);
+    size_t o_seq = 0, n_seq = 0;
+    if (c->sum_mode) {   // the sequential-sum terms get the rest
+        o_seq = take(0);
+        n_seq = off < kScratchBytes ? (kScratchBytes - off) / sizeof(float) : 0;
+        if (n_seq < (size_t)2 * nrhs * c->ncells || n_seq < (size_t)nrhs * std::max(ndirs, 1) * 6)
+            return fail(DDAFFT_EINVAL, "the reference-order sums of this solve exceed the allocation-free scratch");
+    }
+    if (off > kScratchBytes || res > sizeof(c->noalloc_dn))
+        return fail(DDAFFT_EINVAL, "this solve needs " + std::to_string(off) + " bytes of scratch (" +
+                                       std::to_string(res) + " of results); the allocation-free solve has " +
+                                       std::to_string(kScratchBytes) + " (" + std::to_string(sizeof(c->noalloc_dn)) +
+                                       ")");
+    // bind the scratch for the duration of the call (every capacity check
+    // then passes without allocating), then put the context back
+    struct Saved {
+        double* partials; size_t partials_cap;
+        double* hist; int hist_cap;
+        float2* esc; int esc_cap;
+        float2* fftab; size_t fftab_cap;
+        float* dirs; int dirs_cap, ndirs;
+        float* seqbuf; size_t seqbuf_cap;
+        char *dn, *dn_dev; size_t dn_cap;
+    } sv{c->partials, c->partials_cap, c->hist, c->hist_cap, c->esc, c->esc_cap, c->fftab, c->fftab_cap, c->dirs,
+         c->dirs_cap, c->ndirs, c->seqbuf, c->seqbuf_cap, c->dn, c->dn_dev, c->dn_cap};
+    HIPCHECK(hipStreamSynchronize(c->stream));   // nothing of an earlier call still uses the old buffers
+    char* sc = c->scratch;
+    c->partials = reinterpret_cast<double*>(sc + o_part);
+    c->partials_cap = n_part;
+    c->hist = reinterpret_cast<double*>(sc + o_hist);
+    c->hist_cap = max_step + 1;
+    c->esc = reinterpret_cast<float2*>(sc + o_esc);
+    c->esc_cap = (int)n_esc;
+    c->fftab = reinterpret_cast<float2*>(sc + o_tab);
+    c->fftab_cap = n_tab;
+    if (dirs) {
+        c->dirs = reinterpret_cast<float*>(sc + o_dirs);
+        c->dirs_cap = ndirs;
+    }
+    if (c->sum_mode) {
+        c->seqbuf = reinterpret_cast<float*>(sc + o_seq);
+        c->seqbuf_cap = n_seq;
+    }
+    c->dn = c->noalloc_dn;
+    c->dn_dev = sc + o_res;
+    c->dn_cap = sizeof(c->noalloc_dn);
+    t_noalloc = true;
+    const int rc = ddafft_solve(c, nrhs, solver, tau, max_step, err_max, alpha, kinc, einc, ndirs, dirs, esc, cabs,
+                                cext, err_hist, info);
+    t_noalloc = false;
+    if (rc) hipStreamSynchronize(c->stream);   // a failed call may have left work queued on the scratch
+    c->partials = sv.partials;
+    c->partials_cap = sv.partials_cap;
+    c->hist = sv.hist;
+    c->hist_cap = sv.hist_cap;
+    c->esc = sv.esc;
+    c->esc_cap = sv.esc_cap;
+    c->fftab = sv.fftab;
+    c->fftab_cap = sv.fftab_cap;
+    c->dirs = sv.dirs;
+    c->dirs_cap = sv.dirs_cap;
+    c->ndirs = sv.ndirs;
+    c->seqbuf = sv.seqbuf;
+    c->seqbuf_cap = sv.seqbuf_cap;
+    c->dn = sv.dn;
+    c->dn_dev = sv.dn_dev;
+    c->dn_cap = sv.dn_cap;
+    return rc;
 }
 
 int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, int solver, float tau, int max_step,

@@ -30,7 +30,7 @@ EXPORTS = (
     "ddafft_target_file_read", "ddafft_target_file_write", "ddafft_orientation_mc", "ddafft_rotate",
     "ddafft_target_mixture", "ddafft_agglomerate_count", "ddafft_target_agglomerate",
     "ddafft_host_alloc", "ddafft_host_free", "ddafft_set_sum_mode",
-    "ddafft_set_tuning",
+    "ddafft_set_tuning", "ddafft_solve_noalloc",
 )
 
 
@@ -68,6 +68,7 @@ def load(path=LIB_PATH):
     lib.ddafft_convolution.argtypes = [P, P, P, I]
     lib.ddafft_matvec.argtypes = [P, P, P, P, I]
     lib.ddafft_solve.argtypes = [P, I, I, F, I, F, P, P, P, I, P, P, P, P, P, P]
+    lib.ddafft_solve_noalloc.argtypes = [P, I, I, F, I, F, P, P, P, I, P, P, P, P, P, P]
     lib.ddafft_get_dipoles.argtypes = [P, I, P]
     lib.ddafft_cabs_cext.argtypes = [P, P, P, P, P, P, P]
     lib.ddafft_far_field.argtypes = [P, P, I, P, P]
@@ -188,9 +189,11 @@ class Context:
         _check(_lib.ddafft_matvec(self._h, _p(a), _p(x), _p(y), int(conj)))
         return y
 
-    def solve(self, solver, alpha, kinc, einc, dirs=None, max_step=20, err_max=1e-3, tau=0.0, ndirs=None):
+    def solve(self, solver, alpha, kinc, einc, dirs=None, max_step=20, err_max=1e-3, tau=0.0, ndirs=None,
+              noalloc=False):
         """alpha: [nrhs, nmat] (or [nmat]); einc: [nrhs, 3] (or [3]) complex.
-        Returns a dict of per-right-hand-side arrays."""
+        Returns a dict of per-right-hand-side arrays.  noalloc: through
+        ddafft_solve_noalloc (no device or page-locked allocation)."""
         solver = SOLVERS.get(solver, solver)
         alpha = np.ascontiguousarray(np.atleast_2d(alpha), np.complex64)
         einc = np.ascontiguousarray(np.atleast_2d(einc), np.complex64)
@@ -208,9 +211,9 @@ class Context:
         cext = np.zeros(nrhs, np.float32)
         hist = np.zeros((nrhs, max_step + 1), np.float64)
         info = np.zeros((nrhs, 4), np.int32)
-        _check(_lib.ddafft_solve(self._h, nrhs, int(solver), C.c_float(tau), int(max_step), C.c_float(err_max),
-                                 _p(alpha), _p(kinc), _p(einc), nd, _p(dirs), _p(esc), _p(cabs), _p(cext),
-                                 _p(hist), _p(info)))
+        fn = _lib.ddafft_solve_noalloc if noalloc else _lib.ddafft_solve
+        _check(fn(self._h, nrhs, int(solver), C.c_float(tau), int(max_step), C.c_float(err_max), _p(alpha), _p(kinc),
+                  _p(einc), nd, _p(dirs), _p(esc), _p(cabs), _p(cext), _p(hist), _p(info)))
         out = []
         for r in range(nrhs):
             steps = int(info[r, 0])

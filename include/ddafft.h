@@ -100,6 +100,20 @@ int ddafft_solve(ddafft_ctx* ctx, int nrhs, int solver, float tau, int max_step,
                  const float* alpha, const float* kinc, const float* einc, int ndirs, const float* dirs,
                  float* esc, float* cabs, float* cext, double* err_hist, int* info);
 
+/* ddafft_solve that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate memory during a solve (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as ddafft_solve, bit for bit.  Its scratch
+ * (block partials, discrepancy history, far-field partials and phase tables,
+ * per-call directions, results on their way back, reference-order sum terms)
+ * lives in a fixed 32 MiB device array, one per GPU, that every such call on
+ * that GPU shares: calls that overlap on one GPU race.  A solve whose
+ * scratch would exceed it fails with DDAFFT_EINVAL and a message giving the
+ * size needed; results are limited to 1 MiB per call.  Inputs are copied from
+ * the caller's memory (no page-locked staging is allocated). */
+int ddafft_solve_noalloc(ddafft_ctx* ctx, int nrhs, int solver, float tau, int max_step, float err_max,
+                         const float* alpha, const float* kinc, const float* einc, int ndirs, const float* dirs,
+                         float* esc, float* cabs, float* cext, double* err_hist, int* info);
 /* Dipole moments P of right-hand side `rhs` after the last solve. */
 int ddafft_get_dipoles(ddafft_ctx* ctx, int rhs, float* p);
 

@@ -213,7 +213,9 @@ inline unsigned __brev(unsigned x) {
 
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorNotReady = 600 };
-enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
+enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3,
+                     hipMemcpyDefault = 4 };
+#define __align__(n) alignas(n)
 typedef struct emul_stream* hipStream_t;
 struct emul_event {
     std::chrono::steady_clock::time_point t;
@@ -242,6 +244,11 @@ inline hipError_t hipFree(void* p) {
     return hipSuccess;
 }
 inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) { return hipMalloc(p, bytes); }
+#define HIP_SYMBOL(x) (x)
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+    *p = const_cast<void*>(sym);   // device variables are host arrays here
+    return hipSuccess;
+}
 inline hipError_t hipHostGetDevicePointer(void** d, void* h, unsigned) {
     *d = h;   // one address space here, as with page-locked memory mapped for the GPU
     return hipSuccess;

@@ -90,6 +90,7 @@ def main():
     ok &= check_batch(orc)
     ok &= check_rhs_groups(orc)
     ok &= check_reference_sums(orc)
+    ok &= check_noalloc(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -174,6 +175,30 @@ def check_rhs_groups(orc):
                                 return_p=True)
                 ok &= check(res[r]["steps"] == o["steps"] and np.array_equal(bits(ctx.dipoles(r)), bits(o["p"])),
                             "6 right-hand sides, x_blocks %d, x_rb %d, rhs %d" % (xb, rb, r))
+    return ok
+
+
+def check_noalloc(orc):
+    """ddafft_solve_noalloc: the results of ddafft_solve bit for bit, in both
+    sum modes, twice in a row (the second smaller)."""
+    ok = True
+    for dims, nrhs in (((8, 8, 8), 2), ((4, 4, 8), 1)):
+        kd = np.float32(0.6)
+        occ, _ = orc.target_ellipsoid(dims, 1.0, 1.0)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.01j)], np.complex64)
+        kinc = np.array([0, 0, 1], np.float32)
+        einc = np.eye(3, dtype=np.complex64)[:nrhs]
+        dirs = directions(np.arange(0, 181, 30))
+        for mode in (0, 1):
+            with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+                ctx.set_target(occ, [0, 1])
+                ctx.set_sum_mode(mode)
+                a = ctx.solve("bicgstab", alpha, kinc, einc, dirs=dirs, max_step=5, err_max=1e-6, noalloc=True)
+                b = ctx.solve("bicgstab", alpha, kinc, einc, dirs=dirs, max_step=5, err_max=1e-6)
+            good = all(x["steps"] == y["steps"] and np.array_equal(bits(x["esc"]), bits(y["esc"]))
+                       and np.float32(x["cext"]).tobytes() == np.float32(y["cext"]).tobytes()
+                       and np.array_equal(x["err_hist"], y["err_hist"]) for x, y in zip(a, b))
+            ok &= check(good, "no-alloc solve %s mode %d" % (dims, mode))
     return ok
 
 
