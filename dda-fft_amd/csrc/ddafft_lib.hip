@@ -228,6 +228,12 @@ struct ddafft_ctx {
     int sum_mode = 0;                  // observables: 0 FP64 sums, 1 the reference's sequential float sums
     int x_blocks = 1 << 30;            // X pass: blocks per launch up to which right-hand-side groups are spread
     int x_rb = 0;                      // X pass: right-hand sides per tile (0: 4 when they come in fours, else 2)
+    // one solver iteration captured as a HIP graph and replayed (use_graph);
+    // recaptured when anything its launches bake in changes
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<char> gkey;
     float* seqbuf = nullptr;           // terms of the sequential float sums (sum_mode 1)
     size_t seqbuf_cap = 0;
     SolveState* state_ring = nullptr;  // pinned [4][nrhs_max]: solver states after each queued iteration
@@ -416,14 +422,14 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
     {
         const int tj = std::min(tile_zf(d.nl), d.nj);
         dim3 grid(d.ni * (d.nj / tj), C);
-        DDA_SWITCH_SMALL(d.nl, hipLaunchKernelGGL((k_zf<NN>), grid, kBlock, 0, s, d, pa, st, iter, c->dipf, c->b1, c->wpre_l,
+        DDA_SWITCH_SMALL(d.nl, hipLaunchKernelGGL((k_zf<NN>), grid, kBlock, 0, s, d, pa, st, c->dipf, c->b1, c->wpre_l,
                                                                   c->tw_l));
     }
     if (ev) HIPCHECK(hipEventRecord(ev[1], s));
     {
         const int ti = std::min(tile_zf(d.nj), d.ni);
         dim3 grid(2 * d.nl * (d.ni / ti), C);
-        DDA_SWITCH_SMALL(d.nj, hipLaunchKernelGGL((k_yf<NN>), grid, kBlock, 0, s, d, st, iter, c->b1, c->b2, c->wpre_j, c->tw_j));
+        DDA_SWITCH_SMALL(d.nj, hipLaunchKernelGGL((k_yf<NN>), grid, kBlock, 0, s, d, st, c->b1, c->b2, c->wpre_j, c->tw_j));
     }
     if (ev) HIPCHECK(hipEventRecord(ev[2], s));
     {
@@ -442,7 +448,7 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
         const int gpb = (ngroups + ysets - 1) / ysets;
         dim3 grid(ntile, (ngroups + gpb - 1) / gpb);
 #define DDA_KX(RB, RUN)                                                                                   \
-    DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN, RB, RUN>), grid, kBlock, 0, s, d, st, iter, c->b2, c->b3,   \
+    DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN, RB, RUN>), grid, kBlock, 0, s, d, st, c->b2, c->b3,   \
                                               c->ahat, conj, c->wpre_i, c->tw_i, c->tw_2i, gpb))
         if (gpb > 1) {
             if (rb == 4) DDA_KX(4, true) else if (rb == 2) DDA_KX(2, true) else DDA_KX(1, true)
@@ -455,12 +461,12 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
     {
         const int tl = std::min(tile_yi(2 * d.nj), 2 * d.nl);
         dim3 grid(d.ni * (2 * d.nl / tl), C);
-        DDA_SWITCH_LARGE(2 * d.nj, hipLaunchKernelGGL((k_yi<NN>), grid, kBlock, 0, s, d, st, iter, c->b3, c->b1, c->tw_2j));
+        DDA_SWITCH_LARGE(2 * d.nj, hipLaunchKernelGGL((k_yi<NN>), grid, kBlock, 0, s, d, st, c->b3, c->b1, c->tw_2j));
     }
     if (ev) HIPCHECK(hipEventRecord(ev[4], s));
     {
         dim3 grid(zi_blocks(d), nrhs);
-        DDA_SWITCH_LARGE(2 * d.nl, hipLaunchKernelGGL((k_zi<NN>), grid, kBlock, 0, s, d, st, iter, c->b1, ea, c->occ, c->dipf,
+        DDA_SWITCH_LARGE(2 * d.nl, hipLaunchKernelGGL((k_zi<NN>), grid, kBlock, 0, s, d, st, c->b1, ea, c->occ, c->dipf,
                                                                       c->tw_2l));
     }
     if (ev) HIPCHECK(hipEventRecord(ev[5], s));
@@ -828,6 +834,8 @@ int ddafft_destroy(ddafft_ctx* c) {
     for (hipEvent_t& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->fin_ev) hipEventDestroy(c->fin_ev);
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return DDAFFT_OK;
@@ -987,6 +995,10 @@ int ddafft_set_tuning(ddafft_ctx* c, int key, int value) {
                 return fail(DDAFFT_EINVAL, "x_rb must be 0 (automatic), 1, 2 or 4");
             c->x_rb = value;
             return DDAFFT_OK;
+        case DDAFFT_TUNE_GRAPH:
+            if (value != 0 && value != 1) return fail(DDAFFT_EINVAL, "graph must be 0 or 1");
+            c->use_graph = value != 0;
+            return DDAFFT_OK;
         default:
             return fail(DDAFFT_EINVAL, "unknown tuning key");
     }
@@ -1067,11 +1079,13 @@ struct Seq {
         f.nblk = nblk;
         f.partials = c->partials;
         f.hist = c->hist;
-        // the step that closes iteration `iter` publishes the states to ring slot iter & 3
-        f.ring = ends_step(kind) ? c->ring_dev + (size_t)(iter & 3) * c->nrhs_max : nullptr;
+        // the step that closes an iteration publishes the states to the ring
+        // (slot: the iteration & 3, read on the device)
+        f.ring = ends_step(kind) ? c->ring_dev : nullptr;
+        f.ring_stride = c->nrhs_max;
         if (!f.partials || !f.hist || (ends_step(kind) && !c->ring_dev))
             return fail(DDAFFT_ESTATE, "solver scratch not allocated");
-        hipLaunchKernelGGL(k_finalize, nrhs, kBlock, 0, c->stream, c->st, iter, f);
+        hipLaunchKernelGGL(k_finalize, nrhs, kBlock, 0, c->stream, c->st, f);
         HIPCHECK(hipGetLastError());
         return DDAFFT_OK;
     }
@@ -1092,7 +1106,7 @@ struct Seq {
         if (!ua.P || !ua.R || !ua.S || !ua.W || !ua.partials || (ua.kind == UK_BICG && !ua.G))
             return fail(DDAFFT_ESTATE, "internal: update launched with a missing buffer");
         dim3 g(update_blocks(d), nrhs);
-        hipLaunchKernelGGL(k_update, g, kBlock, 0, c->stream, d, c->st, iter, ua, c->dipf);
+        hipLaunchKernelGGL(k_update, g, kBlock, 0, c->stream, d, c->st, ua, c->dipf);
         HIPCHECK(hipGetLastError());
         return fin_kind >= 0 ? fin(fin_kind, iter, update_blocks(d)) : DDAFFT_OK;
     }
@@ -1430,12 +1444,84 @@ int run_begin(Run& r) {
 
 bool run_more(const Run& r) { return r.o <= r.max_step && r.o < r.known_stop; }
 
+// Everything the launches of one captured iteration bake in: if any of it
+// changes, the graph is captured again.
+void graph_key(const Run& r, std::vector<char>& out) {
+    const ddafft_ctx* c = r.c;
+    struct {
+        int solver, nrhs, max_step, x_rb, x_blocks, nmat;
+        float tau;
+        double err_max, inv_ndip;
+        const void *hist, *partials, *alpha, *ring, *mv;
+    } k;
+    std::memset(&k, 0, sizeof(k));
+    k.solver = r.solver;
+    k.nrhs = r.nrhs;
+    k.max_step = r.max_step;
+    k.x_rb = c->x_rb;
+    k.x_blocks = c->x_blocks;
+    k.nmat = c->nmat;
+    k.tau = r.tau;
+    k.err_max = r.q.fa.err_max;
+    k.inv_ndip = r.q.fa.inv_ndip;
+    k.hist = c->hist;
+    k.partials = c->partials;
+    k.alpha = c->alpha;
+    k.ring = c->ring_dev;
+    k.mv = c->mv;
+    const char* b = reinterpret_cast<const char*>(&k);
+    out.assign(b, b + sizeof(k));
+}
+
+// Enqueues one steady-state iteration as a replay of its captured graph
+// (captured on first use: the kernels take their iteration from the solver
+// states, so the same launches serve every iteration).
+int graph_iteration(Run& r) {
+    ddafft_ctx* c = r.c;
+    std::vector<char> key;
+    graph_key(r, key);
+    if (!c->gexec || key != c->gkey) {
+        if (c->gexec) HIPCHECK(hipGraphExecDestroy(c->gexec));
+        if (c->graph) HIPCHECK(hipGraphDestroy(c->graph));
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        TRY(flush_small(c));   // nothing staged may be copied from inside the graph
+        HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = iteration(r.q, r.solver, r.o, r.tau);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHECK(e);
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            hipGraphDestroy(g);
+            return fail(DDAFFT_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        c->graph = g;
+        c->gexec = x;
+        c->gkey.swap(key);
+    }
+    HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    return DDAFFT_OK;
+}
+
 // enqueues iteration r.o; its last scalar step publishes the states (the
-// stopping decisions) to ring slot o & 3, which the event marks as written
+// stopping decisions) to ring slot o & 3, which the event marks as written.
+// Iterations of one form run as graph replays; CG's first iteration has a
+// form of its own, profiling records events per pass, and the
+// allocation-free solve does not instantiate graphs: those run eagerly.
 int run_enqueue(Run& r) {
     ddafft_ctx* c = r.c;
     const int o = r.o;
-    TRY(iteration(r.q, r.solver, o, r.tau));
+    const bool steady = !(r.solver == DDAFFT_SOLVER_CG && o == 1);
+    if (c->use_graph && steady && !c->prof && !t_noalloc)
+        TRY(graph_iteration(r));
+    else
+        TRY(iteration(r.q, r.solver, o, r.tau));
     HIPCHECK(hipEventRecord(c->ev[o & 3], c->stream));
     r.o = o + 1;
     return DDAFFT_OK;

@@ -36,7 +36,9 @@ struct SolveState {
     int stop_at;            // iterations o >= stop_at do not run
     int snapshot;           // copy P into P_min at the next prologue
     int restored;           // P was replaced by P_min at the end
-    int pad1;
+    int it;                 // iteration the solve is in (0: step 0); the scalar step that closes an
+                            // iteration advances it, so the kernels of a replayed iteration read their
+                            // iteration from here rather than from a launch argument
 };
 
 // Host-visible summary of a finished solve.

@@ -33,16 +33,18 @@ struct FinArgs {
     int max_step;
     double err_max;
     double inv_ndip;       // 1 / number of dipoles (ldError's n)
-    SolveState* ring;      // steps that end an iteration: the host-visible copy of the states
-                           // ([nrhs], page-locked host memory), else nullptr
+    SolveState* ring;      // steps that end an iteration: the host-visible copies of the states,
+                           // [4][ring_stride] in page-locked host memory (slot: iteration & 3), else nullptr
+    int ring_stride;
 };
 
-// Publishes one right-hand side's state to the host-visible ring.  The host
-// reads it only after the event recorded behind the iteration has completed,
-// which orders the kernel's stores to host memory before it (no fence here:
-// a system-scope fence writes back the whole L2).
-__device__ __forceinline__ void publish_state(const FinArgs& fa, int rhs, const SolveState& s) {
-    if (fa.ring) fa.ring[rhs] = s;
+// Publishes one right-hand side's state after iteration `iter` to the
+// host-visible ring.  The host reads it only after the event recorded behind
+// the iteration has completed, which orders the kernel's stores to host
+// memory before it (no fence here: a system-scope fence writes back the
+// whole L2).
+__device__ __forceinline__ void publish_state(const FinArgs& fa, int rhs, int iter, const SolveState& s) {
+    if (fa.ring) fa.ring[(size_t)(iter & 3) * fa.ring_stride + rhs] = s;
 }
 
 // End-of-step kinds: the scalar step that closes an iteration (or step 0).
@@ -177,7 +179,10 @@ __device__ __forceinline__ void finalize_block(SolveState* __restrict__ st, int 
         default:
             break;
     }
-    publish_state(fa, rhs, s);
+    if (ends_step(fa.kind)) {
+        s.it = iter + 1;
+        publish_state(fa, rhs, iter, s);
+    }
 }
 
 }  // namespace ddafft

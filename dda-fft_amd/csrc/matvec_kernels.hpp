@@ -109,7 +109,7 @@ struct ProArgs {
 // ---------------------------------------------------------------------------
 
 template <int NL>
-__global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveState* __restrict__ st, int iter,
+__global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveState* __restrict__ st,
                                                const int8_t* __restrict__ dipf, float2* __restrict__ b1,
                                                const float2* __restrict__ wpre, const float2* __restrict__ tw) {
     constexpr int TJ = TileZF<NL>::T;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
     float2 c1 = make_float2(0.f, 0.f), c2 = c1;
     int snap = 0;
     if (st) {
-        if (iter >= st[rhs].stop_at) return;
+        if (st[rhs].it >= st[rhs].stop_at) return;
         c1 = st[rhs].pro_c1;
         c2 = st[rhs].pro_c2;
         // only the first matvec of an iteration carries the P -> P_min copy
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
 // ---------------------------------------------------------------------------
 
 template <int NJ>
-__global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restrict__ st, int iter,
+__global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restrict__ st,
                                                const float2* __restrict__ b1, float2* __restrict__ b2,
                                                const float2* __restrict__ wpre, const float2* __restrict__ tw) {
     constexpr int TI = TileZF<NJ>::T;
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
     __shared__ float2 s[2 * TI * LS];
     __shared__ float2 stw[NJ];
     const int c = blockIdx.y, rhs = c / 3;
-    if (st && iter >= st[rhs].stop_at) return;
+    if (st && st[rhs].it >= st[rhs].stop_at) return;
     constexpr int LNJ = ilog2c(NJ);
     const int NL2 = 2 * d.nl;
     // the tile body, with the tile width a compile-time constant for full
@@ -287,11 +287,11 @@ __device__ __forceinline__ void ahat_apply(const float2* a, float2 x, float2 y, 
 
 // Right-hand sides [rhs0, rhs1) that exist and are still running, as bits
 // from rhs0 (at most 64 right-hand sides per context).
-__device__ __forceinline__ unsigned long long x_active(const Dims& d, const SolveState* __restrict__ st, int iter,
-                                                       int rhs0, int rhs1) {
+__device__ __forceinline__ unsigned long long x_active(const Dims& d, const SolveState* __restrict__ st, int rhs0,
+                                                       int rhs1) {
     unsigned long long m = 0;
     for (int r = rhs0; r < rhs1 && r < d.nrhs; ++r)
-        if (!(st && iter >= st[r].stop_at)) m |= 1ull << (r - rhs0);
+        if (!(st && st[r].it >= st[r].stop_at)) m |= 1ull << (r - rhs0);
     return m;
 }
 
@@ -451,7 +451,7 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
 // groups of RB (RUN; otherwise gpb = 1); each block reads its tile's Â once
 // for all of its groups.
 template <int NI, int RB, bool RUN>
-__global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st, int iter,
+__global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st,
                                               const float2* __restrict__ b2, float2* __restrict__ b3,
                                               const float2* __restrict__ ahat, int conj,
                                               const float2* __restrict__ wpre, const float2* __restrict__ tw1,
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     const int g0 = blockIdx.y * gpb, g1 = RUN ? min(g0 + gpb, ngroups) : g0 + 1;
     // which right-hand sides of the run are running, read before any other
     // load is issued (so that no later wait on them waits for Â as well)
-    const unsigned long long amask = x_active(d, st, iter, g0 * RB, g1 * RB);
+    const unsigned long long amask = x_active(d, st, g0 * RB, g1 * RB);
     if (!amask) return;
     // Measured on MI355X (2 RHS), against a run-time tile width: n_i = 64
     // 57.8 -> 56.3 us per launch, n_i = 128 612 -> 534 us, n_i = 256
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
 // ---------------------------------------------------------------------------
 
 template <int NJ2>
-__global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restrict__ st, int iter,
+__global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restrict__ st,
                                                const float2* __restrict__ b3, float2* __restrict__ b4,
                                                const float2* __restrict__ tw) {
     constexpr int TL = TileYI<NJ2>::T;
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
     __shared__ float2 s[TL * LS];
     __shared__ float2 stw[NJ2];
     const int c = blockIdx.y, rhs = c / 3;
-    if (st && iter >= st[rhs].stop_at) return;
+    if (st && st[rhs].it >= st[rhs].stop_at) return;
     const int NL2 = 2 * d.nl;
     // the tile body, with the tile width a compile-time constant for full
     // tiles and a run-time one otherwise
@@ -582,7 +582,7 @@ __device__ __forceinline__ void dot_acc(float2 a, float2 b, double& re, double& 
 __device__ __forceinline__ double norm_term(float2 a) { return (double)(a.x * a.x + a.y * a.y); }
 
 template <int NL2>
-__global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ st, int iter,
+__global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ st,
                                                const float2* __restrict__ b4, EpiArgs ea,
                                                const int8_t* __restrict__ occ, const int8_t* __restrict__ dipf,
                                                const float2* __restrict__ tw) {
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ 
     const int rhs = blockIdx.y;
     float rB = 0.f;
     if (st) {
-        if (iter >= st[rhs].stop_at) return;
+        if (st[rhs].it >= st[rhs].stop_at) return;
         rB = st[rhs].rB;
     }
     // the tile body, with the row count a compile-time constant for full tiles

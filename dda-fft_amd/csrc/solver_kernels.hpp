@@ -106,11 +106,11 @@ struct UpdArgs {
     double* partials;  // [nrhs][gridDim.x][kSlots]
 };
 
-__global__ __launch_bounds__(kBlock) void k_update(Dims d, SolveState* __restrict__ st, int iter, UpdArgs ua,
+__global__ __launch_bounds__(kBlock) void k_update(Dims d, SolveState* __restrict__ st, UpdArgs ua,
                                                    const int8_t* __restrict__ dipf) {
     __shared__ double red[kSlots * 16];
     const int rhs = blockIdx.y;
-    if (iter >= st[rhs].stop_at) return;
+    if (st[rhs].it >= st[rhs].stop_at) return;
     const float2 ncA = cneg(st[rhs].cA), ncW = cneg(st[rhs].cW);
     const float rA = st[rhs].rA;
     const long long rb = (long long)rhs * 3 * d.n;
@@ -162,11 +162,15 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, SolveState* __restric
 
 // Scalar steps: the fixed-order sum of the block partials and the solver
 // state update of one right-hand side (one block per right-hand side).
-__global__ __launch_bounds__(kBlock) void k_finalize(SolveState* __restrict__ st, int iter, FinArgs fa) {
+__global__ __launch_bounds__(kBlock) void k_finalize(SolveState* __restrict__ st, FinArgs fa) {
     __shared__ double red[kSlots * 16];
     const int rhs = blockIdx.x;
-    if (iter >= st[rhs].stop_at) {   // stopped earlier: its state still goes out
-        if (threadIdx.x == 0) publish_state(fa, rhs, st[rhs]);
+    const int iter = st[rhs].it;
+    if (iter >= st[rhs].stop_at) {   // stopped earlier: its iteration still advances, its state still goes out
+        if (threadIdx.x == 0 && ends_step(fa.kind)) {
+            st[rhs].it = iter + 1;
+            publish_state(fa, rhs, iter, st[rhs]);
+        }
         return;
     }
     finalize_block(st, iter, fa, rhs, red);

@@ -162,14 +162,17 @@ class Context:
         sequential float sums in its cell order (ddafft_set_sum_mode)."""
         _check(_lib.ddafft_set_sum_mode(self._h, int(mode)))
 
-    def set_tuning(self, x_blocks=None, x_rb=None):
+    def set_tuning(self, x_blocks=None, x_rb=None, graph=None):
         """Launch schedule (ddafft_set_tuning): x_blocks, the X-pass launch
         size up to which right-hand-side groups are spread over blocks;
-        x_rb, right-hand sides per X tile (0 automatic, 1, 2 or 4)."""
+        x_rb, right-hand sides per X tile (0 automatic, 1, 2 or 4); graph,
+        iterations replayed as captured HIP graphs (True) or launched."""
         if x_blocks is not None:
             _check(_lib.ddafft_set_tuning(self._h, 0, int(x_blocks)))
         if x_rb is not None:
             _check(_lib.ddafft_set_tuning(self._h, 1, int(x_rb)))
+        if graph is not None:
+            _check(_lib.ddafft_set_tuning(self._h, 2, int(bool(graph))))
 
     def set_directions(self, dirs):
         dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)

@@ -143,6 +143,10 @@ int ddafft_set_sum_mode(ddafft_ctx* ctx, int mode);
  * launch has X_BLOCKS blocks (default: every group its own block). */
 #define DDAFFT_TUNE_X_BLOCKS 0
 #define DDAFFT_TUNE_X_RB 1
+/* DDAFFT_TUNE_GRAPH: 1 (default) replays each solver iteration as a captured
+ * HIP graph (its kernels read their iteration from the device-resident
+ * solver states), 0 launches every kernel of every iteration. */
+#define DDAFFT_TUNE_GRAPH 2
 int ddafft_set_tuning(ddafft_ctx* ctx, int key, int value);
 /* One instance's solve for ddafft_solve_ex / ddafft_solve_batch: nrhs
  * right-hand sides, each with its own incidence (Monte-Carlo orientations x

@@ -136,8 +136,24 @@ inline void resume(int t) {
 #endif
 }
 
+// Stream capture: while a capture is open, launches are recorded (by value)
+// instead of run; a graph launch runs them in order.
+inline std::vector<std::function<void()>>* g_capture = nullptr;
+
+template <class F, class... A>
+void launch_now(F f, dim3 g, dim3 b, A... args);
+
 template <class F, class... A>
 void launch(F f, dim3 g, dim3 b, A... args) {
+    if (g_capture) {
+        g_capture->push_back([=]() { launch_now(f, g, b, args...); });
+        return;
+    }
+    launch_now(f, g, b, args...);
+}
+
+template <class F, class... A>
+void launch_now(F f, dim3 g, dim3 b, A... args) {
     static std::mutex serial;   // one kernel at a time, blocks in order
     std::lock_guard<std::mutex> lk(serial);
     const int nt = (int)(b.x * b.y * b.z);
@@ -259,7 +275,40 @@ inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
     return hipSuccess;
 }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
+    if (emul::g_capture) {
+        emul::g_capture->push_back([=]() { hipMemcpy(d, s, n, k); });
+        return hipSuccess;
+    }
     return hipMemcpy(d, s, n, k);
+}
+struct emul_graph {
+    std::vector<std::function<void()>> ops;
+};
+typedef emul_graph* hipGraph_t;
+typedef emul_graph* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1 };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    emul::g_capture = new std::vector<std::function<void()>>();
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    *g = new emul_graph{std::move(*emul::g_capture)};
+    delete emul::g_capture;
+    emul::g_capture = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t g, void*, void*, size_t) {
+    *x = g;
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t) {
+    for (auto& op : x->ops) op();
+    return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+    delete g;
+    return hipSuccess;
 }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
     *s = reinterpret_cast<hipStream_t>(new char[1]);

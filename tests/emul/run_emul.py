@@ -91,6 +91,7 @@ def main():
     ok &= check_rhs_groups(orc)
     ok &= check_reference_sums(orc)
     ok &= check_noalloc(orc)
+    ok &= check_graph(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -175,6 +176,31 @@ def check_rhs_groups(orc):
                                 return_p=True)
                 ok &= check(res[r]["steps"] == o["steps"] and np.array_equal(bits(ctx.dipoles(r)), bits(o["p"])),
                             "6 right-hand sides, x_blocks %d, x_rb %d, rhs %d" % (xb, rb, r))
+    return ok
+
+
+def check_graph(orc):
+    """Iterations replayed as captured graphs against launched ones: the same
+    bits for every solver."""
+    ok = True
+    dims, kd = (8, 4, 8), np.float32(0.5)
+    occ, _ = orc.target_ellipsoid(dims, 0.5, 1.0)
+    alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.01j)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = directions(np.arange(0, 181, 45))
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, [0, 1])
+        for solver in (0, 1, 2, 3):
+            out = []
+            for graph in (False, True, True):
+                ctx.set_tuning(graph=graph)
+                res = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=6, err_max=1e-9, tau=0.5)
+                out.append((res, [ctx.dipoles(r) for r in range(2)]))
+            good = all(a["steps"] == b["steps"] and np.array_equal(bits(a["esc"]), bits(b["esc"]))
+                       and np.array_equal(a["err_hist"], b["err_hist"]) and np.array_equal(bits(pa), bits(pb))
+                       for k in (1, 2) for a, b, pa, pb in zip(out[0][0], out[k][0], out[0][1], out[k][1]))
+            ok &= check(good, "graph replays, solver %d" % solver)
     return ok
 
 

@@ -236,3 +236,42 @@ def test_reference_sums_golden(orc32, tag, solvers):
                 assert bits(np.float32(got["cext"])) == bits(np.float32(g[key + "_cext"])), key
                 assert bits(np.float32(got["cabs"])) == bits(np.float32(g[key + "_cabs"])), key
                 assert e < 2e-6, key
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+def test_graph_replays_match_launches(orc32, solver):
+    """Every iteration replayed from its captured HIP graph gives the bits of
+    the launched iterations: three solves with graphs, each after eager work
+    (a launched solve, a matvec) and a synchronization, each checked against
+    the launched solve, and against the oracle's dipoles."""
+    dims, x, m, tol, steps = SOLVE_CASES[0]
+    kd = np.float32(2 * x / dims[0])
+    occ = target(orc32, dims)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, m)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = directions(np.arange(0, 181, 10))
+    rng = np.random.default_rng(9)
+    print("graph replays, solver", solver, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        ctx.set_tuning(graph=False)
+        eager = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=steps, err_max=tol, tau=0.5)
+        P0 = [ctx.dipoles(r) for r in range(2)]
+        A = ctx.ahat()
+        for rep in range(3):
+            ctx.set_tuning(graph=False)
+            ctx.solve(solver, alpha, kinc, einc[::-1].copy(), dirs=dirs, max_step=3, err_max=tol, tau=0.5)
+            ctx.matvec(alpha, (rng.standard_normal(dims + (3,)) + 0j).astype(np.complex64))
+            ctx.set_tuning(graph=True)
+            got = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=steps, err_max=tol, tau=0.5)
+            P = [ctx.dipoles(r) for r in range(2)]
+            for r in range(2):
+                assert got[r]["steps"] == eager[r]["steps"] and got[r]["step_min"] == eager[r]["step_min"], rep
+                assert np.array_equal(got[r]["err_hist"], eager[r]["err_hist"]), rep
+                assert np.array_equal(bits(got[r]["esc"]), bits(eager[r]["esc"])), rep
+                assert np.array_equal(bits(P[r]), bits(P0[r])), rep
+    ref = orc32.dda_fft(solver, occ, mv, alpha, kd, kinc, einc[0], A, dirs, max_step=steps, err_max=tol, tau=0.5,
+                        return_p=True)
+    assert np.array_equal(bits(P0[0]), bits(ref["p"]))
