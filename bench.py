@@ -287,7 +287,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-instance form")
     ap.add_argument("--batched-frac", type=float, default=1.0,
                     help="fraction of each lattice group of the 128-realization batched workload")
-    ap.add_argument("--batched-width", type=int, default=8, help="realizations in flight in the batch driver")
+    ap.add_argument("--batched-width", type=int, default=0,
+                    help="realizations in flight in the batch driver (0: the autotuner's choice)")
     ap.add_argument("--no-gradient", action="store_true", help="skip the forward+adjoint form")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 single-target solve")
     ap.add_argument("--matvec-scaling", action="store_true",

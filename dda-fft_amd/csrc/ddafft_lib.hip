@@ -21,6 +21,7 @@
 #include "gradient_kernels.hpp"
 #include "matvec_kernels.hpp"
 #include "solver_kernels.hpp"
+#include "tuned_table.hpp"
 
 using namespace ddafft;
 
@@ -48,7 +49,11 @@ int fail(int code, const std::string& msg) {
 
 const double kPi = 3.1415926535897932384626433832795;
 
-bool pow2_ok(int n) { return n >= 4 && n <= 256 && (n & (n - 1)) == 0; }
+bool pow2_ok(int n) { return n >= 4 && n <= 512 && (n & (n - 1)) == 0; }
+
+// Cells of a lattice at most 2^24: the passes address Â (6 complex values
+// per padded cell, 8 N cells) with 32-bit offsets.
+constexpr long long kMaxCells = 1LL << 24;
 
 // FourierPhase (fft_func.cpp:77-88): W_2n^k for k < n by a float recurrence.
 std::vector<float2> fourier_phase(int n) {
@@ -226,11 +231,13 @@ struct ddafft_ctx {
     float2* fftab = nullptr;           // far-field per-axis phases [ndirs][ni + nj + nl]
     size_t fftab_cap = 0;
     int sum_mode = 0;                  // observables: 0 FP64 sums, 1 the reference's sequential float sums
-    int x_blocks = 1 << 30;            // X pass: blocks per launch up to which right-hand-side groups are spread
-    int x_rb = 0;                      // X pass: right-hand sides per tile (0: 4 when they come in fours, else 2)
-    // one solver iteration captured as a HIP graph and replayed (use_graph);
-    // recaptured when anything its launches bake in changes
-    bool use_graph = true;
+    // launch schedule, 0 / -1: from the tuned table, else the built-in rule
+    int x_blocks = 0;                  // X pass: blocks per launch up to which right-hand-side groups are spread
+    int x_rb = 0;                      // X pass: right-hand sides per tile
+    // one solver iteration captured as a HIP graph and replayed (use_graph:
+    // 1 yes, 0 no, -1 the tuned choice); recaptured when anything its
+    // launches bake in changes
+    int use_graph = -1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<char> gkey;
@@ -272,6 +279,30 @@ namespace {
 int flush_small(ddafft_ctx* c);
 int inc_tab_len(const ddafft_ctx* c);
 
+// The launch schedule of one matvec / iteration: the tuned table's entry for
+// this lattice and right-hand-side count (tuned_table.hpp, written by
+// dda-fft_amd/tune.py on an MI355X), else the built-in rule; a value set
+// through ddafft_set_tuning wins over both.
+struct Schedule {
+    int x_rb, x_blocks, graph;
+};
+
+Schedule schedule(const ddafft_ctx* c, int nrhs) {
+    // built-in rule: four right-hand sides per X tile when they come in fours
+    // on lines of at most 32 points (32^3, 16 RHS: 41 us against 44.5 in
+    // pairs; 64^3: 425 against 315), every group its own block, graphs on
+    Schedule s{(nrhs % 4 == 0 && c->d.ni <= 32) ? 4 : 2, 1 << 30, 1};
+    for (const TunedEntry& e : kTuned)
+        if (e.ni == c->d.ni && e.nj == c->d.nj && e.nl == c->d.nl && e.nrhs == nrhs) {
+            s = Schedule{e.x_rb, e.x_blocks, e.graph};
+            break;
+        }
+    if (c->x_rb > 0) s.x_rb = c->x_rb;
+    if (c->x_blocks > 0) s.x_blocks = c->x_blocks;
+    if (c->use_graph >= 0) s.graph = c->use_graph;
+    return s;
+}
+
 int ensure_partials(ddafft_ctx* c, size_t doubles) {
     if (doubles <= c->partials_cap) return DDAFFT_OK;
     if (c->stream) HIPCHECK(hipStreamSynchronize(c->stream));
@@ -294,6 +325,7 @@ int tile_zf(int n) {
         case 64: return TileZF<64>::T;
         case 128: return TileZF<128>::T;
         case 256: return TileZF<256>::T;
+        case 512: return TileZF<512>::T;
     }
     return 1;
 }
@@ -307,6 +339,7 @@ int tile_x_rb(int n) {
         case 64: return TileX<64, RB>::T;
         case 128: return TileX<128, RB>::T;
         case 256: return TileX<256, RB>::T;
+        case 512: return TileX<512, 1>::T;
     }
     return 1;
 }
@@ -320,6 +353,7 @@ int tile_yi(int n2) {
         case 128: return TileYI<128>::T;
         case 256: return TileYI<256>::T;
         case 512: return TileYI<512>::T;
+        case 1024: return TileYI<1024>::T;
     }
     return 1;
 }
@@ -332,6 +366,7 @@ int tile_zi(int n2) {
         case 128: return TileZI<128>::T;
         case 256: return TileZI<256>::T;
         case 512: return TileZI<512>::T;
+        case 1024: return TileZI<1024>::T;
     }
     return 1;
 }
@@ -347,7 +382,18 @@ int zi_blocks(const Dims& d) {
         __VA_ARGS__;          \
     } break;
 
+// line lengths of the forward passes (input lines of n points)
 #define DDA_SWITCH_SMALL(n, ...)                                                                        \
+    switch (n) {                                                                                         \
+        DDA_CASE_N(4, __VA_ARGS__) DDA_CASE_N(8, __VA_ARGS__) DDA_CASE_N(16, __VA_ARGS__)                \
+        DDA_CASE_N(32, __VA_ARGS__) DDA_CASE_N(64, __VA_ARGS__) DDA_CASE_N(128, __VA_ARGS__)             \
+        DDA_CASE_N(256, __VA_ARGS__) DDA_CASE_N(512, __VA_ARGS__)                                        \
+        default: return fail(DDAFFT_EINVAL, "unsupported line length");                                 \
+    }
+
+// X-pass tiles of two or four right-hand sides: lines up to 256 points (a
+// 512-point axis takes one right-hand side per tile, whose LDS fits)
+#define DDA_SWITCH_X(n, ...)                                                                            \
     switch (n) {                                                                                         \
         DDA_CASE_N(4, __VA_ARGS__) DDA_CASE_N(8, __VA_ARGS__) DDA_CASE_N(16, __VA_ARGS__)                \
         DDA_CASE_N(32, __VA_ARGS__) DDA_CASE_N(64, __VA_ARGS__) DDA_CASE_N(128, __VA_ARGS__)             \
@@ -355,11 +401,12 @@ int zi_blocks(const Dims& d) {
         default: return fail(DDAFFT_EINVAL, "unsupported line length");                                 \
     }
 
+// line lengths of the inverse passes and of the A-hat transform (2 n points)
 #define DDA_SWITCH_LARGE(n, ...)                                                                        \
     switch (n) {                                                                                         \
         DDA_CASE_N(8, __VA_ARGS__) DDA_CASE_N(16, __VA_ARGS__) DDA_CASE_N(32, __VA_ARGS__)               \
         DDA_CASE_N(64, __VA_ARGS__) DDA_CASE_N(128, __VA_ARGS__) DDA_CASE_N(256, __VA_ARGS__)            \
-        DDA_CASE_N(512, __VA_ARGS__)                                                                     \
+        DDA_CASE_N(512, __VA_ARGS__) DDA_CASE_N(1024, __VA_ARGS__)                                       \
         default: return fail(DDAFFT_EINVAL, "unsupported line length");                                 \
     }
 
@@ -434,26 +481,29 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
     if (ev) HIPCHECK(hipEventRecord(ev[2], s));
     {
         // right-hand sides go through the X pass in groups of rb that share
-        // each Â value (4 when they come in fours on lines of at most 32
-        // points -- measured: 32^3, 16 RHS 41 us against 44.5 in pairs; 64^3
-        // 425 against 315 -- else 2, or the tuned choice); a block takes
-        // `gpb` consecutive groups and reads its
+        // each Â value; a block takes `gpb` consecutive groups and reads its
         // tile's Â once for all of them, as many groups as keep about
         // x_blocks blocks in the launch
-        int rb = c->x_rb > 0 ? c->x_rb : ((nrhs % 4 == 0 && d.ni <= 32) ? 4 : 2);
+        const Schedule sch = schedule(c, nrhs);
+        int rb = sch.x_rb;
         while (rb > 1 && rb > nrhs) rb >>= 1;
+        if (d.ni > 256) rb = 1;
         const int tj = std::min(tile_x(d.ni, rb), 2 * d.nj);
         const int ntile = 2 * d.nl * (2 * d.nj / tj), ngroups = (nrhs + rb - 1) / rb;
-        const int ysets = std::max(1, std::min(ngroups, (c->x_blocks + ntile - 1) / ntile));
+        const int ysets = std::max(1, std::min(ngroups, (int)std::min<long long>(((long long)sch.x_blocks + ntile - 1) / ntile, 1 << 20)));
         const int gpb = (ngroups + ysets - 1) / ysets;
         dim3 grid(ntile, (ngroups + gpb - 1) / gpb);
-#define DDA_KX(RB, RUN)                                                                                   \
-    DDA_SWITCH_SMALL(d.ni, hipLaunchKernelGGL((k_x<NN, RB, RUN>), grid, kBlock, 0, s, d, st, c->b2, c->b3,   \
-                                              c->ahat, conj, c->wpre_i, c->tw_i, c->tw_2i, gpb))
+#define DDA_KX(SW, RB, RUN)                                                                               \
+    SW(d.ni, hipLaunchKernelGGL((k_x<NN, RB, RUN>), grid, kBlock, 0, s, d, st, c->b2, c->b3, c->ahat, conj,     \
+                                c->wpre_i, c->tw_i, c->tw_2i, gpb))
         if (gpb > 1) {
-            if (rb == 4) DDA_KX(4, true) else if (rb == 2) DDA_KX(2, true) else DDA_KX(1, true)
+            if (rb == 4) DDA_KX(DDA_SWITCH_X, 4, true)
+            else if (rb == 2) DDA_KX(DDA_SWITCH_X, 2, true)
+            else DDA_KX(DDA_SWITCH_SMALL, 1, true)
         } else {
-            if (rb == 4) DDA_KX(4, false) else if (rb == 2) DDA_KX(2, false) else DDA_KX(1, false)
+            if (rb == 4) DDA_KX(DDA_SWITCH_X, 4, false)
+            else if (rb == 2) DDA_KX(DDA_SWITCH_X, 2, false)
+            else DDA_KX(DDA_SWITCH_SMALL, 1, false)
         }
 #undef DDA_KX
     }
@@ -512,6 +562,7 @@ int build_ahat(ddafft_ctx* c) {
                 DDA_CASE_N(128, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
                 DDA_CASE_N(256, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
                 DDA_CASE_N(512, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
+                DDA_CASE_N(1024, tt = std::min(clamp16(pow2floor(8192 / (6 * (NN + kLpad)))), ext))
                 default: return fail(DDAFFT_EINVAL, "unsupported axis length");
             }
             dim3 g(outer * (ext / tt));
@@ -845,7 +896,9 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
     if (!out) return fail(DDAFFT_EINVAL, "out is NULL");
     *out = nullptr;
     if (!pow2_ok(ni) || !pow2_ok(nj) || !pow2_ok(nl))
-        return fail(DDAFFT_EINVAL, "lattice dimensions must be powers of two in [4, 256]");
+        return fail(DDAFFT_EINVAL, "lattice dimensions must be powers of two in [4, 512]");
+    if ((long long)ni * nj * nl > kMaxCells)
+        return fail(DDAFFT_EINVAL, "lattice of more than 2^24 cells (e.g. 256^3 or 512x256x128)");
     if (!(kd > 0.f) || nrhs_max < 1 || nrhs_max > 64) return fail(DDAFFT_EINVAL, "bad kd or nrhs_max");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(DDAFFT_ENODEV, "no HIP device");
@@ -987,7 +1040,7 @@ int ddafft_set_tuning(ddafft_ctx* c, int key, int value) {
     if (!c) return fail(DDAFFT_EINVAL, "NULL context");
     switch (key) {
         case DDAFFT_TUNE_X_BLOCKS:
-            if (value < 1) return fail(DDAFFT_EINVAL, "x_blocks must be positive");
+            if (value < 0) return fail(DDAFFT_EINVAL, "x_blocks must be 0 (tuned) or positive");
             c->x_blocks = value;
             return DDAFFT_OK;
         case DDAFFT_TUNE_X_RB:
@@ -996,8 +1049,8 @@ int ddafft_set_tuning(ddafft_ctx* c, int key, int value) {
             c->x_rb = value;
             return DDAFFT_OK;
         case DDAFFT_TUNE_GRAPH:
-            if (value != 0 && value != 1) return fail(DDAFFT_EINVAL, "graph must be 0 or 1");
-            c->use_graph = value != 0;
+            if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "graph must be -1 (tuned), 0 or 1");
+            c->use_graph = value;
             return DDAFFT_OK;
         default:
             return fail(DDAFFT_EINVAL, "unknown tuning key");
@@ -1286,12 +1339,21 @@ int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_s
                 HIPCHECK(hipGetLastError());
                 tab_dirs = dd;
             }
-            if (pair)
-                hipLaunchKernelGGL((k_far_field<2>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells, c->ncells,
-                                   ndirs, r0, c->P, c->partials);
+            // phase rows in LDS sized for lattices up to 256 per axis, or the
+            // larger form for 512-point axes
+            const bool big = d.ni + d.nj + d.nl > kFFAxis;
+            if (pair && !big)
+                hipLaunchKernelGGL((k_far_field<2, kFFAxis>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells,
+                                   c->ncells, ndirs, r0, c->P, c->partials);
+            else if (pair)
+                hipLaunchKernelGGL((k_far_field<2, kFFAxisMax>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells,
+                                   c->ncells, ndirs, r0, c->P, c->partials);
+            else if (!big)
+                hipLaunchKernelGGL((k_far_field<1, kFFAxis>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells,
+                                   c->ncells, ndirs, r0, c->P, c->partials);
             else
-                hipLaunchKernelGGL((k_far_field<1>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells, c->ncells,
-                                   ndirs, r0, c->P, c->partials);
+                hipLaunchKernelGGL((k_far_field<1, kFFAxisMax>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells,
+                                   c->ncells, ndirs, r0, c->P, c->partials);
             HIPCHECK(hipGetLastError());
             r0 += pair ? 2 : 1;
         }
@@ -1518,7 +1580,7 @@ int run_enqueue(Run& r) {
     ddafft_ctx* c = r.c;
     const int o = r.o;
     const bool steady = !(r.solver == DDAFFT_SOLVER_CG && o == 1);
-    if (c->use_graph && steady && !c->prof && !t_noalloc)
+    if (schedule(c, r.nrhs).graph && steady && !c->prof && !t_noalloc)
         TRY(graph_iteration(r));
     else
         TRY(iteration(r.q, r.solver, o, r.tau));
@@ -1845,7 +1907,7 @@ int ddafft_solve_batch(int n, ddafft_ctx* const* ctxs, const ddafft_job* jobs, i
     for (int k = 0; k < n; ++k)
         for (int m = 0; m < k; ++m)
             if (ctxs[k] == ctxs[m]) return fail(DDAFFT_EINVAL, "a context appears twice in one batch");
-    if (width < 1) width = 4;
+    if (width < 1) width = kTunedBatchWidth;   // the measured choice
     width = std::min(width, n);
     struct Slot {
         Run r;

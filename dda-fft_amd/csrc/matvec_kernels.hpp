@@ -48,7 +48,8 @@ template <int N> struct TileZF { static constexpr int T = clamp16(pow2floor(6144
 // tile's LDS and its transforms stay those of the two-right-hand-side tile
 // while each Â value serves twice the right-hand sides.
 template <int N, int RB> struct TileX {
-    static constexpr int T = RB == 4 ? (N <= 16 ? clamp16(pow2floor(2048 / (12 * (N + kLpad)))) : N <= 64 ? 2 : 1)
+    static constexpr int T = N >= 512 ? 1
+                             : RB == 4 ? (N <= 16 ? clamp16(pow2floor(2048 / (12 * (N + kLpad)))) : N <= 64 ? 2 : 1)
                              : (RB == 2 && (N == 64 || N == 128)) ? 4
                              : N >= 128                           ? 2
                                                                   : clamp16(pow2floor(2048 / (6 * RB * (N + kLpad))));

@@ -263,7 +263,8 @@ __global__ __launch_bounds__(kBlock) void k_cabs_cext(Dims d, Geometry g, const 
 // ---------------------------------------------------------------------------
 
 constexpr int kFD = 4;            // directions per block
-constexpr int kFFAxis = 3 * 256;  // largest ni + nj + nl
+constexpr int kFFAxis = 3 * 256;  // ni + nj + nl of lattices up to 256 per axis (AX of k_far_field)
+constexpr int kFFAxisMax = 1088;  // largest ni + nj + nl (axes up to 512, at most 2^24 cells)
 
 // complex product and multiply-accumulate with fused multiply-adds
 __device__ __forceinline__ float2 cmul_fma(float2 a, float2 b) {
@@ -297,12 +298,13 @@ __global__ __launch_bounds__(kBlock) void k_ff_phase(Dims d, Geometry g, const f
     }
 }
 
-template <int R>
+// AX: the phase rows held in LDS per direction (>= ni + nj + nl)
+template <int R, int AX>
 __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, const float2* __restrict__ ptab,
                                                       const int* __restrict__ cells, int ncells, int ndirs, int rhs0,
                                                       const float2* __restrict__ P, double* __restrict__ partials) {
     constexpr int NV = R * 6;
-    __shared__ float2 tab[kFD * kFFAxis];
+    __shared__ float2 tab[kFD * AX];
     __shared__ double red[NV * 16];
     const int nchunk = gridDim.x;
     const int d0 = blockIdx.y * kFD;
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, const float2* __re
     for (int f = threadIdx.x; f < kFD * naxis; f += blockDim.x) {
         const int q = f / naxis, a = f - q * naxis;
         const int dd = min(d0 + q, ndirs - 1);
-        tab[q * kFFAxis + a] = ptab[dd * naxis + a];
+        tab[q * AX + a] = ptab[dd * naxis + a];
     }
     __syncthreads();
     float2 acc[kFD][R][3];
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, const float2* __re
             for (int c = 0; c < 3; ++c) p[r][c] = P[(long long)(rhs0 + r) * 3 * d.n + c * d.n + k];
 #pragma unroll
         for (int q = 0; q < kFD; ++q) {
-            const float2* t = tab + q * kFFAxis;
+            const float2* t = tab + q * AX;
             const float2 ph = cmul_fma(cmul_fma(t[i], t[d.ni + j]), t[d.ni + d.nj + l]);
 #pragma unroll
             for (int r = 0; r < R; ++r)

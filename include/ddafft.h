@@ -134,18 +134,22 @@ int ddafft_far_field(ddafft_ctx* ctx, const float* p, int ndirs, const float* di
  * reference's float argument); a parity mode, slower than mode 0. */
 int ddafft_set_sum_mode(ddafft_ctx* ctx, int mode);
 /* Launch schedule of the passes (no counterpart in the reference: the
- * knobs of the autotuner, which measures them per lattice and right-hand
- * side count).  The fused X pass works on groups of X_RB right-hand sides
- * that share each Â value read (1, 2 or 4; 0, the default: 4 when the
- * right-hand sides come in fours and n_i <= 32, else 2); each block takes a tile of
- * (J, L) columns and a run of groups, and reads the tile's Â once for the
- * whole run.  Groups are spread over more blocks (shorter runs) until a
- * launch has X_BLOCKS blocks (default: every group its own block). */
+ * knobs of the autotuner, dda-fft_amd/tune.py, which measures them on the
+ * GPU per lattice and right-hand-side count and writes the table the library
+ * uses by default, dda-fft_amd/csrc/tuned_table.hpp).  A value set here wins
+ * over the table for every later call on this context; 0 (-1 for
+ * DDAFFT_TUNE_GRAPH) hands the choice back to the table.
+ *   DDAFFT_TUNE_X_RB: the fused X pass works on groups of 1, 2 or 4
+ *     right-hand sides that share each Â value read;
+ *   DDAFFT_TUNE_X_BLOCKS: each X block takes a tile of (J, L) columns and a
+ *     run of groups, reading the tile's Â once for the whole run; groups are
+ *     spread over more blocks (shorter runs) until a launch has this many
+ *     blocks;
+ *   DDAFFT_TUNE_GRAPH: 1 replays each solver iteration as a captured HIP
+ *     graph (its kernels read their iteration from the device-resident
+ *     solver states), 0 launches every kernel of every iteration. */
 #define DDAFFT_TUNE_X_BLOCKS 0
 #define DDAFFT_TUNE_X_RB 1
-/* DDAFFT_TUNE_GRAPH: 1 (default) replays each solver iteration as a captured
- * HIP graph (its kernels read their iteration from the device-resident
- * solver states), 0 launches every kernel of every iteration. */
 #define DDAFFT_TUNE_GRAPH 2
 int ddafft_set_tuning(ddafft_ctx* ctx, int key, int value);
 /* One instance's solve for ddafft_solve_ex / ddafft_solve_batch: nrhs
@@ -177,7 +181,8 @@ int ddafft_solve_ex(ddafft_ctx* ctx, const ddafft_job* job, int solver, float ta
                     double* err_hist);
 
 /* Solves n instances (distinct contexts, e.g. random realizations of uneven
- * lattices, each with its own A-hat) together: up to `width` of them (0: 4)
+ * lattices, each with its own A-hat) together: up to `width` of them (0: the
+ * autotuner's choice, dda-fft_amd/csrc/tuned_table.hpp)
  * run at once, each on its own stream, one iteration per instance per round,
  * each right-hand side stopping on its own (the reference's sample loop,
  * main.cpp:774-996, batched).  Results as ddafft_solve_ex for each job. */

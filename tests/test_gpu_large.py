@@ -1,11 +1,12 @@
 """The 128-, 256- and 512-point kernels, the 256^3 large-target workload and
 the batched Monte-Carlo agglomerate workload on the GPU, against the oracle.
 
-Long axes: every pass instance with 128/256-point input lines and 256/512-point
-inverse lines (X pass with one and two right-hand sides per tile, Z-inverse,
-Y-inverse, the legacy radix-2 A-hat transform at 512 points) is compared bit
-for bit with the oracle: A-hat, both convolutions, the matvec and a 3-step
-BiCGStab of two right-hand sides (dipoles bit-identical, same discrepancy log).
+Long axes: every pass instance with 128/256/512-point input lines and
+256/512/1024-point inverse lines (X pass with one and two right-hand sides per
+tile, Z-inverse, Y-inverse, the legacy radix-2 A-hat transform up to 1024
+points) is compared bit for bit with the oracle: A-hat, both convolutions, the
+matvec and a 3-step BiCGStab of two right-hand sides (dipoles bit-identical,
+same discrepancy log); a 512x64x64 lattice solves bit for bit as well.
 
 Large target (sphere in 256^3, 512^3 padded FFT, x = 30, m = 1.33+1e-4i, LDR,
 BiCGStab, 19 directions): A-hat bit for bit, two BiCGStab steps of both
@@ -32,7 +33,8 @@ from conftest import bits, directions
 
 pytestmark = pytest.mark.gpu
 
-LONG_DIMS = [(256, 8, 8), (8, 256, 8), (8, 8, 256), (128, 16, 8), (128, 8, 4), (4, 4, 256)]
+LONG_DIMS = [(256, 8, 8), (8, 256, 8), (8, 8, 256), (128, 16, 8), (128, 8, 4), (4, 4, 256), (512, 8, 8),
+             (8, 512, 8), (8, 8, 512), (512, 4, 4)]
 
 
 def rnd(rng, shape):
@@ -205,3 +207,31 @@ def test_batched_agglomerates(orc32):
             assert np.array_equal(bits(P[k][r]), bits(ref["p"])), (k, r)
             assert e_esc < 1e-5
             assert abs(got["cext"] / cext - 1) < 1e-5 and abs(got["cabs"] / cabs - 1) < 1e-5
+
+
+@pytest.mark.timeout(600)
+def test_512_axis_lattice(orc32):
+    """A 512x64x64 lattice (1024x128x128 padded FFT): an elongated spheroid,
+    four BiCGStab steps of both polarizations, dipoles bit-identical to the
+    oracle's, the same discrepancy log."""
+    dims = (512, 64, 64)
+    kd = np.float32(0.1)
+    occ, ndip = ddafft.target_ellipsoid(dims, 0.125, 0.125)
+    alpha = np.array([0, orc32.polarizability(1, kd, 1.4 + 0.01j)], np.complex64)
+    kinc = np.array([1, 0, 0], np.float32)   # along the long axis
+    einc = np.array([[0, 1, 0], [0, 0, 1]], np.complex64)
+    dirs = directions(np.arange(0, 181, 20))
+    print("512x64x64: %d dipoles" % ndip, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, [0, 1])
+        res = ctx.solve("bicgstab", alpha, kinc, einc, dirs=dirs, max_step=4, err_max=1e-12)
+        P = [ctx.dipoles(r) for r in range(2)]
+        A = ctx.ahat()
+    assert np.array_equal(bits(A), bits(orc32.build_ahat(dims, kd)))
+    for pol in range(2):
+        ref = orc32.dda_fft(2, occ, [0, 1], alpha, kd, kinc, einc[pol], A, dirs, max_step=4, err_max=1e-12,
+                            return_p=True)
+        print("pol", pol, "steps", res[pol]["steps"], ref["steps"], res[pol]["err_hist"], flush=True)
+        assert res[pol]["steps"] == ref["steps"] == 4
+        np.testing.assert_allclose(res[pol]["err_hist"], ref["err_hist"], rtol=1e-9)
+        assert np.array_equal(bits(P[pol]), bits(ref["p"]))

@@ -31,6 +31,17 @@ def test_exports_match_header(lib):
         assert hasattr(lib, n), n
 
 
+def test_create_validates_lattice(lib):
+    """Axes are powers of two in [4, 512] (the reference accepts any power of
+    two, options.cpp:506-513; 512 is this build's longest line) and a
+    lattice has at most 2^24 cells; both are checked before any GPU work."""
+    h = C.c_void_p()
+    for dims in ((1024, 4, 4), (4, 2, 4), (12, 4, 4), (512, 512, 128)):
+        assert lib.ddafft_create(*dims, C.c_float(0.5), 2, C.byref(h)) == -1, dims
+        assert not h.value
+    assert "2^24" in lib.ddafft_last_error().decode()
+
+
 @pytest.mark.parametrize("key", ["ell_32_32_32_1_1", "ell_64_64_64_1_0.5", "ell_32_32_64_1_2", "ell_16_8_32_0.5_1",
                                  "par_16_16_16_0.5_0.75"])
 def test_targets_bitexact(lib, key):
