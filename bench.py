@@ -165,7 +165,7 @@ def batched_run(frac=1.0, width=8):
                             for a, b, d, c in MC_LATTICES), MC_ORIENT)}
 
 
-def large_run():
+def large_run(cpu=False):
     """The large single target: a sphere in a 256^3 lattice (512^3 padded FFT),
     x = 30, m = 1.33+1e-4i, LDR (alpha per polarization), BiCGStab to 1e-4
     (max 2000 steps), 2 polarizations, 8 planes x 361 angles (2888 directions),
@@ -186,7 +186,10 @@ def large_run():
     res = ctx.solve(2, alpha, kinc, einc, max_step=2000, err_max=1e-4)
     dt = time.perf_counter() - t0
     km, tot = ctx.time_matvec(2, 5)
+    ahat = ctx.ahat() if cpu else None
     ctx.close()
+    cpu_ref = cpu_large_baseline(occ, alpha[0], kd, kinc, einc[0], dirs.reshape(-1, 3)[:1], ahat) if cpu else None
+    del ahat
     n = int(np.prod(grid))
     by = matvec_bytes(n, 2)
     return {"value": round(2 / dt, 4), "unit": "solves/s", "seconds": round(dt, 4),
@@ -196,7 +199,8 @@ def large_run():
             "matvec_ms_2rhs": round(tot, 4), "matvec_gbps": round(sum(by.values()) / (tot * 1e-3) / 1e9, 1),
             "pass_ms": {k: round(float(v), 4) for k, v in zip(("zf", "yf", "x", "yi", "zi"), km)},
             "workload": "sphere in 256^3 (512^3 padded FFT), x=30, m=1.33+1e-4i, LDR, BiCGStab resid 1e-4 "
-                        "(max 2000), 2 polarizations, 8 planes x 361 angles, solved to convergence"}
+                        "(max 2000), 2 polarizations, 8 planes x 361 angles, solved to convergence",
+            "cpu_reference": cpu_ref}
 
 
 def matvec_scaling(reps=50):
@@ -244,6 +248,88 @@ def gradient_run(ctx, w, fwd_ms, reps=5):
             "objective": "d(mean Cext)/d(Re m, Im m) and d/d(chi) per dipole, both polarizations"}
 
 
+def host_cores():
+    """Cores this process may run on (the GPU's share of the host)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_large_baseline(occ, alpha, kd, kinc, einc, dirs, ahat):
+    """The reference's own DDA_FFT (oracle/_ref) on the 256^3 large target,
+    one core (it runs one sample), A-hat taken from the GPU (bit-identical to
+    the reference's): BiCGStab step 0 and two iterations (7 matvecs), far
+    field in one direction.  s/iteration = 2/7 of the run (an iteration is two
+    of its seven matvecs; the reductions and updates are a few percent)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    try:
+        ref = oracle.Reference()
+    except (FileNotFoundError, OSError):
+        return None
+    print("bench: reference DDA_FFT on the 256^3 target, 2 steps (one core)...", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ref.dda_fft(2, occ, np.array([0, 1], np.int32), np.asarray(alpha, np.complex64), kd, kinc, einc, ahat, dirs,
+                max_step=2, err_max=1e-12, log_path=os.devnull)
+    dt = time.perf_counter() - t0
+    return {"seconds": round(dt, 2), "matvecs": 7, "s_per_iteration": round(2 * dt / 7, 2), "cores": 1,
+            "kind": "reference", "sample": "BiCGStab step 0 + 2 iterations of one polarization (DDA_FFT, -S 2), "
+                                            "A-hat prebuilt, 1 far-field direction"}
+
+
+def cpu_batched_baseline(frac=0.125):
+    """The reference program itself (oracle/_ref/dda_fft.out) on a labelled
+    subset of the batched Monte-Carlo workload: each realization a target-11
+    shape file run with 8 Monte-Carlo orientations (-n 8 -O mc, 16 solves),
+    one process per realization and as many processes at once as this process
+    has cores (at most 16, the GPU's share of the host).  Wall time from the
+    first start to the last exit."""
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "dda_fft.out")
+    if not os.path.exists(exe):
+        return None
+    lib = ddafft.load()
+    workers = max(1, min(16, host_cores()))
+    insts = mc_instances(frac)
+    tmp = tempfile.mkdtemp(prefix="ddafft_cpu_")
+    cmds = []
+    for k, it in enumerate(insts):
+        d = os.path.join(tmp, "r%d" % k)
+        os.makedirs(d)
+        ni, nj, nl = it["dims"]
+        lib.ddafft_target_file_write.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        lib.ddafft_target_file_write(os.path.join(d, "shape.txt").encode(), ni, nj, nl, _p(it["occ"]))
+        x = float(it["kd"]) * ni / 2.0
+        cmds.append(([exe, "-t", "file", "-f", "shape.txt", "-g", str(ni), str(nj), str(nl), "-x", "%.9g" % x,
+                      "-m", str(MC_M[0]), str(MC_M[1]), "-P", "cm", "-M", "bicgstab", "-e", str(TOL), "-S",
+                      str(MAX_STEPS), "-O", "mc", "-n", str(MC_ORIENT), "-p", "1", "-a", "0", "-b", "180", "-s",
+                      "10", "-j", "1", "--seed", str(MC_SEED + k), "-o", "out.dat", "-D", "out.dst"], d))
+    print("bench: reference program on %d realizations, %d at once..." % (len(cmds), workers), file=sys.stderr,
+          flush=True)
+    running, pending, failed = [], list(cmds), 0
+    t0 = time.perf_counter()
+    while pending or running:
+        while pending and len(running) < workers:
+            cmd, d = pending.pop(0)
+            running.append(subprocess.Popen(cmd, cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+        time.sleep(0.02)
+        for pr in list(running):
+            if pr.poll() is not None:
+                failed += pr.returncode != 0
+                running.remove(pr)
+    dt = time.perf_counter() - t0
+    solves = 2 * MC_ORIENT * len(cmds)
+    return {"value": round(solves / dt, 3), "unit": "solves/s", "seconds": round(dt, 2), "solves": solves,
+            "cores": workers, "kind": "reference", "failed_runs": failed,
+            "sample": "%d of the 128 realizations (%s), each the reference program with -t file -n 8 -O mc, "
+                      "%d processes at once" % (len(cmds), "/".join(
+                          "%dx%s" % (max(1, int(round(c * frac))), "x".join(map(str, (a, b, e))))
+                          for a, b, e, c in MC_LATTICES), workers)}
+
+
 def cpu_baseline(w, budget_s=10.0):
     """The reference's own DDA_FFT (oracle/_ref, compiled from its sources) on
     this host for the same sample; falls back to the oracle restatement."""
@@ -272,6 +358,7 @@ def cpu_baseline(w, budget_s=10.0):
         "value": round(solves / t_total, 4),
         "unit": "solves/s",
         "cores": 1,
+        "host_cores": host_cores(),
         "kind": kind,
         "sample": "%d BiCGStab solve(s) of the same 64^3 workload (DDA_FFT incl. far field, A-hat prebuilt), "
                   "single worker as the reference runs one sample" % solves,
@@ -391,9 +478,12 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_run(args.batched_frac, args.batched_width)
     if not args.no_large:
-        out["large"] = large_run()
+        out["large"] = large_run(cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
+        out["cpu_baseline"]["batched"] = cpu_batched_baseline()
+        if "large" in out:
+            out["cpu_baseline"]["large"] = out["large"].pop("cpu_reference")
     print(json.dumps(out), flush=True)
     return 0
 
