@@ -234,6 +234,7 @@ struct ddafft_ctx {
     // launch schedule, 0 / -1: from the tuned table, else the built-in rule
     int x_blocks = 0;                  // X pass: blocks per launch up to which right-hand-side groups are spread
     int x_rb = 0;                      // X pass: right-hand sides per tile
+    int fuse_zy = -1;                  // z/y forward fused per i-plane where the plane allows it (-1: schedule)
     // one solver iteration captured as a HIP graph and replayed (use_graph:
     // 1 yes, 0 no, -1 the tuned choice); recaptured when anything its
     // launches bake in changes
@@ -284,22 +285,27 @@ int inc_tab_len(const ddafft_ctx* c);
 // dda-fft_amd/tune.py on an MI355X), else the built-in rule; a value set
 // through ddafft_set_tuning wins over both.
 struct Schedule {
-    int x_rb, x_blocks, graph;
+    int x_rb, x_blocks, graph, fuse;
 };
 
 Schedule schedule(const ddafft_ctx* c, int nrhs) {
     // built-in rule: four right-hand sides per X tile when they come in fours
     // on lines of at most 32 points (32^3, 16 RHS: 41 us against 44.5 in
-    // pairs; 64^3: 425 against 315), every group its own block, graphs on
-    Schedule s{(nrhs % 4 == 0 && c->d.ni <= 32) ? 4 : 2, 1 << 30, 1};
+    // pairs; 64^3: 425 against 315), every group its own block, graphs on;
+    // z/y forward fused when its launch (one block per (component, i)-plane)
+    // has at least 192 blocks: below that the split passes' wider grids win
+    // (32^3, 1 RHS: 38.0 us a matvec split, 39.3 fused; 64^3, 16 RHS: 709
+    // against 675)
+    Schedule s{(nrhs % 4 == 0 && c->d.ni <= 32) ? 4 : 2, 1 << 30, 1, c->d.ni * 3 * nrhs >= 192 ? 1 : 0};
     for (const TunedEntry& e : kTuned)
         if (e.ni == c->d.ni && e.nj == c->d.nj && e.nl == c->d.nl && e.nrhs == nrhs) {
-            s = Schedule{e.x_rb, e.x_blocks, e.graph};
+            s = Schedule{e.x_rb, e.x_blocks, e.graph, e.fuse};
             break;
         }
     if (c->x_rb > 0) s.x_rb = c->x_rb;
     if (c->x_blocks > 0) s.x_blocks = c->x_blocks;
     if (c->use_graph >= 0) s.graph = c->use_graph;
+    if (c->fuse_zy >= 0) s.fuse = c->fuse_zy;
     return s;
 }
 
@@ -439,7 +445,27 @@ int check_args(const ProArgs& pa, const EpiArgs& ea, bool solver) {
     return DDAFFT_OK;
 }
 
-// One matvec of nrhs right-hand sides: ZF -> YF -> X -> YI -> ZI.
+// The fused z/y-forward pass for the plane shapes it is built for (n_j, n_l
+// in {8, 16, 32, 64}); false when the plane needs ZF + YF.
+bool zyf_launch(ddafft_ctx* c, const Dims& d, const ProArgs& pa, SolveState* st, int C, hipStream_t s) {
+    const dim3 grid(d.ni, C);
+#define DDA_ZYF(NJ, NL)                                                                                        \
+    case NJ * 1024 + NL:                                                                                       \
+        hipLaunchKernelGGL((k_zyf<NJ, NL>), grid, kBlock, 0, s, d, pa, st, c->dipf, c->b2, c->wpre_l, c->tw_l, \
+                           c->wpre_j, c->tw_j);                                                                \
+        return true;
+    switch (d.nj * 1024 + d.nl) {
+        DDA_ZYF(8, 8) DDA_ZYF(8, 16) DDA_ZYF(8, 32) DDA_ZYF(8, 64)
+        DDA_ZYF(16, 8) DDA_ZYF(16, 16) DDA_ZYF(16, 32) DDA_ZYF(16, 64)
+        DDA_ZYF(32, 8) DDA_ZYF(32, 16) DDA_ZYF(32, 32) DDA_ZYF(32, 64)
+        DDA_ZYF(64, 8) DDA_ZYF(64, 16) DDA_ZYF(64, 32) DDA_ZYF(64, 64)
+        default:
+            return false;
+    }
+#undef DDA_ZYF
+}
+
+// One matvec of nrhs right-hand sides: ZF -> YF (or the fused ZYF) -> X -> YI -> ZI.
 int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, int iter, const EpiArgs& ea,
                   int conj, hipEvent_t* ev = nullptr) {
     if (!ev && c->prof) {
@@ -465,7 +491,12 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
     d.nrhs = nrhs;
     const int C = 3 * nrhs;
     hipStream_t s = c->stream;
+    const Schedule sch = schedule(c, nrhs);
     if (ev) HIPCHECK(hipEventRecord(ev[0], s));
+    // z and y forward: fused per i-plane for planes up to 64 x 64, else ZF then YF
+    if (sch.fuse && zyf_launch(c, d, pa, st, C, s)) {
+        if (ev) HIPCHECK(hipEventRecord(ev[1], s));
+    } else {
     {
         const int tj = std::min(tile_zf(d.nl), d.nj);
         dim3 grid(d.ni * (d.nj / tj), C);
@@ -474,9 +505,10 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
     }
     if (ev) HIPCHECK(hipEventRecord(ev[1], s));
     {
-        const int ti = std::min(tile_zf(d.nj), d.ni);
-        dim3 grid(2 * d.nl * (d.ni / ti), C);
+        const int tl = std::min(tile_zf(d.nj), 2 * d.nl);   // L' lines per block
+        dim3 grid(d.ni * (2 * d.nl / tl), C);
         DDA_SWITCH_SMALL(d.nj, hipLaunchKernelGGL((k_yf<NN>), grid, kBlock, 0, s, d, st, c->b1, c->b2, c->wpre_j, c->tw_j));
+    }
     }
     if (ev) HIPCHECK(hipEventRecord(ev[2], s));
     {
@@ -484,12 +516,11 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
         // each Â value; a block takes `gpb` consecutive groups and reads its
         // tile's Â once for all of them, as many groups as keep about
         // x_blocks blocks in the launch
-        const Schedule sch = schedule(c, nrhs);
         int rb = sch.x_rb;
         while (rb > 1 && rb > nrhs) rb >>= 1;
         if (d.ni > 256) rb = 1;
-        const int tj = std::min(tile_x(d.ni, rb), 2 * d.nj);
-        const int ntile = 2 * d.nl * (2 * d.nj / tj), ngroups = (nrhs + rb - 1) / rb;
+        const int tj = std::min(tile_x(d.ni, rb), 2 * d.nl);   // L' columns per tile
+        const int ntile = 2 * d.nj * (2 * d.nl / tj), ngroups = (nrhs + rb - 1) / rb;
         const int ysets = std::max(1, std::min(ngroups, (int)std::min<long long>(((long long)sch.x_blocks + ntile - 1) / ntile, 1 << 20)));
         const int gpb = (ngroups + ysets - 1) / ysets;
         dim3 grid(ntile, (ngroups + gpb - 1) / gpb);
@@ -1048,6 +1079,10 @@ int ddafft_set_tuning(ddafft_ctx* c, int key, int value) {
                 return fail(DDAFFT_EINVAL, "x_rb must be 0 (automatic), 1, 2 or 4");
             c->x_rb = value;
             return DDAFFT_OK;
+        case DDAFFT_TUNE_FUSE_ZY:
+            if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "fuse_zy must be -1 (tuned), 0 or 1");
+            c->fuse_zy = value;
+            return DDAFFT_OK;
         case DDAFFT_TUNE_GRAPH:
             if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "graph must be -1 (tuned), 0 or 1");
             c->use_graph = value;
@@ -1511,7 +1546,7 @@ bool run_more(const Run& r) { return r.o <= r.max_step && r.o < r.known_stop; }
 void graph_key(const Run& r, std::vector<char>& out) {
     const ddafft_ctx* c = r.c;
     struct {
-        int solver, nrhs, max_step, x_rb, x_blocks, nmat;
+        int solver, nrhs, max_step, x_rb, x_blocks, nmat, fuse;
         float tau;
         double err_max, inv_ndip;
         const void *hist, *partials, *alpha, *ring, *mv;
@@ -1522,6 +1557,7 @@ void graph_key(const Run& r, std::vector<char>& out) {
     k.max_step = r.max_step;
     k.x_rb = c->x_rb;
     k.x_blocks = c->x_blocks;
+    k.fuse = c->fuse_zy;
     k.nmat = c->nmat;
     k.tau = r.tau;
     k.err_max = r.q.fa.err_max;

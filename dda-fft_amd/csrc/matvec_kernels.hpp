@@ -4,13 +4,15 @@
 //
 // HBM layouts (float2 = complex64, C = 3*nrhs component planes):
 //   field x, out : [C][ni][nj][nl]            (one plane per component)
-//   B1 / B4      : [C][ni][2nl][nj] / [C][ni][nj][2nl]
-//   B2           : [C][2nj][2nl][ni]
-//   B3           : [C][ni][2nl][2nj]
+//   B1 / B4      : [C][ni][2nl'][nj] / [C][ni][nj][2nl']
+//   B2 / B3      : [C][ni][2nj][2nl']         (plane-major: one i-plane after another)
 //   Â            : [2nj][2nl][2ni][6]         (e00 e01 e02 e11 e12 e22)
-// Every pass reads whole contiguous lines and writes its result transposed so
-// that the next pass again reads contiguous lines; the transposed writes go
-// out in runs of a tile of adjacent lines.
+// L' is the z-frequency in parity-split order: L' = L / 2 for even L and
+// n_l + L / 2 for odd L, the order in which the pruned z transform produces
+// them (two n_l-point transforms, fft_func.cpp:447-462); the Z-inverse pass
+// puts them back in natural order in LDS.  Lines along l, j and L' are
+// contiguous; the X pass reads and writes runs of its tile's L' columns in
+// every i-plane.
 //
 //   ZF  z forward, pruned (fft_func.cpp:442-463)      x  -> B1
 //   YF  y forward, pruned (fft_func.cpp:484-504)      B1 -> B2
@@ -198,10 +200,10 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
 #pragma unroll
         for (int q = 0; q < NST; ++q) {
             const int f = threadIdx.x + q * kBlock;
-            if (f < 2 * NL * tj) {
-                const int L = f >> ltj, t = f - (L << ltj);
-                const float2 v = (L & 1) ? s[(tj + t) * LS + (L >> 1)] : s[t * LS + (L >> 1)];
-                dst[(L << lnj) + t] = v;
+            if (f < 2 * NL * tj) {   // row L' of B1: the even transform's outputs, then the odd one's
+                const int Lp = f >> ltj, t = f - (Lp << ltj);
+                const float2 v = Lp >= NL ? s[(tj + t) * LS + (Lp - NL)] : s[t * LS + Lp];
+                dst[(Lp << lnj) + t] = v;
             }
         }
     };
@@ -212,7 +214,173 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
 }
 
 // ---------------------------------------------------------------------------
-// YF: lines (c, i, L) along j; n_j -> 2 n_j.
+// ZYF: the z- and y-forward passes of one (c, i) plane in one block, for
+// planes of at most 64 x 64 cells (ZF + YF cover the larger ones).  The block
+// loads the plane once (with the prologue of ZF, each cell read and written
+// by this block only), then for each parity p of the z frequencies: the n_l
+// point transforms of its n_j rows in LDS, the result taken into registers
+// transposed, and for each parity q of the y frequencies the n_j point
+// transforms of the n_l columns, written to B2 as whole rows J = 2 Jh + q
+// (columns L' = p n_l + Lh).  The same butterflies, twiddles and order as
+// ZF then YF: the same bits.  B1 is not touched.
+// ---------------------------------------------------------------------------
+
+template <int NJ, int NL>
+struct ZyfShape {
+    static constexpr int LSZ = NL + kLpad;                      // z lines: NJ rows of NL points
+    static constexpr int LSY = NJ + kLpad;                      // y lines: NL columns of NJ points
+    static constexpr int WORDS = NJ * LSZ > NL * LSY ? NJ * LSZ : NL * LSY;
+    static constexpr int E = (NJ * NL + kBlock - 1) / kBlock;   // plane cells per thread
+};
+
+template <int NJ, int NL>
+__global__ __launch_bounds__(kBlock) void k_zyf(Dims d, ProArgs pa, const SolveState* __restrict__ st,
+                                                const int8_t* __restrict__ dipf, float2* __restrict__ b2,
+                                                const float2* __restrict__ wpre_l, const float2* __restrict__ tw_l,
+                                                const float2* __restrict__ wpre_j, const float2* __restrict__ tw_j) {
+    using S = ZyfShape<NJ, NL>;
+    constexpr int E = S::E, LSZ = S::LSZ, LSY = S::LSY;
+    constexpr int NP = NJ * NL;
+    constexpr int LNL = ilog2c(NL), LNJ = ilog2c(NJ);
+    __shared__ float2 w[S::WORDS];
+    __shared__ float2 stl[NL], stj[NJ];
+    const int i = blockIdx.x, c = blockIdx.y, rhs = c / 3;
+    float2 c1 = make_float2(0.f, 0.f), c2 = c1;
+    int snap = 0;
+    if (st) {
+        if (st[rhs].it >= st[rhs].stop_at) return;
+        c1 = st[rhs].pro_c1;
+        c2 = st[rhs].pro_c2;
+        snap = (pa.Pmin != nullptr && pa.P != nullptr) ? st[rhs].snapshot : 0;
+    }
+    const long long cell0 = (long long)i * NP;            // first cell of the plane
+    const long long cb = (long long)c * d.n + cell0;      // its cells of component c
+    // the plane, through the prologue (dda_func.cpp:843-924), in chunks of
+    // KC cells per thread: the loads of a chunk are issued before the first is
+    // used, and a chunk's operands are dead before the next chunk's arrive
+    // (all at once they took the kernel to 198 VGPRs at 64 x 64)
+    for (int f = threadIdx.x; f < NL; f += kBlock) stl[f] = tw_l[f];
+    for (int f = threadIdx.x; f < NJ; f += kBlock) stj[f] = tw_j[f];
+    float2 v[E];
+    constexpr int KC = E < 4 ? E : 4;
+    const int mode = pa.mode;
+#pragma unroll
+    for (int q0 = 0; q0 < E; q0 += KC) {
+        bool dip[KC];
+        float2 va[KC], vb[KC], vd[KC], vp[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int f = threadIdx.x + (q0 + k) * kBlock;
+            dip[k] = false;
+            va[k] = vb[k] = vd[k] = vp[k] = make_float2(0.f, 0.f);
+            if (NP % kBlock == 0 || f < NP) {
+                dip[k] = dipf[cell0 + f] != 0;
+                va[k] = pa.A[cb + f];
+                if (mode != PRO_COPY) vb[k] = pa.B[cb + f];
+                if (mode == PRO_LC4) vd[k] = pa.D[cb + f];
+                if (snap) vp[k] = pa.P[cb + f];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int q = q0 + k;
+            const int f = threadIdx.x + q * kBlock;
+            if (NP % kBlock == 0 || f < NP) {
+                float2 x;
+                switch (mode) {
+                    case PRO_LC1:
+                        x = va[k];
+                        if (dip[k]) {
+                            x = cadd(va[k], cmul(c1, vb[k]));
+                            pa.A[cb + f] = x;
+                        }
+                        break;
+                    case PRO_LC4:
+                        x = dip[k] ? cadd(cmul(c2, vd[k]), cadd(va[k], cmul(c1, vb[k]))) : make_float2(0.f, 0.f);
+                        pa.B[cb + f] = x;
+                        break;
+                    case PRO_LC2:
+                        x = vb[k];
+                        if (dip[k]) {
+                            x = cadd(va[k], cmul(c1, vb[k]));
+                            pa.B[cb + f] = x;
+                        }
+                        break;
+                    default:
+                        x = va[k];
+                }
+                if (snap) pa.Pmin[cb + f] = vp[k];
+                v[q] = x;
+            } else {
+                v[q] = make_float2(0.f, 0.f);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);   // keep the next chunk's loads behind this chunk's use
+    }
+    float2* dst = b2 + ((long long)c * d.ni + i) * (2 * NJ) * (2 * NL);   // plane (c, i) of B2 [J][L']
+    // where the prologue left the matvec input (this thread's own cells: it
+    // reads back what it wrote), so that the plane is not held in registers
+    // across the first parity
+    const float2* xin = (mode == PRO_LC4 || mode == PRO_LC2) ? pa.B : pa.A;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        if (p) {
+            __syncthreads();   // the previous parity's last reads of w are done
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (NP % kBlock == 0 || f < NP) v[q] = xin[cb + f];
+            }
+        }
+        // rows j along l: x (p = 0) or W_2nl^l x (p = 1), as ZF forms them
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            if (NP % kBlock == 0 || f < NP) {
+                const int j = f >> LNL, l = f & (NL - 1);
+                w[j * LSZ + l] = p ? cmul(wpre_l[l], v[q]) : v[q];
+            }
+        }
+        __syncthreads();
+        fft_lds<NL, NJ, LSZ>(w, NJ, stl);
+        // Z_p[j][Lh] into registers, transposed: element f = Lh * NJ + j
+        float2 u[E];
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int f = threadIdx.x + q * kBlock;
+            u[q] = make_float2(0.f, 0.f);
+            if (NP % kBlock == 0 || f < NP) u[q] = w[(f & (NJ - 1)) * LSZ + (f >> LNJ)];
+        }
+#pragma unroll
+        for (int qy = 0; qy < 2; ++qy) {
+            __syncthreads();   // every read of w above (or of the previous qy's output) is done
+            // columns Lh along j: Z (qy = 0) or W_2nj^j Z (qy = 1), as YF forms them
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (NP % kBlock == 0 || f < NP) {
+                    const int lh = f >> LNJ, j = f & (NJ - 1);
+                    w[lh * LSY + j] = qy ? cmul(wpre_j[j], u[q]) : u[q];
+                }
+            }
+            __syncthreads();
+            fft_lds<NJ, NL, LSY>(w, NL, stj);
+            // row J = 2 Jh + qy of the plane, columns p NL .. p NL + NL - 1
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (NP % kBlock == 0 || f < NP) {
+                    const int jh = f >> LNL, lh = f & (NL - 1);
+                    dst[(long long)(2 * jh + qy) * (2 * NL) + p * NL + lh] = w[lh * LSY + jh];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// YF: lines (c, i, L') along j; n_j -> 2 n_j.  A tile: consecutive L' of one
+// i-plane, written as runs of L' in each row J of B2.
 // ---------------------------------------------------------------------------
 
 template <int NJ>
@@ -230,17 +398,17 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
     // the tile body, with the tile width a compile-time constant for full
     // tiles and a run-time one otherwise
     auto tile = [&](auto tile_arg) {
-        const int ti = tile_arg, nit = d.ni / ti;
-        const int L = blockIdx.x / nit, i0 = (blockIdx.x - L * nit) * ti;
-        const int lnl2 = __builtin_ctz(NL2), lni = __builtin_ctz(d.ni), lti = __builtin_ctz(ti);
+        const int ti = tile_arg, nlt = NL2 / ti;
+        const int i = blockIdx.x / nlt, L0 = (blockIdx.x - i * nlt) * ti;
+        const int lnl2 = __builtin_ctz(NL2), lti = __builtin_ctz(ti);
         {
             constexpr int NLD = (TI * NJ + kBlock - 1) / kBlock;
-            const float2* src = b1 + (((long long)c * d.ni + i0) * NL2 + L) * NJ;
+            const float2* src = b1 + (((long long)c * d.ni + i) * NL2 + L0) * NJ;   // ti contiguous lines
             float2 v[NLD];
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {
                 const int f = threadIdx.x + q * kBlock;
-                if (f < ti * NJ) v[q] = src[((f >> LNJ) << (lnl2 + LNJ)) + (f & (NJ - 1))];
+                if (f < ti * NJ) v[q] = src[f];
             }
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {
@@ -255,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
         for (int f = threadIdx.x; f < NJ; f += kBlock) stw[f] = tw[f];
         __syncthreads();
         fft_lds<NJ, 2 * TI, LS>(s, 2 * ti, stw);
-        float2* dst = b2 + ((long long)c * 2 * NJ * NL2 + L) * d.ni + i0;
+        float2* dst = b2 + ((long long)c * d.ni + i) * (2 * NJ) * NL2 + L0;   // row J of plane (c, i), from L0
         constexpr int NST = (2 * NJ * TI + kBlock - 1) / kBlock;
 #pragma unroll
         for (int q = 0; q < NST; ++q) {
@@ -263,14 +431,14 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
             if (f < 2 * NJ * ti) {
                 const int J = f >> lti, t = f - (J << lti);
                 const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
-                dst[(J << (lnl2 + lni)) + t] = v;
+                dst[(J << lnl2) + t] = v;
             }
         }
     };
-    if (d.ni >= TI)
+    if (NL2 >= TI)
         tile(std::integral_constant<int, TI>{});
     else
-        tile(d.ni);
+        tile(NL2);
 }
 
 // ---------------------------------------------------------------------------
@@ -317,9 +485,10 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
     constexpr int LS1 = XStride<NI, RB>::S1, LS2 = XStride<NI, RB>::S2;   // forward / inverse line strides
     constexpr int ITEMS = (TJ * NI2 + kBlock - 1) / kBlock;
     const int NJ2 = 2 * d.nj, NL2 = 2 * d.nl;
-    const int tj = FULL ? TJ : tj_small, njt = NJ2 / tj;
+    const int tj = FULL ? TJ : tj_small, nlt = NL2 / tj;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int L = tile / njt, J0 = (tile - L * njt) * tj;
+    // columns (J, L0 .. L0 + tj - 1) in the parity-split order L'
+    const int J = tile / nlt, L0 = (tile - J * nlt) * tj;
 
     // Â of this thread's points, requested first so that its latency overlaps
     // the field loads and the forward transform
@@ -329,7 +498,8 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
         const int f = threadIdx.x + it * kBlock;
         if (f < tj * NI2) {
             const int I = f % NI2, t = f / NI2;
-            unsigned Ja = J0 + t, La = L, Ia = I;
+            const int Lp = L0 + t;
+            unsigned Ja = J, La = Lp < d.nl ? 2 * Lp : 2 * (Lp - d.nl) + 1, Ia = I;   // natural L
             if (conj) {
                 Ja = (NJ2 - Ja) & (NJ2 - 1);
                 La = (NL2 - La) & (NL2 - 1);
@@ -366,16 +536,17 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
             // block's lines lie at 32-bit offsets from one 64-bit base
             constexpr int NLD = (CB * TJ * NI + kBlock - 1) / kBlock;
             const int nld = CB * tj * NI;
-            const float2* src = b2 + ((long long)(rhs0 * 3) * NJ2 + J0) * ((long long)NL2 * NI) + (long long)L * NI;
-            const int lcs = __builtin_ctz(NJ2) + __builtin_ctz(NL2) + LNI, lts = __builtin_ctz(NL2) + LNI;
+            // runs of tj columns in row J of the (cc, i) planes
+            const float2* src = b2 + ((long long)(rhs0 * 3) * NI * NJ2 + J) * NL2 + L0;
+            const int lrow = __builtin_ctz(NJ2) + __builtin_ctz(NL2);
             float2 v[NLD];
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {
                 const int f = threadIdx.x + q * kBlock;
                 v[q] = make_float2(0.f, 0.f);
                 if (f < nld) {
-                    const int ii = f & (NI - 1), r = f >> LNI, t = r & (tj - 1), cc = r >> ltj;
-                    if (act[cc / 3]) v[q] = src[(cc << lcs) + (t << lts) + ii];
+                    const int t = f & (tj - 1), r = f >> ltj, cc = r >> LNI;   // r = cc * NI + i
+                    if (act[cc / 3]) v[q] = src[(r << lrow) + t];
                 }
             }
             // twiddles staged after the first field loads are issued, so that the
@@ -391,7 +562,8 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
             for (int q = 0; q < NLD; ++q) {
                 const int f = threadIdx.x + q * kBlock;
                 if (f < nld) {
-                    const int ii = f & (NI - 1), line = (f >> LNI) * 2;   // (cc * tj + t) * 2
+                    const int t = f & (tj - 1), r = f >> ltj, ii = r & (NI - 1), cc = r >> LNI;
+                    const int line = (cc * tj + t) * 2;
                     s[line * LS1 + ii] = v[q];
                     s[(line + 1) * LS1 + ii] = cmul(wpre[ii], v[q]);
                 }
@@ -432,7 +604,7 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
         __syncthreads();
         fft_lds<NI2, CB * TJ, LS2>(s, CB * tj, stw2);
         {
-            float2* dst = b3 + (long long)(rhs0 * 3) * NI * ((long long)NL2 * NJ2) + (long long)L * NJ2 + J0;
+            float2* dst = b3 + (long long)(rhs0 * 3) * NI * ((long long)NL2 * NJ2) + (long long)J * NL2 + L0;
             const int lrow = __builtin_ctz(NL2) + __builtin_ctz(NJ2);
             constexpr int NST = (CB * TJ * NI + kBlock - 1) / kBlock;
             const int nst = CB * tj * NI;
@@ -473,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     // Measured on MI355X (2 RHS), against a run-time tile width: n_i = 64
     // 57.8 -> 56.3 us per launch, n_i = 128 612 -> 534 us, n_i = 256
     // 6571 -> 5688 us.
-    const int tj = min(TJ, 2 * d.nj);
+    const int tj = min(TJ, 2 * d.nl);
     if (tj == TJ)
         x_tile<NI, RB, true, RUN>(d, amask, g0, g1, TJ, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
     else
@@ -481,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
 }
 
 // ---------------------------------------------------------------------------
-// YI: lines (c, i, L) along J; 2 n_j -> j < n_j.
+// YI: lines (c, i, L') along J; 2 n_j -> j < n_j.
 // ---------------------------------------------------------------------------
 
 template <int NJ2>
@@ -500,20 +672,21 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
     auto tile = [&](auto tile_arg) {
         const int tl = tile_arg, nlt = NL2 / tl;
         const int i = blockIdx.x / nlt, L0 = (blockIdx.x - i * nlt) * tl;
-        constexpr int LNJ2 = ilog2c(NJ2);
-        const float2* src = b3 + (((long long)c * d.ni + i) * NL2 + L0) * NJ2;
+        const int ltl0 = __builtin_ctz(tl), lnl20 = __builtin_ctz(NL2);
+        // columns L0 .. L0 + tl - 1 of plane (c, i) of B3 [J][L']: runs of tl in every row J
+        const float2* src = b3 + ((long long)c * d.ni + i) * NJ2 * NL2 + L0;
         {
             constexpr int NLD = (TL * NJ2 + kBlock - 1) / kBlock;
             float2 v[NLD];
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {
                 const int f = threadIdx.x + q * kBlock;
-                if (f < tl * NJ2) v[q] = src[f];
+                if (f < tl * NJ2) v[q] = src[((f >> ltl0) << lnl20) + (f & (tl - 1))];
             }
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {
                 const int f = threadIdx.x + q * kBlock;
-                if (f < tl * NJ2) s[(f >> LNJ2) * LS + (f & (NJ2 - 1))] = v[q];
+                if (f < tl * NJ2) s[(f & (tl - 1)) * LS + (f >> ltl0)] = v[q];   // line t, point J
             }
         }
         for (int f = threadIdx.x; f < NJ2; f += kBlock) stw[f] = tw[f];
@@ -624,7 +797,10 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ 
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {
                 const int f = threadIdx.x + q * kBlock;
-                if (f < nld) s[(f >> LNL2) * LS + (f & (NL2 - 1))] = v[q];   // line comp * tj + t
+                if (f < nld) {   // line comp * tj + t; L' back to natural order
+                    const int Lp = f & (NL2 - 1);
+                    s[(f >> LNL2) * LS + (Lp < NLH ? 2 * Lp : 2 * (Lp - NLH) + 1)] = v[q];
+                }
             }
         }
         for (int f = threadIdx.x; f < NL2; f += kBlock) stw[f] = tw[f];

@@ -162,18 +162,22 @@ class Context:
         sequential float sums in its cell order (ddafft_set_sum_mode)."""
         _check(_lib.ddafft_set_sum_mode(self._h, int(mode)))
 
-    def set_tuning(self, x_blocks=None, x_rb=None, graph=None):
+    def set_tuning(self, x_blocks=None, x_rb=None, graph=None, fuse_zy=None):
         """Launch schedule (ddafft_set_tuning); None leaves a setting as it
         is.  x_blocks: X-pass launch size up to which right-hand-side groups
         are spread over blocks (0: the tuned table's); x_rb: right-hand sides
         per X tile (0: tuned, 1, 2 or 4); graph: iterations replayed as
-        captured HIP graphs (True / False, or -1: tuned)."""
+        captured HIP graphs (True / False, or -1: tuned); fuse_zy: z- and
+        y-forward transforms of an i-plane in one kernel where it fits (True /
+        False, or -1: tuned)."""
         if x_blocks is not None:
             _check(_lib.ddafft_set_tuning(self._h, 0, int(x_blocks)))
         if x_rb is not None:
             _check(_lib.ddafft_set_tuning(self._h, 1, int(x_rb)))
         if graph is not None:
             _check(_lib.ddafft_set_tuning(self._h, 2, -1 if graph == -1 else int(bool(graph))))
+        if fuse_zy is not None:
+            _check(_lib.ddafft_set_tuning(self._h, 3, -1 if fuse_zy == -1 else int(bool(fuse_zy))))
 
     def set_directions(self, dirs):
         dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)

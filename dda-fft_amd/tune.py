@@ -6,6 +6,8 @@ Measures, on the GPU it runs on, for each (lattice, right-hand sides) case:
   every grouping of the right-hand sides: 1, 2 or 4 per tile sharing each
   A-hat read, and runs of groups per block (one group per block, or runs
   long enough to cut the launch to 8192 / 2048 blocks);
+- the z- and y-forward transforms as one kernel per i-plane against two
+  passes (planes up to 64 x 64);
 - a whole BiCGStab solve with its iterations replayed from captured HIP
   graphs against launched kernel by kernel;
 - for the batched Monte-Carlo workload, the number of realizations in
@@ -56,6 +58,11 @@ def time_x(ctx, nrhs, reps):
     return float(km[2])
 
 
+def time_zy(ctx, nrhs, reps):
+    km, _ = ctx.time_matvec(nrhs, reps)
+    return float(km[0] + km[1])
+
+
 def target(dims, x=5.0, m=1.5 + 0.01j):
     kd = np.float32(2.0 * x / dims[0])
     occ, _ = ddafft.target_ellipsoid(dims, min(1.0, dims[1] / dims[0]), min(1.0, dims[2] / dims[0]))
@@ -78,6 +85,17 @@ def tune_case(dims, nrhs, reps, log):
             best = (t, rb, xb)
     _, rb, xb = best
     ctx.set_tuning(x_rb=rb, x_blocks=xb)
+    fuse = 0
+    if dims[1] <= 64 and dims[2] <= 64:
+        tz = {}
+        for f in (0, 1, 0, 1):
+            ctx.set_tuning(fuse_zy=f)
+            time_zy(ctx, nrhs, 2)
+            tz[f] = min(tz.get(f, 1e30), min(time_zy(ctx, nrhs, reps) for _ in range(2)))
+        fuse = 1 if tz[1] < tz[0] else 0
+        log("%-14s R=%-2d z/y forward: two passes %.1f us, fused %.1f us -> fuse=%d" % (
+            "x".join(map(str, dims)), nrhs, tz[0] * 1e3, tz[1] * 1e3, fuse))
+    ctx.set_tuning(fuse_zy=fuse)
     # graphs against launches on a whole solve of nrhs right-hand sides
     kinc = np.array([0, 0, 1], np.float32)
     einc = np.array([np.eye(3)[r % 2] for r in range(nrhs)], np.complex64)
@@ -98,7 +116,7 @@ def tune_case(dims, nrhs, reps, log):
     log("%-14s R=%-2d solve (%d BiCGStab steps): launched %.3f ms, graphs %.3f ms -> graph=%d" % (
         "x".join(map(str, dims)), nrhs, steps, tg[0] * 1e3, tg[1] * 1e3, graph))
     ctx.close()
-    return dict(dims=dims, nrhs=nrhs, x_rb=rb, x_blocks=xb, graph=graph, x_us=best[0] * 1e3,
+    return dict(dims=dims, nrhs=nrhs, x_rb=rb, x_blocks=xb, graph=graph, fuse=fuse, x_us=best[0] * 1e3,
                 solve_ms=min(tg.values()) * 1e3)
 
 
@@ -133,18 +151,19 @@ def write_table(entries, width, path):
              "// dda-fft_amd/tune.py from measurements on an MI355X.  x_rb: right-hand",
              "// sides per X-pass tile; x_blocks: X-pass launch size up to which groups of",
              "// right-hand sides are spread over blocks (1 << 30: each group its own",
-             "// block); graph: iterations replayed as captured HIP graphs.",
+             "// block); graph: iterations replayed as captured HIP graphs; fuse: z- and",
+             "// y-forward transforms of an i-plane in one kernel.",
              "#pragma once", "", "namespace ddafft {", "",
-             "struct TunedEntry {", "    int ni, nj, nl, nrhs, x_rb, x_blocks, graph;", "};", "",
+             "struct TunedEntry {", "    int ni, nj, nl, nrhs, x_rb, x_blocks, graph, fuse;", "};", "",
              "// realizations in flight in ddafft_solve_batch for the batched Monte-Carlo",
-             "// workload (the call's width argument; 0 there means 4)",
+             "// workload (the call's width argument 0 stands for it)",
              "constexpr int kTunedBatchWidth = %d;" % width, "",
              "constexpr TunedEntry kTuned[] = {"]
     for e in entries:
         xb = "1 << 30" if e["x_blocks"] == 1 << 30 else str(e["x_blocks"])
-        lines.append("    {%d, %d, %d, %d, %d, %s, %d},   // X pass %.1f us, solve %.3f ms" % (
-            e["dims"][0], e["dims"][1], e["dims"][2], e["nrhs"], e["x_rb"], xb, e["graph"], e["x_us"],
-            e["solve_ms"]))
+        lines.append("    {%d, %d, %d, %d, %d, %s, %d, %d},   // X pass %.1f us, solve %.3f ms" % (
+            e["dims"][0], e["dims"][1], e["dims"][2], e["nrhs"], e["x_rb"], xb, e["graph"], e["fuse"],
+            e["x_us"], e["solve_ms"]))
     lines += ["};", "", "}  // namespace ddafft", ""]
     with open(path, "w") as f:
         f.write("\n".join(lines))

@@ -138,7 +138,8 @@ int ddafft_set_sum_mode(ddafft_ctx* ctx, int mode);
  * GPU per lattice and right-hand-side count and writes the table the library
  * uses by default, dda-fft_amd/csrc/tuned_table.hpp).  A value set here wins
  * over the table for every later call on this context; 0 (-1 for
- * DDAFFT_TUNE_GRAPH) hands the choice back to the table.
+ * DDAFFT_TUNE_GRAPH and DDAFFT_TUNE_FUSE_ZY) hands the choice back to the
+ * table.
  *   DDAFFT_TUNE_X_RB: the fused X pass works on groups of 1, 2 or 4
  *     right-hand sides that share each Â value read;
  *   DDAFFT_TUNE_X_BLOCKS: each X block takes a tile of (J, L) columns and a
@@ -151,6 +152,10 @@ int ddafft_set_sum_mode(ddafft_ctx* ctx, int mode);
 #define DDAFFT_TUNE_X_BLOCKS 0
 #define DDAFFT_TUNE_X_RB 1
 #define DDAFFT_TUNE_GRAPH 2
+/* DDAFFT_TUNE_FUSE_ZY: 1 runs the z- and y-forward transforms of one
+ * i-plane in one kernel where n_j, n_l <= 64, 0 always as two passes, -1
+ * (default) as the table says. */
+#define DDAFFT_TUNE_FUSE_ZY 3
 int ddafft_set_tuning(ddafft_ctx* ctx, int key, int value);
 /* One instance's solve for ddafft_solve_ex / ddafft_solve_batch: nrhs
  * right-hand sides, each with its own incidence (Monte-Carlo orientations x

@@ -40,6 +40,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 #define __shared__ static
+// instruction-scheduling fence: no effect on results
+inline void __builtin_amdgcn_sched_barrier(int) {}
 
 struct float2 {
     float x, y;

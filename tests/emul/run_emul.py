@@ -92,6 +92,7 @@ def main():
     ok &= check_reference_sums(orc)
     ok &= check_noalloc(orc)
     ok &= check_graph(orc)
+    ok &= check_fuse(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -123,6 +124,48 @@ def check_exact_phases(orc):
             ok &= check(res[p]["steps"] == o["steps"] and np.array_equal(bits(ctx.dipoles(p)), bits(o["p"])),
                         "exact phases, rotated incidence, pol %d" % p)
     return ok
+
+def check_fuse(orc):
+    """The fused z/y-forward kernel and the two-pass ZF + YF path, each forced
+    on, against the oracle bit for bit: convolution, matvec and the four
+    solvers (every prologue mode of the first pass)."""
+    rng = np.random.default_rng(17)
+    ok = True
+    dims, kd = (4, 8, 16), np.float32(0.55)
+    occ, _ = orc.target_ellipsoid(dims, 1.0, 1.0)
+    occ = (occ * (1 + (rng.random(dims) < 0.3))).astype(np.int8)
+    mv = np.array([0, 1, 1], np.int32)
+    alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.01j), orc.polarizability(1, kd, 1.7 + 0.1j)],
+                     np.complex64)
+    A = orc.build_ahat(dims, kd)
+    x = rnd(rng, dims + (3,))
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = directions(np.arange(0, 181, 45), 0.0)
+    refs = {}
+    for fuse in (1, 0):
+        with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+            ctx.set_target(occ, mv)
+            ctx.set_tuning(fuse_zy=fuse)
+            for conj in (False, True):
+                y = ctx.convolution(x, conj)
+                ok &= check(np.array_equal(bits(y), bits(orc.convolution(A, x, conj))),
+                            "fuse %d conv %d" % (fuse, conj))
+                y = ctx.matvec(alpha, x, conj)
+                ok &= check(np.array_equal(bits(y), bits(orc.matvec(A, occ, mv, alpha, x, conj))),
+                            "fuse %d matvec %d" % (fuse, conj))
+            for solver in (0, 1, 2, 3):
+                res = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=5, err_max=1e-6, tau=0.5)
+                for pol in range(2):
+                    if (solver, pol) not in refs:
+                        refs[solver, pol] = orc.dda_fft(solver, occ, mv, alpha, kd, kinc, einc[pol], A, dirs,
+                                                        max_step=5, err_max=1e-6, tau=0.5, return_p=True)
+                    ref = refs[solver, pol]
+                    good = (res[pol]["steps"] == ref["steps"]
+                            and np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])))
+                    ok &= check(good, "fuse %d solve solver %d pol %d" % (fuse, solver, pol))
+    return ok
+
 
 def check_long_axes(orc):
     """256- and 128-point axes (the transform sizes of the 256^3 workload, 512

@@ -275,3 +275,57 @@ def test_graph_replays_match_launches(orc32, solver):
     ref = orc32.dda_fft(solver, occ, mv, alpha, kd, kinc, einc[0], A, dirs, max_step=steps, err_max=tol, tau=0.5,
                         return_p=True)
     assert np.array_equal(bits(P0[0]), bits(ref["p"]))
+
+
+ZY_PLANES = [(nj, nl) for nj in (8, 16, 32, 64) for nl in (8, 16, 32, 64)]
+
+
+@pytest.mark.parametrize("plane", ZY_PLANES)
+def test_zy_forward_fused_and_split_bitexact(orc32, plane):
+    """The z/y-forward transforms as one kernel per i-plane (every plane shape
+    it is built for) and as the two passes ZF + YF: convolution, conjugate
+    convolution and matvec bit-identical to the oracle either way."""
+    dims = (4,) + plane
+    rng = np.random.default_rng(plane[0] * 131 + plane[1])
+    kd = np.float32(0.4)
+    x = rnd(rng, dims + (3,))
+    occ = (rng.random(dims) < 0.7).astype(np.int8)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, 1.5 + 0.02j)], np.complex64)
+    print("zy forward", dims, flush=True)
+    with ddafft.Context(dims, kd, nrhs_max=1) as ctx:
+        ctx.set_target(occ, mv)
+        A = ctx.ahat()
+        for fuse in (1, 0):
+            ctx.set_tuning(fuse_zy=fuse)
+            for conj in (False, True):
+                y = ctx.convolution(x, conj=conj)
+                assert np.array_equal(bits(y), bits(orc32.convolution(A, x, conj=conj))), (fuse, conj)
+            y = ctx.matvec(alpha, x, True)
+            assert np.array_equal(bits(y), bits(orc32.matvec(A, occ, mv, alpha, x, True))), fuse
+
+
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_zy_forward_solvers_bitexact(orc32, fuse):
+    """Every prologue mode of the first pass (the four solvers' AXPYs folded
+    into it) through the fused kernel and through ZF: dipoles bit-identical
+    to the oracle's."""
+    dims, kd = (8, 32, 64), np.float32(0.35)
+    occ = target(orc32, dims, 1.0, 1.0)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, 1.6 + 0.01j)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = directions(np.arange(0, 181, 30))
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        ctx.set_tuning(fuse_zy=fuse)
+        A = ctx.ahat()
+        for solver in (0, 1, 2, 3):
+            print("zy solve", fuse, solver, flush=True)
+            res = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=6, err_max=1e-6, tau=0.5)
+            for pol in range(2):
+                ref = orc32.dda_fft(solver, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=6,
+                                    err_max=1e-6, tau=0.5, return_p=True)
+                assert res[pol]["steps"] == ref["steps"]
+                assert np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])), (solver, pol)
