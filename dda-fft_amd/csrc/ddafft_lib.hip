@@ -451,7 +451,7 @@ bool zyf_launch(ddafft_ctx* c, const Dims& d, const ProArgs& pa, SolveState* st,
     const dim3 grid(d.ni, C);
 #define DDA_ZYF(NJ, NL)                                                                                        \
     case NJ * 1024 + NL:                                                                                       \
-        hipLaunchKernelGGL((k_zyf<NJ, NL>), grid, kBlock, 0, s, d, pa, st, c->dipf, c->b2, c->wpre_l, c->tw_l, \
+        hipLaunchKernelGGL((k_zyf<NJ, NL>), grid, (ZyfShape<NJ, NL>::BT), 0, s, d, pa, st, c->dipf, c->b2, c->wpre_l, c->tw_l, \
                            c->wpre_j, c->tw_j);                                                                \
         return true;
     switch (d.nj * 1024 + d.nl) {

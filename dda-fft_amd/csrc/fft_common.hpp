@@ -268,7 +268,7 @@ __device__ __forceinline__ void lane_pos(unsigned f, unsigned& line, unsigned& b
     }
 }
 
-template <int N, int NL, int LS, int T>
+template <int N, int NL, int LS, int T, int BT = kBlock>
 struct FixedStage {
     static __device__ __forceinline__ void run(float2* s, const float2* __restrict__ tw) {
         if constexpr (T < Plan<N>::NST) {
@@ -278,13 +278,13 @@ struct FixedStage {
             constexpr int span = H * R;
             constexpr int step = N / span;
             constexpr int W = NL * (N / R);
-            constexpr int IT = (W + kBlock - 1) / kBlock;
+            constexpr int IT = (W + BT - 1) / BT;
             constexpr int G = LaneGroup<N, NL, LS, T>::G;
             static_assert((N / R) % G == 0, "lane group divides the butterflies of a line");
 #pragma unroll
             for (int it = 0; it < IT; ++it) {
-                const unsigned f = threadIdx.x + it * kBlock;
-                if (W % kBlock == 0 || f < (unsigned)W) {
+                const unsigned f = threadIdx.x + it * BT;
+                if (W % BT == 0 || f < (unsigned)W) {
                     unsigned line, b;
                     lane_pos<G, NL>(f, line, b);
                     const unsigned g = b / H, i = b % H;
@@ -310,24 +310,25 @@ struct FixedStage {
                 }
             }
             __syncthreads();
-            FixedStage<N, NL, LS, T + 1>::run(s, tw);
+            FixedStage<N, NL, LS, T + 1, BT>::run(s, tw);
         }
     }
 };
 
-template <int N, int NL, int LS>
+// BT: threads per block
+template <int N, int NL, int LS, int BT = kBlock>
 __device__ __forceinline__ void fft_lds_fixed(float2* s, const float2* __restrict__ tw) {
     constexpr int K = Plan<N>::K;
     constexpr int R0 = plan_radix(K, 0);
     constexpr int W = NL * (N / R0);
-    constexpr int IT = (W + kBlock - 1) / kBlock;
+    constexpr int IT = (W + BT - 1) / BT;
     constexpr int G = LaneGroup<N, NL, LS, 0>::G;
     static_assert((N / R0) % G == 0, "lane group divides the butterflies of a line");
     float2 z[IT][R0];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const unsigned f = threadIdx.x + it * kBlock;
-        if (W % kBlock == 0 || f < (unsigned)W) {
+        const unsigned f = threadIdx.x + it * BT;
+        if (W % BT == 0 || f < (unsigned)W) {
             unsigned line, b;
             lane_pos<G, NL>(f, line, b);
             const float2* v = s + line * LS + rest_natural<N>(b);
@@ -339,8 +340,8 @@ __device__ __forceinline__ void fft_lds_fixed(float2* s, const float2* __restric
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const unsigned f = threadIdx.x + it * kBlock;
-        if (W % kBlock == 0 || f < (unsigned)W) {
+        const unsigned f = threadIdx.x + it * BT;
+        if (W % BT == 0 || f < (unsigned)W) {
             unsigned line, b;
             lane_pos<G, NL>(f, line, b);
             float2* v = s + line * LS + b * R0;
@@ -349,7 +350,7 @@ __device__ __forceinline__ void fft_lds_fixed(float2* s, const float2* __restric
         }
     }
     __syncthreads();
-    FixedStage<N, NL, LS, 1>::run(s, tw);
+    FixedStage<N, NL, LS, 1, BT>::run(s, tw);
 }
 
 // MAXL: the most lines any call of this instance passes (sizes the registers

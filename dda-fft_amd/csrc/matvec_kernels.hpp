@@ -5,14 +5,15 @@
 // HBM layouts (float2 = complex64, C = 3*nrhs component planes):
 //   field x, out : [C][ni][nj][nl]            (one plane per component)
 //   B1 / B4      : [C][ni][2nl'][nj] / [C][ni][nj][2nl']
-//   B2 / B3      : [C][ni][2nj][2nl']         (plane-major: one i-plane after another)
+//   B2 / B3      : [C][2nj][ni][2nl']         (an X tile's i rows 2nl apart)
 //   Â            : [2nj][2nl][2ni][6]         (e00 e01 e02 e11 e12 e22)
 // L' is the z-frequency in parity-split order: L' = L / 2 for even L and
 // n_l + L / 2 for odd L, the order in which the pruned z transform produces
 // them (two n_l-point transforms, fft_func.cpp:447-462); the Z-inverse pass
 // puts them back in natural order in LDS.  Lines along l, j and L' are
-// contiguous; the X pass reads and writes runs of its tile's L' columns in
-// every i-plane.
+// contiguous; the z/y passes write and read whole runs of L' in each row J
+// of an i-plane, and the X pass reads and writes runs of its tile's L'
+// columns in the rows i of one J, 2 n_l float2 apart.
 //
 //   ZF  z forward, pruned (fft_func.cpp:442-463)      x  -> B1
 //   YF  y forward, pruned (fft_func.cpp:484-504)      B1 -> B2
@@ -227,19 +228,23 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
 
 template <int NJ, int NL>
 struct ZyfShape {
+    // threads per block: four plane cells per thread from 32 x 32 up (a 64 x 64
+    // plane takes 16 waves), at least 256
+    static constexpr int BT = NJ * NL / 4 < 256 ? 256 : NJ * NL / 4 > 1024 ? 1024 : NJ * NL / 4;
     static constexpr int LSZ = NL + kLpad;                      // z lines: NJ rows of NL points
     static constexpr int LSY = NJ + kLpad;                      // y lines: NL columns of NJ points
     static constexpr int WORDS = NJ * LSZ > NL * LSY ? NJ * LSZ : NL * LSY;
-    static constexpr int E = (NJ * NL + kBlock - 1) / kBlock;   // plane cells per thread
+    static constexpr int E = (NJ * NL + BT - 1) / BT;   // plane cells per thread
+    static_assert(E <= 4, "the plane stays in registers across both parities");
 };
 
 template <int NJ, int NL>
-__global__ __launch_bounds__(kBlock) void k_zyf(Dims d, ProArgs pa, const SolveState* __restrict__ st,
+__global__ __launch_bounds__((ZyfShape<NJ, NL>::BT)) void k_zyf(Dims d, ProArgs pa, const SolveState* __restrict__ st,
                                                 const int8_t* __restrict__ dipf, float2* __restrict__ b2,
                                                 const float2* __restrict__ wpre_l, const float2* __restrict__ tw_l,
                                                 const float2* __restrict__ wpre_j, const float2* __restrict__ tw_j) {
     using S = ZyfShape<NJ, NL>;
-    constexpr int E = S::E, LSZ = S::LSZ, LSY = S::LSY;
+    constexpr int E = S::E, LSZ = S::LSZ, LSY = S::LSY, BT = S::BT;
     constexpr int NP = NJ * NL;
     constexpr int LNL = ilog2c(NL), LNJ = ilog2c(NJ);
     __shared__ float2 w[S::WORDS];
@@ -259,8 +264,8 @@ __global__ __launch_bounds__(kBlock) void k_zyf(Dims d, ProArgs pa, const SolveS
     // KC cells per thread: the loads of a chunk are issued before the first is
     // used, and a chunk's operands are dead before the next chunk's arrive
     // (all at once they took the kernel to 198 VGPRs at 64 x 64)
-    for (int f = threadIdx.x; f < NL; f += kBlock) stl[f] = tw_l[f];
-    for (int f = threadIdx.x; f < NJ; f += kBlock) stj[f] = tw_j[f];
+    for (int f = threadIdx.x; f < NL; f += BT) stl[f] = tw_l[f];
+    for (int f = threadIdx.x; f < NJ; f += BT) stj[f] = tw_j[f];
     float2 v[E];
     constexpr int KC = E < 4 ? E : 4;
     const int mode = pa.mode;
@@ -270,10 +275,10 @@ __global__ __launch_bounds__(kBlock) void k_zyf(Dims d, ProArgs pa, const SolveS
         float2 va[KC], vb[KC], vd[KC], vp[KC];
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
-            const int f = threadIdx.x + (q0 + k) * kBlock;
+            const int f = threadIdx.x + (q0 + k) * BT;
             dip[k] = false;
             va[k] = vb[k] = vd[k] = vp[k] = make_float2(0.f, 0.f);
-            if (NP % kBlock == 0 || f < NP) {
+            if (NP % BT == 0 || f < NP) {
                 dip[k] = dipf[cell0 + f] != 0;
                 va[k] = pa.A[cb + f];
                 if (mode != PRO_COPY) vb[k] = pa.B[cb + f];
@@ -284,8 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_zyf(Dims d, ProArgs pa, const SolveS
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
             const int q = q0 + k;
-            const int f = threadIdx.x + q * kBlock;
-            if (NP % kBlock == 0 || f < NP) {
+            const int f = threadIdx.x + q * BT;
+            if (NP % BT == 0 || f < NP) {
                 float2 x;
                 switch (mode) {
                     case PRO_LC1:
@@ -317,39 +322,30 @@ __global__ __launch_bounds__(kBlock) void k_zyf(Dims d, ProArgs pa, const SolveS
         }
         __builtin_amdgcn_sched_barrier(0);   // keep the next chunk's loads behind this chunk's use
     }
-    float2* dst = b2 + ((long long)c * d.ni + i) * (2 * NJ) * (2 * NL);   // plane (c, i) of B2 [J][L']
-    // where the prologue left the matvec input (this thread's own cells: it
-    // reads back what it wrote), so that the plane is not held in registers
-    // across the first parity
-    const float2* xin = (mode == PRO_LC4 || mode == PRO_LC2) ? pa.B : pa.A;
+    // row J of plane (c, i) of B2 [C][J][i][L'] at dst + J * rowj
+    float2* dst = b2 + ((long long)c * (2 * NJ) * d.ni + i) * (2 * NL);
+    const int rowj = d.ni * (2 * NL);
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        if (p) {
-            __syncthreads();   // the previous parity's last reads of w are done
-#pragma unroll
-            for (int q = 0; q < E; ++q) {
-                const int f = threadIdx.x + q * kBlock;
-                if (NP % kBlock == 0 || f < NP) v[q] = xin[cb + f];
-            }
-        }
+        if (p) __syncthreads();   // the previous parity's last reads of w are done
         // rows j along l: x (p = 0) or W_2nl^l x (p = 1), as ZF forms them
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (NP % kBlock == 0 || f < NP) {
+            const int f = threadIdx.x + q * BT;
+            if (NP % BT == 0 || f < NP) {
                 const int j = f >> LNL, l = f & (NL - 1);
                 w[j * LSZ + l] = p ? cmul(wpre_l[l], v[q]) : v[q];
             }
         }
         __syncthreads();
-        fft_lds<NL, NJ, LSZ>(w, NJ, stl);
+        fft_lds_fixed<NL, NJ, LSZ, BT>(w, stl);
         // Z_p[j][Lh] into registers, transposed: element f = Lh * NJ + j
         float2 u[E];
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-            const int f = threadIdx.x + q * kBlock;
+            const int f = threadIdx.x + q * BT;
             u[q] = make_float2(0.f, 0.f);
-            if (NP % kBlock == 0 || f < NP) u[q] = w[(f & (NJ - 1)) * LSZ + (f >> LNJ)];
+            if (NP % BT == 0 || f < NP) u[q] = w[(f & (NJ - 1)) * LSZ + (f >> LNJ)];
         }
 #pragma unroll
         for (int qy = 0; qy < 2; ++qy) {
@@ -357,21 +353,21 @@ __global__ __launch_bounds__(kBlock) void k_zyf(Dims d, ProArgs pa, const SolveS
             // columns Lh along j: Z (qy = 0) or W_2nj^j Z (qy = 1), as YF forms them
 #pragma unroll
             for (int q = 0; q < E; ++q) {
-                const int f = threadIdx.x + q * kBlock;
-                if (NP % kBlock == 0 || f < NP) {
+                const int f = threadIdx.x + q * BT;
+                if (NP % BT == 0 || f < NP) {
                     const int lh = f >> LNJ, j = f & (NJ - 1);
                     w[lh * LSY + j] = qy ? cmul(wpre_j[j], u[q]) : u[q];
                 }
             }
             __syncthreads();
-            fft_lds<NJ, NL, LSY>(w, NL, stj);
+            fft_lds_fixed<NJ, NL, LSY, BT>(w, stj);
             // row J = 2 Jh + qy of the plane, columns p NL .. p NL + NL - 1
 #pragma unroll
             for (int q = 0; q < E; ++q) {
-                const int f = threadIdx.x + q * kBlock;
-                if (NP % kBlock == 0 || f < NP) {
+                const int f = threadIdx.x + q * BT;
+                if (NP % BT == 0 || f < NP) {
                     const int jh = f >> LNL, lh = f & (NL - 1);
-                    dst[(long long)(2 * jh + qy) * (2 * NL) + p * NL + lh] = w[lh * LSY + jh];
+                    dst[(2 * jh + qy) * rowj + p * NL + lh] = w[lh * LSY + jh];
                 }
             }
         }
@@ -423,7 +419,9 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
         for (int f = threadIdx.x; f < NJ; f += kBlock) stw[f] = tw[f];
         __syncthreads();
         fft_lds<NJ, 2 * TI, LS>(s, 2 * ti, stw);
-        float2* dst = b2 + ((long long)c * d.ni + i) * (2 * NJ) * NL2 + L0;   // row J of plane (c, i), from L0
+        // row J of plane (c, i) of B2 [C][J][i][L'], from L0
+        float2* dst = b2 + ((long long)c * (2 * NJ) * d.ni + i) * NL2 + L0;
+        const int rowj = d.ni << lnl2;
         constexpr int NST = (2 * NJ * TI + kBlock - 1) / kBlock;
 #pragma unroll
         for (int q = 0; q < NST; ++q) {
@@ -431,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
             if (f < 2 * NJ * ti) {
                 const int J = f >> lti, t = f - (J << lti);
                 const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
-                dst[(J << lnl2) + t] = v;
+                dst[J * rowj + t] = v;
             }
         }
     };
@@ -536,9 +534,9 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
             // block's lines lie at 32-bit offsets from one 64-bit base
             constexpr int NLD = (CB * TJ * NI + kBlock - 1) / kBlock;
             const int nld = CB * tj * NI;
-            // runs of tj columns in row J of the (cc, i) planes
-            const float2* src = b2 + ((long long)(rhs0 * 3) * NI * NJ2 + J) * NL2 + L0;
-            const int lrow = __builtin_ctz(NJ2) + __builtin_ctz(NL2);
+            // runs of tj columns in row J of the (cc, i) planes: B2 [C][J][i][L']
+            const float2* src = b2 + ((long long)(rhs0 * 3) * NJ2 + J) * NI * NL2 + L0;
+            const int lcomp = __builtin_ctz(NJ2) + LNI + __builtin_ctz(NL2), lnl2 = __builtin_ctz(NL2);
             float2 v[NLD];
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {
@@ -546,7 +544,7 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
                 v[q] = make_float2(0.f, 0.f);
                 if (f < nld) {
                     const int t = f & (tj - 1), r = f >> ltj, cc = r >> LNI;   // r = cc * NI + i
-                    if (act[cc / 3]) v[q] = src[(r << lrow) + t];
+                    if (act[cc / 3]) v[q] = src[(cc << lcomp) + ((r & (NI - 1)) << lnl2) + t];
                 }
             }
             // twiddles staged after the first field loads are issued, so that the
@@ -604,8 +602,9 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
         __syncthreads();
         fft_lds<NI2, CB * TJ, LS2>(s, CB * tj, stw2);
         {
-            float2* dst = b3 + (long long)(rhs0 * 3) * NI * ((long long)NL2 * NJ2) + (long long)J * NL2 + L0;
-            const int lrow = __builtin_ctz(NL2) + __builtin_ctz(NJ2);
+            // B3 [C][J][i][L'], as B2
+            float2* dst = b3 + ((long long)(rhs0 * 3) * NJ2 + J) * NI * NL2 + L0;
+            const int lcomp = __builtin_ctz(NJ2) + LNI + __builtin_ctz(NL2), lnl2 = __builtin_ctz(NL2);
             constexpr int NST = (CB * TJ * NI + kBlock - 1) / kBlock;
             const int nst = CB * tj * NI;
 #pragma unroll
@@ -613,7 +612,7 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
                 const int f = threadIdx.x + q * kBlock;
                 if (f < nst) {
                     const int t = f & (tj - 1), r = f >> ltj, i = r & (NI - 1), cc = r >> LNI;
-                    if (act[cc / 3]) dst[(r << lrow) + t] = s[(cc * tj + t) * LS2 + i];
+                    if (act[cc / 3]) dst[(cc << lcomp) + (i << lnl2) + t] = s[(cc * tj + t) * LS2 + i];
                 }
             }
         }
@@ -673,15 +672,17 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
         const int tl = tile_arg, nlt = NL2 / tl;
         const int i = blockIdx.x / nlt, L0 = (blockIdx.x - i * nlt) * tl;
         const int ltl0 = __builtin_ctz(tl), lnl20 = __builtin_ctz(NL2);
-        // columns L0 .. L0 + tl - 1 of plane (c, i) of B3 [J][L']: runs of tl in every row J
-        const float2* src = b3 + ((long long)c * d.ni + i) * NJ2 * NL2 + L0;
+        // columns L0 .. L0 + tl - 1 of plane (c, i) of B3 [C][J][i][L']: runs of
+        // tl in every row J
+        const float2* src = b3 + ((long long)c * NJ2 * d.ni + i) * NL2 + L0;
+        const int rowj = d.ni << lnl20;
         {
             constexpr int NLD = (TL * NJ2 + kBlock - 1) / kBlock;
             float2 v[NLD];
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {
                 const int f = threadIdx.x + q * kBlock;
-                if (f < tl * NJ2) v[q] = src[((f >> ltl0) << lnl20) + (f & (tl - 1))];
+                if (f < tl * NJ2) v[q] = src[(f >> ltl0) * rowj + (f & (tl - 1))];
             }
 #pragma unroll
             for (int q = 0; q < NLD; ++q) {

@@ -17,16 +17,16 @@ struct TunedEntry {
 constexpr int kTunedBatchWidth = 16;
 
 constexpr TunedEntry kTuned[] = {
-    {32, 32, 32, 2, 1, 2048, 0, 1},   // X pass 12.7 us, solve 0.913 ms
-    {64, 64, 64, 2, 2, 1 << 30, 0, 1},   // X pass 49.1 us, solve 2.520 ms
-    {128, 128, 128, 2, 2, 1 << 30, 0, 0},   // X pass 483.3 us, solve 9.271 ms
-    {256, 256, 256, 2, 1, 1 << 30, 0, 0},   // X pass 5536.2 us, solve 94.661 ms
-    {32, 32, 32, 16, 2, 8192, 0, 1},   // X pass 38.2 us, solve 2.538 ms
-    {32, 32, 64, 16, 2, 1 << 30, 0, 1},   // X pass 73.0 us, solve 4.333 ms
-    {64, 64, 64, 16, 2, 1 << 30, 0, 1},   // X pass 311.5 us, solve 16.310 ms
-    {32, 32, 64, 2, 2, 1 << 30, 0, 1},   // X pass 18.6 us, solve 1.168 ms
-    {64, 64, 64, 4, 2, 1 << 30, 0, 1},   // X pass 96.6 us, solve 4.522 ms
-    {128, 128, 128, 4, 2, 1 << 30, 1, 0},   // X pass 939.5 us, solve 17.606 ms
+    {32, 32, 32, 2, 1, 2048, 0, 1},   // X pass 12.6 us, solve 0.881 ms
+    {64, 64, 64, 2, 2, 1 << 30, 0, 0},   // X pass 46.8 us, solve 2.583 ms
+    {128, 128, 128, 2, 2, 1 << 30, 0, 0},   // X pass 433.0 us, solve 8.857 ms
+    {256, 256, 256, 2, 1, 1 << 30, 0, 0},   // X pass 5041.1 us, solve 91.992 ms
+    {32, 32, 32, 16, 2, 8192, 0, 1},   // X pass 37.4 us, solve 2.343 ms
+    {32, 32, 64, 16, 2, 1 << 30, 0, 1},   // X pass 72.2 us, solve 4.093 ms
+    {64, 64, 64, 16, 2, 1 << 30, 0, 1},   // X pass 322.9 us, solve 15.907 ms
+    {32, 32, 64, 2, 2, 1 << 30, 0, 1},   // X pass 18.5 us, solve 1.094 ms
+    {64, 64, 64, 4, 2, 8192, 0, 0},   // X pass 95.1 us, solve 4.682 ms
+    {128, 128, 128, 4, 2, 1 << 30, 0, 0},   // X pass 796.1 us, solve 17.004 ms
 };
 
 }  // namespace ddafft
