@@ -17,9 +17,14 @@
 namespace ddafft {
 
 // Octant table oct[a][b][c][6] with a in [0, ni], b in [0, nj], c in [0, nl]
-// -> full grid g[I][J][L][6], I < 2ni, J < 2nj, L < 2nl.
+// -> full grid g[I][J][L][6], I < 2ni, J < 2nj, L < 2nl.  zero_nyquist: the
+// cells on the planes I = ni, J = nj, L = nl (r = -n d along an axis, where
+// the padded grid has no mirror cell) are set to zero; A is then exactly even
+// or odd along every axis, and so is its transform.  Those cells never pair a
+// source with a target inside the lattice (|r_x| <= (ni - 1) d there), so
+// the convolution is mathematically unchanged.
 __global__ __launch_bounds__(kBlock) void k_ahat_expand(int ni, int nj, int nl, const float2* __restrict__ oct,
-                                                        float2* __restrict__ g) {
+                                                        float2* __restrict__ g, int zero_nyquist) {
     const long long NI2 = 2 * ni, NJ2 = 2 * nj, NL2 = 2 * nl;
     const long long total = NI2 * NJ2 * NL2;
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
@@ -36,8 +41,27 @@ __global__ __launch_bounds__(kBlock) void k_ahat_expand(int ni, int nj, int nl, 
         if (sx != sy) v[1] = cneg(v[1]);
         if (sx != sz) v[2] = cneg(v[2]);
         if (sy != sz) v[4] = cneg(v[4]);
+        if (zero_nyquist && (I == ni || J == nj || L == nl))
+#pragma unroll
+            for (int e = 0; e < 6; ++e) v[e] = make_float2(0.f, 0.f);
 #pragma unroll
         for (int e = 0; e < 6; ++e) g[k * 6 + e] = v[e];
+    }
+}
+
+// The octant J <= nj, L <= nl, I <= ni of the transformed grid full[J][L][I][6]
+// (2nj x 2nl x 2ni) -> oct[J][L][I][6] ((nj+1) x (nl+1) x (ni+1)).
+__global__ __launch_bounds__(kBlock) void k_ahat_octant(int ni, int nj, int nl, const float2* __restrict__ full,
+                                                        float2* __restrict__ oct) {
+    const long long total = (long long)(nj + 1) * (nl + 1) * (ni + 1);
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int I = (int)(k % (ni + 1));
+        const long long jl = k / (ni + 1);
+        const int L = (int)(jl % (nl + 1)), J = (int)(jl / (nl + 1));
+        const float2* src = full + (((long long)J * (2 * nl) + L) * (2 * ni) + I) * 6;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) oct[k * 6 + e] = src[e];
     }
 }
 

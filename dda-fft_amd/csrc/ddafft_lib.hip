@@ -231,6 +231,7 @@ struct ddafft_ctx {
     float2* fftab = nullptr;           // far-field per-axis phases [ndirs][ni + nj + nl]
     size_t fftab_cap = 0;
     int sum_mode = 0;                  // observables: 0 FP64 sums, 1 the reference's sequential float sums
+    int ahat_mode = 0;                 // 0: full Â as the reference builds it; 1: one octant (Nyquist planes zeroed)
     // launch schedule, 0 / -1: from the tuned table, else the built-in rule
     int x_blocks = 0;                  // X pass: blocks per launch up to which right-hand-side groups are spread
     int x_rb = 0;                      // X pass: right-hand sides per tile
@@ -524,18 +525,24 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
         const int ysets = std::max(1, std::min(ngroups, (int)std::min<long long>(((long long)sch.x_blocks + ntile - 1) / ntile, 1 << 20)));
         const int gpb = (ngroups + ysets - 1) / ysets;
         dim3 grid(ntile, (ngroups + gpb - 1) / gpb);
-#define DDA_KX(SW, RB, RUN)                                                                               \
-    SW(d.ni, hipLaunchKernelGGL((k_x<NN, RB, RUN>), grid, kBlock, 0, s, d, st, c->b2, c->b3, c->ahat, conj,     \
+#define DDA_KX(SW, RB, RUN, CP)                                                                           \
+    SW(d.ni, hipLaunchKernelGGL((k_x<NN, RB, RUN, CP>), grid, kBlock, 0, s, d, st, c->b2, c->b3, c->ahat, conj, \
                                 c->wpre_i, c->tw_i, c->tw_2i, gpb))
+#define DDA_KX_RB(RUN, CP)                              \
+    {                                                   \
+        if (rb == 4) DDA_KX(DDA_SWITCH_X, 4, RUN, CP)      \
+        else if (rb == 2) DDA_KX(DDA_SWITCH_X, 2, RUN, CP) \
+        else DDA_KX(DDA_SWITCH_SMALL, 1, RUN, CP)          \
+    }
+        const bool compact = c->ahat_mode == 1;
         if (gpb > 1) {
-            if (rb == 4) DDA_KX(DDA_SWITCH_X, 4, true)
-            else if (rb == 2) DDA_KX(DDA_SWITCH_X, 2, true)
-            else DDA_KX(DDA_SWITCH_SMALL, 1, true)
+            if (compact) DDA_KX_RB(true, true)
+            else DDA_KX_RB(true, false)
         } else {
-            if (rb == 4) DDA_KX(DDA_SWITCH_X, 4, false)
-            else if (rb == 2) DDA_KX(DDA_SWITCH_X, 2, false)
-            else DDA_KX(DDA_SWITCH_SMALL, 1, false)
+            if (compact) DDA_KX_RB(false, true)
+            else DDA_KX_RB(false, false)
         }
+#undef DDA_KX_RB
 #undef DDA_KX
     }
     if (ev) HIPCHECK(hipEventRecord(ev[3], s));
@@ -555,21 +562,28 @@ int launch_matvec(ddafft_ctx* c, int nrhs, const ProArgs& pa, SolveState* st, in
     return DDAFFT_OK;
 }
 
+// Â into c->ahat: the full padded grid [2nj][2nl][2ni][6] (mode 0), or one
+// octant [nj+1][nl+1][ni+1][6] of the transform of A with its Nyquist planes
+// zeroed (mode 1), which the X pass mirrors.
 int build_ahat(ddafft_ctx* c) {
     const int ni = c->d.ni, nj = c->d.nj, nl = c->d.nl;
+    const bool compact = c->ahat_mode == 1;
     std::vector<float2> oct;
     green_octant(ni, nj, nl, c->kd, oct);
-    float2 *doct = nullptr, *grid = nullptr, *tw = nullptr;
+    float2 *doct = nullptr, *grid = nullptr, *tw = nullptr, *full = nullptr;
     const size_t P = (size_t)8 * ni * nj * nl;
     int rc = dev_alloc(&doct, oct.size());
     if (!rc) rc = dev_alloc(&grid, P * 6);
     if (!rc) rc = dev_alloc(&tw, (size_t)(ni + nj + nl));
+    if (!rc && compact) rc = dev_alloc(&full, P * 6);
     if (rc) {
         hipFree(doct);
         hipFree(grid);
         hipFree(tw);
+        hipFree(full);
         return rc;
     }
+    float2* dst = compact ? full : c->ahat;
     std::vector<float2> wi = fourier_phase(ni), wj = fourier_phase(nj), wl = fourier_phase(nl);
     hipStream_t s = c->stream;
     hipError_t e = hipMemcpyAsync(doct, oct.data(), oct.size() * sizeof(float2), hipMemcpyHostToDevice, s);
@@ -579,7 +593,7 @@ int build_ahat(ddafft_ctx* c) {
     if (e == hipSuccess) {
         const long long total = (long long)P;
         const int blocks = (int)std::min<long long>((total + kBlock - 1) / kBlock, 65536);
-        hipLaunchKernelGGL(k_ahat_expand, blocks, kBlock, 0, s, ni, nj, nl, doct, grid);
+        hipLaunchKernelGGL(k_ahat_expand, blocks, kBlock, 0, s, ni, nj, nl, doct, grid, compact ? 1 : 0);
         const float norma = (float)(1. / ((2 * ni) * (2 * nj) * (2 * nl)));
         auto axis_pass = [&](int axis, int na, const float2* w) -> int {
             int ext = axis == 0 ? 2 * nj : 2 * nl;
@@ -597,18 +611,24 @@ int build_ahat(ddafft_ctx* c) {
                 default: return fail(DDAFFT_EINVAL, "unsupported axis length");
             }
             dim3 g(outer * (ext / tt));
-            DDA_SWITCH_LARGE(na, hipLaunchKernelGGL((k_legacy_axis<NN>), g, kBlock, 0, s, ni, nj, nl, axis, grid, c->ahat, w, norma));
+            DDA_SWITCH_LARGE(na, hipLaunchKernelGGL((k_legacy_axis<NN>), g, kBlock, 0, s, ni, nj, nl, axis, grid, dst, w, norma));
             return DDAFFT_OK;
         };
         rc = axis_pass(0, 2 * nl, tw + ni + nj);
         if (!rc) rc = axis_pass(1, 2 * nj, tw + ni);
         if (!rc) rc = axis_pass(2, 2 * ni, tw);
+        if (!rc && compact) {
+            const long long noct = (long long)(nj + 1) * (nl + 1) * (ni + 1);
+            const int ob = (int)std::min<long long>((noct + kBlock - 1) / kBlock, 65536);
+            hipLaunchKernelGGL(k_ahat_octant, ob, kBlock, 0, s, ni, nj, nl, full, c->ahat);
+        }
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(s);
     }
     hipFree(doct);
     hipFree(grid);
     hipFree(tw);
+    hipFree(full);
     if (rc) return rc;
     if (e != hipSuccess) return fail(DDAFFT_EHIP, std::string("Â build: ") + hipGetErrorString(e));
     return DDAFFT_OK;
@@ -1018,17 +1038,31 @@ int ddafft_create(int ni, int nj, int nl, float kd, int nrhs_max, ddafft_ctx** o
 
 int ddafft_get_ahat(ddafft_ctx* c, float* out) {
     if (!c || !out) return fail(DDAFFT_EINVAL, "NULL argument");
-    const int NI2 = 2 * c->d.ni, NJ2 = 2 * c->d.nj, NL2 = 2 * c->d.nl;
+    const int ni = c->d.ni, nj = c->d.nj, nl = c->d.nl;
+    const int NI2 = 2 * ni, NJ2 = 2 * nj, NL2 = 2 * nl;
     const size_t P = (size_t)NI2 * NJ2 * NL2;
-    std::vector<float2> h(P * 6);
+    const bool compact = c->ahat_mode == 1;
+    std::vector<float2> h((compact ? (size_t)(ni + 1) * (nj + 1) * (nl + 1) : P) * 6);
     HIPCHECK(hipMemcpyAsync(h.data(), c->ahat, h.size() * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     float2* o = reinterpret_cast<float2*>(out);
     for (int J = 0; J < NJ2; ++J)
         for (int L = 0; L < NL2; ++L)
-            for (int I = 0; I < NI2; ++I)
-                memcpy(&o[(((size_t)I * NJ2 + J) * NL2 + L) * 6], &h[(((size_t)J * NL2 + L) * NI2 + I) * 6],
-                       6 * sizeof(float2));
+            for (int I = 0; I < NI2; ++I) {
+                float2* dst = &o[(((size_t)I * NJ2 + J) * NL2 + L) * 6];
+                if (!compact) {
+                    memcpy(dst, &h[(((size_t)J * NL2 + L) * NI2 + I) * 6], 6 * sizeof(float2));
+                    continue;
+                }
+                // the grid the X pass reads: the octant mirrored, odd components negated
+                const bool fx = I > ni, fy = J > nj, fz = L > nl;
+                const int Ic = fx ? NI2 - I : I, Jc = fy ? NJ2 - J : J, Lc = fz ? NL2 - L : L;
+                memcpy(dst, &h[(((size_t)Jc * (nl + 1) + Lc) * (ni + 1) + Ic) * 6], 6 * sizeof(float2));
+                auto neg = [](float2& v) { v.x = -v.x; v.y = -v.y; };
+                if (fx != fy) neg(dst[1]);
+                if (fx != fz) neg(dst[2]);
+                if (fy != fz) neg(dst[4]);
+            }
     return DDAFFT_OK;
 }
 
@@ -1090,6 +1124,18 @@ int ddafft_set_tuning(ddafft_ctx* c, int key, int value) {
         default:
             return fail(DDAFFT_EINVAL, "unknown tuning key");
     }
+}
+
+int ddafft_set_ahat_mode(ddafft_ctx* c, int mode) {
+    if (!c || mode < 0 || mode > 1) return fail(DDAFFT_EINVAL, "A-hat mode must be 0 or 1");
+    if (mode == c->ahat_mode) return DDAFFT_OK;
+    HIPCHECK(hipStreamSynchronize(c->stream));   // a captured iteration is keyed by the mode and is captured again
+    if (c->ahat) HIPCHECK(hipFree(c->ahat));
+    c->ahat = nullptr;
+    c->ahat_mode = mode;
+    const size_t cells = mode == 1 ? (size_t)(c->d.ni + 1) * (c->d.nj + 1) * (c->d.nl + 1) : (size_t)8 * c->d.n;
+    TRY(dev_alloc(&c->ahat, cells * 6));
+    return build_ahat(c);
 }
 
 int ddafft_set_sum_mode(ddafft_ctx* c, int mode) {
@@ -1546,10 +1592,10 @@ bool run_more(const Run& r) { return r.o <= r.max_step && r.o < r.known_stop; }
 void graph_key(const Run& r, std::vector<char>& out) {
     const ddafft_ctx* c = r.c;
     struct {
-        int solver, nrhs, max_step, x_rb, x_blocks, nmat, fuse;
+        int solver, nrhs, max_step, x_rb, x_blocks, nmat, fuse, ahat_mode;
         float tau;
         double err_max, inv_ndip;
-        const void *hist, *partials, *alpha, *ring, *mv;
+        const void *hist, *partials, *alpha, *ring, *mv, *ahat;
     } k;
     std::memset(&k, 0, sizeof(k));
     k.solver = r.solver;
@@ -1558,6 +1604,7 @@ void graph_key(const Run& r, std::vector<char>& out) {
     k.x_rb = c->x_rb;
     k.x_blocks = c->x_blocks;
     k.fuse = c->fuse_zy;
+    k.ahat_mode = c->ahat_mode;
     k.nmat = c->nmat;
     k.tau = r.tau;
     k.err_max = r.q.fa.err_max;
@@ -1567,6 +1614,7 @@ void graph_key(const Run& r, std::vector<char>& out) {
     k.alpha = c->alpha;
     k.ring = c->ring_dev;
     k.mv = c->mv;
+    k.ahat = c->ahat;
     const char* b = reinterpret_cast<const char*>(&k);
     out.assign(b, b + sizeof(k));
 }

@@ -472,11 +472,14 @@ __device__ __forceinline__ unsigned long long x_active(const Dims& d, const Solv
 // (every lattice with 2 n_j >= TJ), so the tile width is a compile-time
 // constant and the index arithmetic reduces to shifts and masks; otherwise
 // tj = 2 n_j.
-template <int NI, int RB, bool FULL, bool RUN>
+template <int NI, int RB, bool FULL, bool RUN, bool COMPACT>
 __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, int g0, int g1, int tj_small, const float2* __restrict__ b2, float2* __restrict__ b3,
                                        const float2* __restrict__ ahat, int conj, const float2* __restrict__ wpre,
                                        const float2* __restrict__ tw1, const float2* __restrict__ tw2, float2* s,
                                        float2* stw1, float2* stw2) {
+    // conj: the conjugate convolution (Â at -k, conjugated).  COMPACT: Â
+    // stored as one octant (the reference's padded A(r) with its Nyquist
+    // planes zeroed, so that Â is even or odd along every axis).
     constexpr int CB = 3 * RB;                       // component lines per (J, L)
     constexpr int TJ = TileX<NI, RB>::T;
     constexpr int NI2 = 2 * NI;
@@ -504,7 +507,20 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
                 Ia = (NI2 - Ia) & (NI2 - 1);
             }
             // < 6 * 8 N <= 6 * 2^27: a 32-bit offset from the base
-            const float4* ap = reinterpret_cast<const float4*>(ahat + ((Ja * NL2 + La) * NI2 + Ia) * 6u);
+            unsigned cell = (Ja * NL2 + La) * NI2 + Ia;
+            bool fx = false, fy = false, fz = false;
+            if constexpr (COMPACT) {
+                // one octant [J <= nj][L <= nl][I <= NI]: a frequency above the
+                // Nyquist index reads its mirror, the components odd along the
+                // mirrored axes negated
+                const unsigned nj = d.nj, nl = d.nl;
+                fx = Ia > (unsigned)NI;
+                fy = Ja > nj;
+                fz = La > nl;
+                const unsigned Jc = fy ? NJ2 - Ja : Ja, Lc = fz ? NL2 - La : La, Ic = fx ? NI2 - Ia : Ia;
+                cell = (Jc * (nl + 1) + Lc) * (NI + 1) + Ic;
+            }
+            const float4* ap = reinterpret_cast<const float4*>(ahat + cell * 6u);
             const float4 q0 = ap[0], q1 = ap[1], q2 = ap[2];
             a[it][0] = make_float2(q0.x, q0.y);
             a[it][1] = make_float2(q0.z, q0.w);
@@ -512,6 +528,11 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
             a[it][3] = make_float2(q1.z, q1.w);
             a[it][4] = make_float2(q2.x, q2.y);
             a[it][5] = make_float2(q2.z, q2.w);
+            if constexpr (COMPACT) {
+                if (fx != fy) a[it][1] = cneg(a[it][1]);   // xy
+                if (fx != fz) a[it][2] = cneg(a[it][2]);   // xz
+                if (fy != fz) a[it][4] = cneg(a[it][4]);   // yz
+            }
             if (conj)
 #pragma unroll
                 for (int e = 0; e < 6; ++e) a[it][e].y = -a[it][e].y;
@@ -622,7 +643,7 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
 // Grid: x over (J, L) tiles, y over sets of `gpb` consecutive right-hand-side
 // groups of RB (RUN; otherwise gpb = 1); each block reads its tile's Â once
 // for all of its groups.
-template <int NI, int RB, bool RUN>
+template <int NI, int RB, bool RUN, bool COMPACT>
 __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st,
                                               const float2* __restrict__ b2, float2* __restrict__ b3,
                                               const float2* __restrict__ ahat, int conj,
@@ -646,9 +667,9 @@ __global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restri
     // 6571 -> 5688 us.
     const int tj = min(TJ, 2 * d.nl);
     if (tj == TJ)
-        x_tile<NI, RB, true, RUN>(d, amask, g0, g1, TJ, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
+        x_tile<NI, RB, true, RUN, COMPACT>(d, amask, g0, g1, TJ, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
     else
-        x_tile<NI, RB, false, RUN>(d, amask, g0, g1, tj, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
+        x_tile<NI, RB, false, RUN, COMPACT>(d, amask, g0, g1, tj, b2, b3, ahat, conj, wpre, tw1, tw2, s, stw1, stw2);
 }
 
 // ---------------------------------------------------------------------------

@@ -30,7 +30,7 @@ EXPORTS = (
     "ddafft_target_file_read", "ddafft_target_file_write", "ddafft_orientation_mc", "ddafft_rotate",
     "ddafft_target_mixture", "ddafft_agglomerate_count", "ddafft_target_agglomerate",
     "ddafft_host_alloc", "ddafft_host_free", "ddafft_set_sum_mode",
-    "ddafft_set_tuning", "ddafft_solve_noalloc",
+    "ddafft_set_tuning", "ddafft_solve_noalloc", "ddafft_set_ahat_mode",
 )
 
 
@@ -64,6 +64,7 @@ def load(path=LIB_PATH):
     lib.ddafft_set_target.argtypes = [P, P, I, P]
     lib.ddafft_set_directions.argtypes = [P, I, P]
     lib.ddafft_set_sum_mode.argtypes = [P, I]
+    lib.ddafft_set_ahat_mode.argtypes = [P, I]
     lib.ddafft_set_tuning.argtypes = [P, I, I]
     lib.ddafft_convolution.argtypes = [P, P, P, I]
     lib.ddafft_matvec.argtypes = [P, P, P, P, I]
@@ -161,6 +162,12 @@ class Context:
         """0: FP64 sums of the observables (default); 1: the reference's
         sequential float sums in its cell order (ddafft_set_sum_mode)."""
         _check(_lib.ddafft_set_sum_mode(self._h, int(mode)))
+
+    def set_ahat_mode(self, mode):
+        """0: the reference's Â over the whole padded grid (default,
+        bit-identical convolutions); 1: one octant of the Â of A(r) with its
+        Nyquist planes zeroed, mirrored in the X pass (ddafft_set_ahat_mode)."""
+        _check(_lib.ddafft_set_ahat_mode(self._h, int(mode)))
 
     def set_tuning(self, x_blocks=None, x_rb=None, graph=None, fuse_zy=None):
         """Launch schedule (ddafft_set_tuning); None leaves a setting as it

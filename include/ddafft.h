@@ -133,6 +133,19 @@ int ddafft_far_field(ddafft_ctx* ctx, const float* p, int ndirs, const float* di
  * terms (far-field phases: the correctly rounded float of exp(i arg) for the
  * reference's float argument); a parity mode, slower than mode 0. */
 int ddafft_set_sum_mode(ddafft_ctx* ctx, int mode);
+
+/* How Â is stored (replaces the Â of main.cpp:466-476: CreateAMatrix,
+ * dda_func.cpp:85-148, and FFT_3D(cmatrix), fft_func.cpp:317-416; rebuilt
+ * on the call).  mode 0 (default): the reference's Â over the whole padded
+ * grid, 48 B per padded cell; every convolution is bit-identical to the
+ * reference's.  mode 1: the padded A(r) with its Nyquist planes (r = -n d
+ * along any axis, cells that never pair two lattice sites) set to zero, which
+ * makes Â even or odd along every axis; only the octant k <= n of its
+ * transform is stored (1/8 of the memory, and of the X pass's Â reads) and
+ * mirrored in the X pass.  The operator is mathematically the reference's;
+ * results differ from it by float rounding.  ddafft_get_ahat returns the
+ * mirrored grid. */
+int ddafft_set_ahat_mode(ddafft_ctx* ctx, int mode);
 /* Launch schedule of the passes (no counterpart in the reference: the
  * knobs of the autotuner, dda-fft_amd/tune.py, which measures them on the
  * GPU per lattice and right-hand-side count and writes the table the library

@@ -351,13 +351,21 @@ void orc_amatrix(int ni, int nj, int nl, real d0, real *out) {
 }
 
 /* fft_func.cpp:317-416: full unpruned legacy-kernel transform of the 6
-   components, axis l then j then i, times 1/(8N); tables FourierPhase(n). */
-void orc_build_ahat(int ni, int nj, int nl, real d0, real *out) {
+   components, axis l then j then i, times 1/(8N); tables FourierPhase(n).
+   zero_nyquist (no counterpart in the reference; the build's compact-A-hat
+   mode): A(r) is set to zero on the planes i = ni, j = nj, l = nl (r = -n d
+   along an axis) before the transform. */
+void orc_build_ahat_ex(int ni, int nj, int nl, real d0, real *out, int zero_nyquist) {
     int Ni = 2 * ni, Nj = 2 * nj, Nl = 2 * nl, Nm = Ni > Nj ? (Ni > Nl ? Ni : Nl) : (Nj > Nl ? Nj : Nl);
     cmat *A = (cmat *)out;
     cplx *wi = (cplx *)malloc(sizeof(cplx) * ni), *wj = (cplx *)malloc(sizeof(cplx) * nj),
          *wl = (cplx *)malloc(sizeof(cplx) * nl);
     orc_amatrix(ni, nj, nl, d0, out);
+    if (zero_nyquist)
+        for (int i = 0; i < Ni; i++)
+            for (int j = 0; j < Nj; j++)
+                for (int l = 0; l < Nl; l++)
+                    if (i == ni || j == nj || l == nl) memset(&A[idx3(i, j, l, Nj, Nl)], 0, sizeof(cmat));
     orc_fourier_phase(ni, (real *)wi); orc_fourier_phase(nj, (real *)wj); orc_fourier_phase(nl, (real *)wl);
     /* independent lines per axis, spread over threads (per-thread scratch) */
 #define ORC_ALINES(n0, n1, len, w, at)                                              \
@@ -382,6 +390,10 @@ void orc_build_ahat(int ni, int nj, int nl, real d0, real *out) {
     cplx *c = (cplx *)out;
     for (size_t k = 0; k < 6 * P; k++) c[k] = rmul(norma, c[k]);
     free(wi); free(wj); free(wl);
+}
+
+void orc_build_ahat(int ni, int nj, int nl, real d0, real *out) {
+    orc_build_ahat_ex(ni, nj, nl, d0, out, 0);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -75,6 +75,26 @@ class _Base:
         self._f("build_ahat")(_I(ni), _I(nj), _I(nl), self._real_arg(kd), _ptr(out))
         return out
 
+    def build_ahat_compact(self, n, kd):
+        """The Â grid of the compact mode: the transform of A(r) with its
+        Nyquist planes zeroed, kept on the octant k <= n and mirrored to the
+        whole grid, the components odd along a mirrored axis negated."""
+        ni, nj, nl = n
+        z = np.zeros((2 * ni, 2 * nj, 2 * nl, 6), self.cplx)
+        self._f("build_ahat_ex")(_I(ni), _I(nj), _I(nl), self._real_arg(kd), _ptr(z), _I(1))
+        out = np.empty_like(z)
+        I = np.arange(2 * ni)
+        J = np.arange(2 * nj)
+        L = np.arange(2 * nl)
+        fx, fy, fz = I > ni, J > nj, L > nl
+        Ic, Jc, Lc = np.where(fx, 2 * ni - I, I), np.where(fy, 2 * nj - J, J), np.where(fz, 2 * nl - L, L)
+        out[...] = z[Ic][:, Jc][:, :, Lc]
+        FX, FY, FZ = np.meshgrid(fx, fy, fz, indexing="ij")
+        for comp, (a, b) in ((1, (FX, FY)), (2, (FX, FZ)), (4, (FY, FZ))):
+            flip = a != b
+            out[..., comp][flip] = -out[..., comp][flip]
+        return out
+
     def fft3d(self, x):
         x = np.ascontiguousarray(x, self.cplx)
         ni, nj, nl = x.shape[:3]

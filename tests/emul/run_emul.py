@@ -93,6 +93,7 @@ def main():
     ok &= check_noalloc(orc)
     ok &= check_graph(orc)
     ok &= check_fuse(orc)
+    ok &= check_compact_ahat(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -164,6 +165,45 @@ def check_fuse(orc):
                     good = (res[pol]["steps"] == ref["steps"]
                             and np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])))
                     ok &= check(good, "fuse %d solve solver %d pol %d" % (fuse, solver, pol))
+    return ok
+
+
+def check_compact_ahat(orc):
+    """The compact Â mode (octant of the transform of A with its Nyquist
+    planes zeroed): Â, convolutions, matvec and a BiCGStab solve bit for bit
+    against the oracle with that Â, then back to the reference's Â."""
+    rng = np.random.default_rng(23)
+    ok = True
+    for dims in ((4, 8, 16), (8, 4, 4)):
+        kd = np.float32(0.45)
+        occ = (rng.random(dims) < 0.6).astype(np.int8)
+        mv = np.array([0, 1], np.int32)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.05j)], np.complex64)
+        x = rnd(rng, dims + (3,))
+        Ac = orc.build_ahat_compact(dims, kd)
+        with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+            ctx.set_target(occ, mv)
+            ctx.set_ahat_mode(1)
+            ok &= check(np.array_equal(bits(ctx.ahat()), bits(Ac)), "compact ahat %s" % (dims,))
+            for conj in (False, True):
+                y = ctx.convolution(x, conj)
+                ok &= check(np.array_equal(bits(y), bits(orc.convolution(Ac, x, conj))),
+                            "compact conv %s %d" % (dims, conj))
+            y = ctx.matvec(alpha, x, False)
+            ok &= check(np.array_equal(bits(y), bits(orc.matvec(Ac, occ, mv, alpha, x, False))),
+                        "compact matvec %s" % (dims,))
+            kinc = np.array([0, 0, 1], np.float32)
+            einc = np.eye(3, dtype=np.complex64)[:2]
+            dirs = directions(np.arange(0, 181, 45), 0.0)
+            res = ctx.solve(2, alpha, kinc, einc, dirs=dirs, max_step=5, err_max=1e-6)
+            ref = orc.dda_fft(2, occ, mv, alpha, kd, kinc, einc[0], Ac, dirs, max_step=5, err_max=1e-6,
+                              return_p=True)
+            ok &= check(res[0]["steps"] == ref["steps"] and np.array_equal(bits(ctx.dipoles(0)), bits(ref["p"])),
+                        "compact solve %s" % (dims,))
+            ctx.set_ahat_mode(0)
+            A = orc.build_ahat(dims, kd)
+            ok &= check(np.array_equal(bits(ctx.convolution(x, False)), bits(orc.convolution(A, x, False))),
+                        "reference ahat again %s" % (dims,))
     return ok
 
 

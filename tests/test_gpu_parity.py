@@ -329,3 +329,89 @@ def test_zy_forward_solvers_bitexact(orc32, fuse):
                                     err_max=1e-6, tau=0.5, return_p=True)
                 assert res[pol]["steps"] == ref["steps"]
                 assert np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])), (solver, pol)
+
+
+@pytest.mark.parametrize("dims", DIMS + [(64, 32, 16)])
+def test_compact_ahat_bitexact(orc32, dims):
+    """Compact Â (ddafft_set_ahat_mode 1): the octant the GPU builds, mirrored,
+    is the oracle's bit for bit; convolution, conjugate convolution and matvec
+    through it are the oracle's with that Â bit for bit, and within rounding
+    of the reference's (mode 0)."""
+    rng = np.random.default_rng(sum(dims))
+    kd = np.float32(0.45)
+    x = rnd(rng, dims + (3,))
+    occ = (rng.random(dims) < 0.6).astype(np.int8)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, 1.5 + 0.05j)], np.complex64)
+    print("compact ahat", dims, flush=True)
+    Ac = orc32.build_ahat_compact(dims, kd)
+    with ddafft.Context(dims, kd, nrhs_max=1) as ctx:
+        ctx.set_target(occ, mv)
+        y0 = ctx.convolution(x)
+        ctx.set_ahat_mode(1)
+        assert np.array_equal(bits(ctx.ahat()), bits(Ac))
+        for conj in (False, True):
+            y = ctx.convolution(x, conj=conj)
+            assert np.array_equal(bits(y), bits(orc32.convolution(Ac, x, conj=conj))), conj
+        y = ctx.matvec(alpha, x, False)
+        assert np.array_equal(bits(y), bits(orc32.matvec(Ac, occ, mv, alpha, x, False)))
+        assert _relerr(ctx.convolution(x), y0) < 2e-6
+        ctx.set_ahat_mode(0)   # back to the reference's Â
+        assert np.array_equal(bits(ctx.convolution(x)), bits(y0))
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+def test_compact_ahat_solves(orc32, solver):
+    """Solves with the compact Â: the iteration count and the dipoles
+    bit-identical to the oracle's solve with that Â (eight steps, as the
+    reference-Â parity cases)."""
+    dims, x, m, tol, steps = SOLVE_CASES[0]
+    kd = np.float32(2 * x / dims[0])
+    occ = target(orc32, dims)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, m)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = directions(np.arange(0, 181, 10))
+    Ac = orc32.build_ahat_compact(dims, kd)
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        ctx.set_ahat_mode(1)
+        res = ctx.solve(solver, alpha, kinc, einc, dirs=dirs, max_step=steps, err_max=tol, tau=0.5)
+        P = [ctx.dipoles(r) for r in range(2)]
+    for pol in range(2):
+        ref = orc32.dda_fft(solver, occ, mv, alpha, kd, kinc, einc[pol], Ac, dirs, max_step=steps, err_max=tol,
+                            tau=0.5, return_p=True)
+        assert res[pol]["steps"] == ref["steps"]
+        assert np.array_equal(bits(P[pol]), bits(ref["p"]))
+
+
+def test_compact_ahat_converged_observables(orc64):
+    """Converged BiCGStab solves (relative residual 1e-6) with the compact Â
+    and with the reference's Â, each against an FP64 solve converged to
+    1e-9: the compact mode is as accurate as the reference mode.  (The two
+    float solves differ from each other by rounding, which can move the
+    stopping step by one: on this case 1e-5 in Cext for one polarization.)"""
+    dims, kd = (32, 32, 32), np.float32(0.3125)
+    occ, _ = ddafft.target_ellipsoid(dims, 1.0, 0.5)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, ddafft.polarizability(1, kd, 1.6 + 0.005j)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:2]
+    dirs = directions(np.arange(0, 181, 5))
+    out = []
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        for mode in (0, 1):
+            ctx.set_ahat_mode(mode)
+            out.append(ctx.solve("bicgstab", alpha, kinc, einc, dirs=dirs, max_step=300, err_max=1e-6))
+    A64 = orc64.build_ahat(dims, float(kd))
+    for pol in range(2):
+        t = orc64.dda_fft(2, occ, mv, alpha.astype(np.complex128), float(kd), kinc.astype(np.float64),
+                          einc[pol].astype(np.complex128), A64, dirs.astype(np.float64), max_step=300, err_max=1e-9)
+        e = [dict(cext=abs(o[pol]["cext"] / t["cext"] - 1), cabs=abs(o[pol]["cabs"] / t["cabs"] - 1),
+                  esc=_relerr(o[pol]["esc"], t["esc"])) for o in out]
+        print("pol", pol, "steps", out[0][pol]["steps"], out[1][pol]["steps"], "vs FP64: reference Â", e[0],
+              "compact Â", e[1], flush=True)
+        for k in ("cext", "cabs", "esc"):
+            assert e[1][k] < max(3 * e[0][k], 3e-5), k

@@ -147,6 +147,41 @@ def test_convolution_is_direct_dipole_sum(orc64, conj):
     assert relerr(y, ref) < 1e-10
 
 
+@pytest.mark.parametrize("dims,kd", [((4, 4, 4), 0.5), ((8, 4, 16), 0.7)])
+def test_compact_ahat_is_transform_of_zeroed_tensor(orc32, orc64, dims, kd):
+    """The compact mode's Â (octant of the transform of A with its Nyquist
+    planes zeroed, mirrored): the FP64 DFT of that tensor, so the mirror
+    and its per-component signs are right on every octant."""
+    r = wrap_coords(dims, kd)
+    A = green(r)
+    grid = np.meshgrid(*[np.arange(2 * n) for n in dims], indexing="ij")
+    nyq = (grid[0] == dims[0]) | (grid[1] == dims[1]) | (grid[2] == dims[2])
+    A[nyq] = 0
+    ref = np.fft.fftn(A, axes=(0, 1, 2)) / np.prod(r.shape[:3])
+    assert relerr(orc64.build_ahat_compact(dims, kd), ref) < 1e-12
+    assert relerr(orc32.build_ahat_compact(dims, kd), ref) < 2e-6
+    # the reference's Â differs from it: its Nyquist planes are not zero
+    assert relerr(orc64.build_ahat(dims, kd), ref) > 1e-3
+
+
+@pytest.mark.parametrize("conj", [False, True])
+def test_compact_convolution_is_direct_dipole_sum(orc32, orc64, conj):
+    """The zeroed Nyquist planes never pair two lattice sites: the convolution
+    with the compact Â is the direct dipole sum, and in float it agrees with
+    the reference's Â to rounding."""
+    dims, kd = (4, 4, 8), 0.6
+    rng = np.random.default_rng(5)
+    x = rnd(rng, dims + (3,), np.complex128)
+    y = orc64.convolution(orc64.build_ahat_compact(dims, kd), x, conj=conj)
+    ref = (full_matrix(dims, kd, conj) @ x.reshape(-1)).reshape(x.shape)
+    assert relerr(y, ref) < 1e-10
+    x32 = x.astype(np.complex64)
+    y32 = orc32.convolution(orc32.build_ahat_compact(dims, kd), x32, conj=conj)
+    r32 = orc32.convolution(orc32.build_ahat(dims, kd), x32, conj=conj)
+    assert relerr(y32, r32) < 2e-6
+    assert relerr(y32, ref) < 2e-6
+
+
 def test_convolution_float_vs_double(orc32, orc64):
     dims, kd = (8, 8, 8), 0.4
     rng = np.random.default_rng(3)
