@@ -53,6 +53,7 @@ struct Options {
     unsigned long long seed = 0;
     int group = 8;   // samples per batched solve (this program's own option)
     bool reference_sums = false;   // observables summed as the reference sums them (own option)
+    bool compact_ahat = false;     // A-hat kept as one octant, Nyquist planes of A(r) zeroed (own option)
 };
 
 struct Name {
@@ -150,7 +151,10 @@ void usage(const char* prog) {
                 "  --seed <u64>                     random seed for reproducible runs\n"
                 "  --group <N>                      samples solved together on the GPU, 1..32 (default 8)\n"
                 "  --reference-sums                 sum Cext, Cabs and the far field sequentially in float, in the\n"
-                "                                   reference's order (default: FP64 sums of the same terms)\n",
+                "                                   reference's order (default: FP64 sums of the same terms)\n"
+                "  --compact-ahat                   keep one octant of A-hat, the Nyquist planes of A(r) zeroed:\n"
+                "                                   1/8 of its memory, results within float rounding of the\n"
+                "                                   default (which is bit-identical to the reference's)\n",
                 prog);
 }
 
@@ -224,6 +228,8 @@ int parse(int argc, char** argv, Options* o) {
             o->restart = true;
         } else if (key == "--reference-sums") {
             o->reference_sums = true;
+        } else if (key == "--compact-ahat") {
+            o->compact_ahat = true;
         } else if (key == "-o" || key == "--output") {
             if (need(k, key, &v, inl)) o->output_file = v;
         } else if (key == "-D" || key == "--state") {
@@ -605,7 +611,8 @@ int main(int argc, char** argv) {
         std::vector<ddafft_ctx*> ctxs(nctx, nullptr);
         for (int q = 0; q < nctx; ++q)
             if (ddafft_create(ni, nj, nl, kd, fixed_target ? 2 * group : 2, &ctxs[q]) != DDAFFT_OK ||
-                (o.reference_sums && ddafft_set_sum_mode(ctxs[q], 1) != DDAFFT_OK)) {
+                (o.reference_sums && ddafft_set_sum_mode(ctxs[q], 1) != DDAFFT_OK) ||
+                (o.compact_ahat && ddafft_set_ahat_mode(ctxs[q], 1) != DDAFFT_OK)) {
                 std::fprintf(stderr, "%s\n", ddafft_last_error());
                 return 1;
             }
