@@ -235,3 +235,26 @@ def test_512_axis_lattice(orc32):
         assert res[pol]["steps"] == ref["steps"] == 4
         np.testing.assert_allclose(res[pol]["err_hist"], ref["err_hist"], rtol=1e-9)
         assert np.array_equal(bits(P[pol]), bits(ref["p"]))
+
+
+def test_compact_ahat_agglomerates_converged():
+    """The batched workload's agglomerates (32^3, 32x32x64, 64^3; 16
+    right-hand sides each) solved to a relative residual of 1e-7 with the
+    compact Â and with the reference's Â: far fields and cross sections
+    within 1e-4 (measured: far fields 2e-6 to 1.6e-5, Cext and Cabs up to
+    2.6e-5 at 64^3).  (At the workload's own 1e-3 the two float solves
+    can part by a few percent on these ill-conditioned targets, where the
+    iterate at a residual of 1e-3 is itself that far from the solution.)"""
+    for it in agglomerate_instances():
+        out = []
+        with ddafft.Context(it["dims"], it["kd"], nrhs_max=16) as ctx:
+            ctx.set_target(it["occ"], [0, 1])
+            for mode in (0, 1):
+                ctx.set_ahat_mode(mode)
+                out.append(ctx.solve_ex("bicgstab", it["alpha"], it["kinc"], it["einc"], dirs=it["dirs"],
+                                        max_step=1000, err_max=1e-7))
+        e = max(np.max(np.abs(a["esc"] - b["esc"])) / np.max(np.abs(a["esc"])) for a, b in zip(*out))
+        c = max(max(abs(a["cext"] / b["cext"] - 1), abs(a["cabs"] / b["cabs"] - 1)) for a, b in zip(*out))
+        steps = [(a["steps"], b["steps"]) for a, b in zip(*out)]
+        print(it["dims"], "steps", steps[:4], "max esc diff %.2e, cross sections %.2e" % (e, c), flush=True)
+        assert e < 1e-4 and c < 1e-4
