@@ -63,13 +63,19 @@ template <int N, int RB> struct TileX {
 template <int NI, int RB> struct XStride {
     static constexpr int S1 = NI + kLpad, S2 = 2 * NI + kLpad;
 };
-// n_i = 64, two right-hand sides: strides 66 / 131.  Chosen for a two-column
-// tile (24 forward, 12 inverse lines), where with the stage groupings of a
-// 64-bank conflict model they took the pass from 52.2 to 51.0 us; the
-// four-column tile (48 / 24 lines, one butterfly per line per lane group)
-// was measured with the same strides.
-template <> struct XStride<64, 2> { static constexpr int S1 = 66, S2 = 131; };
+// n_i = 64, two right-hand sides (four-column tiles: 48 forward lines of 64
+// points, 24 inverse lines of 128): strides 65 / 129, with six waves per SIMD
+// (XWaves below).  Against 66 / 131 at five waves: 64^3 X pass 47.8 -> 46.3 us
+// at 2 RHS, 320 -> 304 us at 16 RHS.
+template <> struct XStride<64, 2> { static constexpr int S1 = 65, S2 = 129; };
 template <> struct XStride<64, 4> { static constexpr int S1 = 66, S2 = 131; };   // the same 48 / 24 lines
+// Waves per SIMD the X pass is compiled for (the VGPR budget: 6 waves leave 80
+// registers; 1 places no bound).  n_i = 64 with pairs of right-hand sides, one
+// group per block, the reference's Â: its LDS tile (26.5 KB) lets six blocks
+// share a CU, and 83 registers allowed five.  (Runs of groups need 177
+// registers and the compact Â's mirror 90: bounding them spills.)
+template <int NI, int RB, bool RUN, bool COMPACT> struct XWaves { static constexpr int W = 1; };
+template <> struct XWaves<64, 2, false, false> { static constexpr int W = 6; };
 // 12 lines of 128 points at stride 131 (the Z-inverse tile below): a half-wave
 // takes 32 consecutive butterflies of one line in the last stage.
 template <> struct LaneGroup<128, 12, 131, 2> { static constexpr int G = 32; };
@@ -644,7 +650,7 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
 // groups of RB (RUN; otherwise gpb = 1); each block reads its tile's Â once
 // for all of its groups.
 template <int NI, int RB, bool RUN, bool COMPACT>
-__global__ __launch_bounds__(kBlock) void k_x(Dims d, const SolveState* __restrict__ st,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(XWaves<NI, RB, RUN, COMPACT>::W))) void k_x(Dims d, const SolveState* __restrict__ st,
                                               const float2* __restrict__ b2, float2* __restrict__ b3,
                                               const float2* __restrict__ ahat, int conj,
                                               const float2* __restrict__ wpre, const float2* __restrict__ tw1,
