@@ -268,10 +268,11 @@ __device__ __forceinline__ void lane_pos(unsigned f, unsigned& line, unsigned& b
     }
 }
 
-template <int N, int NL, int LS, int T, int BT = kBlock>
+// TEND: the stage to stop before (the caller runs the stages from TEND on).
+template <int N, int NL, int LS, int T, int BT = kBlock, int TEND = Plan<N>::NST>
 struct FixedStage {
     static __device__ __forceinline__ void run(float2* s, const float2* __restrict__ tw) {
-        if constexpr (T < Plan<N>::NST) {
+        if constexpr (T < TEND) {
             constexpr int K = Plan<N>::K;
             constexpr int R = plan_radix(K, T);
             constexpr int H = span_before(N, T);
@@ -310,13 +311,13 @@ struct FixedStage {
                 }
             }
             __syncthreads();
-            FixedStage<N, NL, LS, T + 1, BT>::run(s, tw);
+            FixedStage<N, NL, LS, T + 1, BT, TEND>::run(s, tw);
         }
     }
 };
 
-// BT: threads per block
-template <int N, int NL, int LS, int BT = kBlock>
+// BT: threads per block; TEND: the stage to stop before (Plan<N>::NST: all)
+template <int N, int NL, int LS, int BT = kBlock, int TEND = Plan<N>::NST>
 __device__ __forceinline__ void fft_lds_fixed(float2* s, const float2* __restrict__ tw) {
     constexpr int K = Plan<N>::K;
     constexpr int R0 = plan_radix(K, 0);
@@ -350,7 +351,7 @@ __device__ __forceinline__ void fft_lds_fixed(float2* s, const float2* __restric
         }
     }
     __syncthreads();
-    FixedStage<N, NL, LS, 1, BT>::run(s, tw);
+    FixedStage<N, NL, LS, 1, BT, TEND>::run(s, tw);
 }
 
 // MAXL: the most lines any call of this instance passes (sizes the registers

@@ -75,6 +75,19 @@ template <> struct XStride<64, 4> { static constexpr int S1 = 66, S2 = 131; };  
 // share a CU, and 83 registers allowed five.  (Runs of groups need 177
 // registers and the compact Â's mirror 90: bounding them spills.)
 template <int NI, int RB, bool RUN, bool COMPACT> struct XWaves { static constexpr int W = 1; };
+// Which X-pass stages run in registers on full tiles: FWD, the forward
+// transforms' first stage computed from the loads; INV, the inverse
+// transforms' last stage stored straight to B3.  Against both in LDS (X pass,
+// MI355X): 128^3, 2 RHS 430 -> 382 us; 32^3, 16 RHS 37.7 -> 34.8 us;
+// 32x32x64, 16 RHS 72 -> 68 us.  At n_i = 64 with pairs, the forward stage in
+// registers needs more than the 80 VGPRs of six waves (spills; 46 -> 51 us),
+// so only the inverse one runs there: 64^3 46-49 -> 42.7 us at 2 RHS, 306 ->
+// 295 us at 16 RHS.
+template <int NI, int RB> struct XRegStages { static constexpr bool FWD = true, INV = true; };
+template <> struct XRegStages<64, 2> { static constexpr bool FWD = false, INV = true; };
+#ifndef DDA_WAVES_PER_EU   // the CPU emulation of the kernels (tests/emul) defines it empty
+#define DDA_WAVES_PER_EU(...) __attribute__((amdgpu_waves_per_eu(__VA_ARGS__)))
+#endif
 template <> struct XWaves<64, 2, false, false> { static constexpr int W = 6; };
 // 12 lines of 128 points at stride 131 (the Z-inverse tile below): a half-wave
 // takes 32 consecutive butterflies of one line in the last stage.
@@ -556,46 +569,98 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
 #pragma unroll
         for (int r = 0; r < RB; ++r) act[r] = (gm >> r) & 1u;
         if (grp > g0) __syncthreads();   // the previous group's stores have read the tile
-        {
-            // all loads of this thread are issued before the first is used; the
-            // block's lines lie at 32-bit offsets from one 64-bit base
-            constexpr int NLD = (CB * TJ * NI + kBlock - 1) / kBlock;
-            const int nld = CB * tj * NI;
-            // runs of tj columns in row J of the (cc, i) planes: B2 [C][J][i][L']
-            const float2* src = b2 + ((long long)(rhs0 * 3) * NJ2 + J) * NI * NL2 + L0;
-            const int lcomp = __builtin_ctz(NJ2) + LNI + __builtin_ctz(NL2), lnl2 = __builtin_ctz(NL2);
-            float2 v[NLD];
-#pragma unroll
-            for (int q = 0; q < NLD; ++q) {
-                const int f = threadIdx.x + q * kBlock;
-                v[q] = make_float2(0.f, 0.f);
-                if (f < nld) {
-                    const int t = f & (tj - 1), r = f >> ltj, cc = r >> LNI;   // r = cc * NI + i
-                    if (act[cc / 3]) v[q] = src[(cc << lcomp) + ((r & (NI - 1)) << lnl2) + t];
-                }
-            }
-            // twiddles staged after the first field loads are issued, so that the
-            // Â and field loads are in flight together; strides of the
-            // compile-time block size (reading blockDim made the compiler wait
-            // for every load in flight first)
+        // twiddles staged after the first field loads are issued, so that the
+        // Â and field loads are in flight together; strides of the
+        // compile-time block size (reading blockDim made the compiler wait for
+        // every load in flight first)
+        auto stage_twiddles = [&]() {
             if (!staged) {
                 for (int f = threadIdx.x; f < NI; f += kBlock) stw1[f] = tw1[f];
                 for (int f = threadIdx.x; f < NI2; f += kBlock) stw2[f] = tw2[f];
                 staged = true;
             }
+        };
+        // runs of tj columns in row J of the (cc, i) planes: B2 [C][J][i][L']
+        const float2* src = b2 + ((long long)(rhs0 * 3) * NJ2 + J) * NI * NL2 + L0;
+        const int lcomp = __builtin_ctz(NJ2) + LNI + __builtin_ctz(NL2), lnl2 = __builtin_ctz(NL2);
+        if constexpr (FULL && XRegStages<NI, RB>::FWD) {
+            // Stage 0 of the forward transforms in registers, straight from the
+            // loads.  Item f = (column col = cc TJ + t, stage-0 butterfly b)
+            // loads the R0 inputs of b (rows q NI/R0 + rest(b), the digit-
+            // reversed gather of the LDS transform) from its column, runs b on
+            // them for the column's line x and for its pre-twiddled twin
+            // W_2ni^i x, and writes both results where stage 1 reads them.  The
+            // same butterflies on the same values as a gather from a natural-
+            // order tile, without that tile's LDS round trip and barrier.
+            constexpr int R0 = plan_radix(Plan<NI>::K, 0), NB0 = NI / R0, NCOL = CB * TJ;
+            constexpr int NITEM = NCOL * NB0, IPT = (NITEM + kBlock - 1) / kBlock;
+            float2 xv[IPT][R0];
 #pragma unroll
-            for (int q = 0; q < NLD; ++q) {
-                const int f = threadIdx.x + q * kBlock;
-                if (f < nld) {
-                    const int t = f & (tj - 1), r = f >> ltj, ii = r & (NI - 1), cc = r >> LNI;
-                    const int line = (cc * tj + t) * 2;
-                    s[line * LS1 + ii] = v[q];
-                    s[(line + 1) * LS1 + ii] = cmul(wpre[ii], v[q]);
+            for (int it = 0; it < IPT; ++it) {
+                const int f = threadIdx.x + it * kBlock;
+#pragma unroll
+                for (int q = 0; q < R0; ++q) xv[it][q] = make_float2(0.f, 0.f);
+                if (NITEM % kBlock == 0 || f < NITEM) {
+                    const int col = f % NCOL, bf = f / NCOL;
+                    const int cc = col / TJ, t = col % TJ, i0 = rest_natural<NI>(bf);
+                    if (act[cc / 3])
+#pragma unroll
+                        for (int q = 0; q < R0; ++q) xv[it][q] = src[(cc << lcomp) + ((q * NB0 + i0) << lnl2) + t];
                 }
             }
+            stage_twiddles();
+#pragma unroll
+            for (int it = 0; it < IPT; ++it) {
+                const int f = threadIdx.x + it * kBlock;
+                if (NITEM % kBlock == 0 || f < NITEM) {
+                    const int col = f % NCOL, bf = f / NCOL, i0 = rest_natural<NI>(bf);
+                    float2 z[R0];
+#pragma unroll
+                    for (int q = 0; q < R0; ++q) z[q] = xv[it][q];
+                    butterfly<R0>(z);
+                    float2* v = s + (2 * col) * LS1 + bf * R0;
+#pragma unroll
+                    for (int q = 0; q < R0; ++q) v[q] = z[q];
+#pragma unroll
+                    for (int q = 0; q < R0; ++q) z[q] = cmul(wpre[q * NB0 + i0], xv[it][q]);
+                    butterfly<R0>(z);
+#pragma unroll
+                    for (int q = 0; q < R0; ++q) v[LS1 + q] = z[q];
+                }
+            }
+            __syncthreads();
+            FixedStage<NI, 2 * NCOL, LS1, 1>::run(s, stw1);
+        } else {
+            {
+                // all loads of this thread are issued before the first is used;
+                // the block's lines lie at 32-bit offsets from one 64-bit base
+                constexpr int NLD = (CB * TJ * NI + kBlock - 1) / kBlock;
+                const int nld = CB * tj * NI;
+                float2 v[NLD];
+#pragma unroll
+                for (int q = 0; q < NLD; ++q) {
+                    const int f = threadIdx.x + q * kBlock;
+                    v[q] = make_float2(0.f, 0.f);
+                    if (f < nld) {
+                        const int t = f & (tj - 1), r = f >> ltj, cc = r >> LNI;   // r = cc * NI + i
+                        if (act[cc / 3]) v[q] = src[(cc << lcomp) + ((r & (NI - 1)) << lnl2) + t];
+                    }
+                }
+                stage_twiddles();
+#pragma unroll
+                for (int q = 0; q < NLD; ++q) {
+                    const int f = threadIdx.x + q * kBlock;
+                    if (f < nld) {
+                        const int t = f & (tj - 1), r = f >> ltj, ii = r & (NI - 1), cc = r >> LNI;
+                        const int line = (cc * tj + t) * 2;
+                        s[line * LS1 + ii] = v[q];
+                        s[(line + 1) * LS1 + ii] = cmul(wpre[ii], v[q]);
+                    }
+                }
+            }
+            __syncthreads();
+            fft_lds<NI, 2 * CB * TJ, LS1>(s, 2 * CB * tj, stw1);
         }
-        __syncthreads();
-        fft_lds<NI, 2 * CB * TJ, LS1>(s, 2 * CB * tj, stw1);
 
         // Â multiply in natural order of I, one Â read for every right-hand
         // side; results land conjugated, in natural order, for the inverse
@@ -627,11 +692,47 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
             }
         }
         __syncthreads();
-        fft_lds<NI2, CB * TJ, LS2>(s, CB * tj, stw2);
-        {
-            // B3 [C][J][i][L'], as B2
-            float2* dst = b3 + ((long long)(rhs0 * 3) * NJ2 + J) * NI * NL2 + L0;
-            const int lcomp = __builtin_ctz(NJ2) + LNI + __builtin_ctz(NL2), lnl2 = __builtin_ctz(NL2);
+        // B3 [C][J][i][L'], as B2
+        float2* dst = b3 + ((long long)(rhs0 * 3) * NJ2 + J) * NI * NL2 + L0;
+        constexpr int NSTI = Plan<NI2>::NST;
+        if constexpr (FULL && NSTI >= 2 && XRegStages<NI, RB>::INV) {
+            // every stage but the last in LDS; the last in registers, its
+            // outputs at positions below n_i (the ones the pruned inverse
+            // keeps) stored straight to B3: the same butterflies and twiddles
+            // as the LDS stage, without its LDS writes, barrier and read-back
+            fft_lds_fixed<NI2, CB * TJ, LS2, kBlock, NSTI - 1>(s, stw2);
+            constexpr int RL = plan_radix(Plan<NI2>::K, NSTI - 1), HL = NI2 / RL, NLI = CB * TJ;
+            constexpr int WL = NLI * HL, ITL = (WL + kBlock - 1) / kBlock;
+#pragma unroll
+            for (int it = 0; it < ITL; ++it) {
+                const int f = threadIdx.x + it * kBlock;
+                if (WL % kBlock == 0 || f < WL) {
+                    const int line = f % NLI, bf = f / NLI;   // butterfly bf of the last stage: span NI2, step 1
+                    const int cc = line / TJ, t = line % TJ;
+                    const float2* v = s + line * LS2 + bf;
+                    float2 z[RL];
+#pragma unroll
+                    for (int q = 0; q < RL; ++q) z[q] = v[q * HL];
+                    if (RL > 2 && bf > 0) {
+                        int tq = 0;
+#pragma unroll
+                        for (int q = 1; q < RL; ++q) {
+                            tq += bf;   // q * bf * step
+                            const float2 tv = stw2[tq];
+                            const float tr = z[q].x * tv.x - z[q].y * tv.y;
+                            z[q].y = z[q].x * tv.y + z[q].y * tv.x;
+                            z[q].x = tr;
+                        }
+                    }
+                    butterfly<RL>(z);
+                    if (act[cc / 3])
+#pragma unroll
+                        for (int q = 0; q < RL; ++q)
+                            if (q * HL < NI) dst[(cc << lcomp) + ((bf + q * HL) << lnl2) + t] = z[q];
+                }
+            }
+        } else {
+            fft_lds<NI2, CB * TJ, LS2>(s, CB * tj, stw2);
             constexpr int NST = (CB * TJ * NI + kBlock - 1) / kBlock;
             const int nst = CB * tj * NI;
 #pragma unroll
@@ -650,7 +751,7 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
 // groups of RB (RUN; otherwise gpb = 1); each block reads its tile's Â once
 // for all of its groups.
 template <int NI, int RB, bool RUN, bool COMPACT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(XWaves<NI, RB, RUN, COMPACT>::W))) void k_x(Dims d, const SolveState* __restrict__ st,
+__global__ __launch_bounds__(kBlock) DDA_WAVES_PER_EU(XWaves<NI, RB, RUN, COMPACT>::W) void k_x(Dims d, const SolveState* __restrict__ st,
                                               const float2* __restrict__ b2, float2* __restrict__ b3,
                                               const float2* __restrict__ ahat, int conj,
                                               const float2* __restrict__ wpre, const float2* __restrict__ tw1,

@@ -40,8 +40,9 @@
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 #define __shared__ static
-// instruction-scheduling fence: no effect on results
+// instruction-scheduling fence and occupancy hint: no effect on results
 inline void __builtin_amdgcn_sched_barrier(int) {}
+#define DDA_WAVES_PER_EU(...)
 
 struct float2 {
     float x, y;

@@ -94,6 +94,7 @@ def main():
     ok &= check_graph(orc)
     ok &= check_fuse(orc)
     ok &= check_compact_ahat(orc)
+    ok &= check_x_lines(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -204,6 +205,41 @@ def check_compact_ahat(orc):
             A = orc.build_ahat(dims, kd)
             ok &= check(np.array_equal(bits(ctx.convolution(x, False)), bits(orc.convolution(A, x, False))),
                         "reference ahat again %s" % (dims,))
+    return ok
+
+
+def check_x_lines(orc):
+    """The X pass on its full-tile path at 32-, 64- and 128-point lines, one
+    and two right-hand sides: convolutions (both conjugations) and a matvec
+    bit for bit against the oracle."""
+    rng = np.random.default_rng(29)
+    ok = True
+    for dims in ((32, 8, 4), (64, 4, 4), (128, 4, 4)):
+        kd = np.float32(0.35)
+        A = orc.build_ahat(dims, kd)
+        occ = (rng.random(dims) < 0.7).astype(np.int8)
+        mv = np.array([0, 1], np.int32)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.05j)], np.complex64)
+        for nrhs in (1, 2):
+            with ddafft.Context(dims, kd, nrhs_max=nrhs) as ctx:
+                ctx.set_target(occ, mv)
+                ctx.set_tuning(x_rb=nrhs)
+                x = rnd(rng, dims + (3,))
+                for conj in (False, True):
+                    y = ctx.convolution(x, conj)
+                    ok &= check(np.array_equal(bits(y), bits(orc.convolution(A, x, conj))),
+                                "x lines conv %s R%d %d" % (dims, nrhs, conj))
+                if nrhs == 2:
+                    kinc = np.array([0, 0, 1], np.float32)
+                    einc = np.eye(3, dtype=np.complex64)[:2]
+                    dirs = directions(np.arange(0, 181, 90), 0.0)
+                    res = ctx.solve(2, np.stack([alpha, alpha]), kinc, einc, dirs=dirs, max_step=3, err_max=1e-9)
+                    for pol in range(2):
+                        ref = orc.dda_fft(2, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=3,
+                                          err_max=1e-9, return_p=True)
+                        ok &= check(res[pol]["steps"] == ref["steps"] and
+                                    np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])),
+                                    "x lines solve %s pol %d" % (dims, pol))
     return ok
 
 
