@@ -85,6 +85,11 @@ template <int NI, int RB, bool RUN, bool COMPACT> struct XWaves { static constex
 // 295 us at 16 RHS.
 template <int NI, int RB> struct XRegStages { static constexpr bool FWD = true, INV = true; };
 template <> struct XRegStages<64, 2> { static constexpr bool FWD = false, INV = true; };
+// Y-inverse lines whose first and last stages run in registers (full
+// tiles).  Against LDS throughout (MI355X): 64^3, 2 RHS 17.2 -> 14.9 us;
+// 32^3, 16 RHS 16.5 -> 14.0 us; 128^3 (256-point lines) 122 -> 126 us and
+// 256^3 (512-point lines) 1287 -> 1468 us, so the longer lines stay in LDS.
+template <int NJ2> struct YRegStages { static constexpr bool ON = NJ2 <= 128; };
 #ifndef DDA_WAVES_PER_EU   // the CPU emulation of the kernels (tests/emul) defines it empty
 #define DDA_WAVES_PER_EU(...) __attribute__((amdgpu_waves_per_eu(__VA_ARGS__)))
 #endif
@@ -804,33 +809,101 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
         // tl in every row J
         const float2* src = b3 + ((long long)c * NJ2 * d.ni + i) * NL2 + L0;
         const int rowj = d.ni << lnl20;
-        {
-            constexpr int NLD = (TL * NJ2 + kBlock - 1) / kBlock;
-            float2 v[NLD];
+        constexpr int NSTJ = Plan<NJ2>::NST;
+        constexpr bool FULL = !std::is_same_v<decltype(tile_arg), int>;
+        if constexpr (FULL && NSTJ >= 2 && YRegStages<NJ2>::ON) {
+            // first stage in registers from the loads (item (t, b): the R0 rows
+            // of stage-0 butterfly b in column t, the LDS gather's digit-
+            // reversed order), the middle stages in LDS, the last stage in
+            // registers with its kept outputs (j < n_j) stored to B4: the same
+            // butterflies and twiddles as the transform in LDS, with two LDS
+            // round trips and a barrier fewer
+            constexpr int R0 = plan_radix(Plan<NJ2>::K, 0), NB0 = NJ2 / R0;
+            constexpr int NITEM = TL * NB0, IPT = (NITEM + kBlock - 1) / kBlock;
+            float2 xv[IPT][R0];
 #pragma unroll
-            for (int q = 0; q < NLD; ++q) {
-                const int f = threadIdx.x + q * kBlock;
-                if (f < tl * NJ2) v[q] = src[(f >> ltl0) * rowj + (f & (tl - 1))];
+            for (int it = 0; it < IPT; ++it) {
+                const int f = threadIdx.x + it * kBlock;
+                if (NITEM % kBlock == 0 || f < NITEM) {
+                    const int t = f % TL, bf = f / TL, j0 = rest_natural<NJ2>(bf);
+#pragma unroll
+                    for (int q = 0; q < R0; ++q) xv[it][q] = src[(q * NB0 + j0) * rowj + t];
+                }
             }
+            for (int f = threadIdx.x; f < NJ2; f += kBlock) stw[f] = tw[f];
 #pragma unroll
-            for (int q = 0; q < NLD; ++q) {
-                const int f = threadIdx.x + q * kBlock;
-                if (f < tl * NJ2) s[(f & (tl - 1)) * LS + (f >> ltl0)] = v[q];   // line t, point J
+            for (int it = 0; it < IPT; ++it) {
+                const int f = threadIdx.x + it * kBlock;
+                if (NITEM % kBlock == 0 || f < NITEM) {
+                    const int t = f % TL, bf = f / TL;
+                    butterfly<R0>(xv[it]);
+                    float2* v = s + t * LS + bf * R0;
+#pragma unroll
+                    for (int q = 0; q < R0; ++q) v[q] = xv[it][q];
+                }
             }
-        }
-        for (int f = threadIdx.x; f < NJ2; f += kBlock) stw[f] = tw[f];
-        __syncthreads();
-        fft_lds<NJ2, TL, LS>(s, tl, stw);
-        constexpr int NJ = NJ2 / 2;
-        const int ltl = __builtin_ctz(tl), lnl2 = __builtin_ctz(NL2);
-        float2* dst = b4 + (((long long)c * d.ni + i) * NJ) * NL2 + L0;
-        constexpr int NST = (TL * NJ + kBlock - 1) / kBlock;
+            __syncthreads();
+            FixedStage<NJ2, TL, LS, 1, kBlock, NSTJ - 1>::run(s, stw);
+            constexpr int NJ = NJ2 / 2;
+            constexpr int RL = plan_radix(Plan<NJ2>::K, NSTJ - 1), HL = NJ2 / RL;
+            constexpr int WL = TL * HL, ITL = (WL + kBlock - 1) / kBlock;
+            const int lnl2 = __builtin_ctz(NL2);
+            float2* dst = b4 + (((long long)c * d.ni + i) * NJ) * NL2 + L0;
 #pragma unroll
-        for (int q = 0; q < NST; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < tl * NJ) {
-                const int j = f >> ltl, t = f - (j << ltl);
-                dst[(j << lnl2) + t] = s[t * LS + j];
+            for (int it = 0; it < ITL; ++it) {
+                const int f = threadIdx.x + it * kBlock;
+                if (WL % kBlock == 0 || f < WL) {
+                    const int t = f % TL, bf = f / TL;   // butterfly bf of the last stage: span NJ2, step 1
+                    const float2* v = s + t * LS + bf;
+                    float2 z[RL];
+#pragma unroll
+                    for (int q = 0; q < RL; ++q) z[q] = v[q * HL];
+                    if (RL > 2 && bf > 0) {
+                        int tq = 0;
+#pragma unroll
+                        for (int q = 1; q < RL; ++q) {
+                            tq += bf;   // q * bf * step
+                            const float2 tv = stw[tq];
+                            const float tr = z[q].x * tv.x - z[q].y * tv.y;
+                            z[q].y = z[q].x * tv.y + z[q].y * tv.x;
+                            z[q].x = tr;
+                        }
+                    }
+                    butterfly<RL>(z);
+#pragma unroll
+                    for (int q = 0; q < RL; ++q)
+                        if (q * HL < NJ) dst[((bf + q * HL) << lnl2) + t] = z[q];
+                }
+            }
+        } else {
+            {
+                constexpr int NLD = (TL * NJ2 + kBlock - 1) / kBlock;
+                float2 v[NLD];
+#pragma unroll
+                for (int q = 0; q < NLD; ++q) {
+                    const int f = threadIdx.x + q * kBlock;
+                    if (f < tl * NJ2) v[q] = src[(f >> ltl0) * rowj + (f & (tl - 1))];
+                }
+#pragma unroll
+                for (int q = 0; q < NLD; ++q) {
+                    const int f = threadIdx.x + q * kBlock;
+                    if (f < tl * NJ2) s[(f & (tl - 1)) * LS + (f >> ltl0)] = v[q];   // line t, point J
+                }
+            }
+            for (int f = threadIdx.x; f < NJ2; f += kBlock) stw[f] = tw[f];
+            __syncthreads();
+            fft_lds<NJ2, TL, LS>(s, tl, stw);
+            constexpr int NJ = NJ2 / 2;
+            const int ltl = __builtin_ctz(tl), lnl2 = __builtin_ctz(NL2);
+            float2* dst = b4 + (((long long)c * d.ni + i) * NJ) * NL2 + L0;
+            constexpr int NST = (TL * NJ + kBlock - 1) / kBlock;
+#pragma unroll
+            for (int q = 0; q < NST; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < tl * NJ) {
+                    const int j = f >> ltl, t = f - (j << ltl);
+                    dst[(j << lnl2) + t] = s[t * LS + j];
+                }
             }
         }
     };

@@ -95,6 +95,7 @@ def main():
     ok &= check_fuse(orc)
     ok &= check_compact_ahat(orc)
     ok &= check_x_lines(orc)
+    ok &= check_y_lines(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -240,6 +241,38 @@ def check_x_lines(orc):
                         ok &= check(res[pol]["steps"] == ref["steps"] and
                                     np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])),
                                     "x lines solve %s pol %d" % (dims, pol))
+    return ok
+
+
+def check_y_lines(orc):
+    """The Y-inverse pass on its full-tile path at 16-, 64- and 128-point
+    lines: convolutions (both conjugations) and a BiCGStab solve bit for bit
+    against the oracle."""
+    rng = np.random.default_rng(31)
+    ok = True
+    for dims in ((8, 8, 8), (4, 32, 8), (4, 64, 8)):
+        kd = np.float32(0.35)
+        A = orc.build_ahat(dims, kd)
+        occ = (rng.random(dims) < 0.7).astype(np.int8)
+        mv = np.array([0, 1], np.int32)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.05j)], np.complex64)
+        with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+            ctx.set_target(occ, mv)
+            x = rnd(rng, dims + (3,))
+            for conj in (False, True):
+                y = ctx.convolution(x, conj)
+                ok &= check(np.array_equal(bits(y), bits(orc.convolution(A, x, conj))),
+                            "y lines conv %s %d" % (dims, conj))
+            kinc = np.array([0, 0, 1], np.float32)
+            einc = np.eye(3, dtype=np.complex64)[:2]
+            dirs = directions(np.arange(0, 181, 90), 0.0)
+            res = ctx.solve(2, np.stack([alpha, alpha]), kinc, einc, dirs=dirs, max_step=3, err_max=1e-9)
+            for pol in range(2):
+                ref = orc.dda_fft(2, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=3, err_max=1e-9,
+                                  return_p=True)
+                ok &= check(res[pol]["steps"] == ref["steps"] and
+                            np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])),
+                            "y lines solve %s pol %d" % (dims, pol))
     return ok
 
 
