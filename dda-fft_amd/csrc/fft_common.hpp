@@ -354,6 +354,48 @@ __device__ __forceinline__ void fft_lds_fixed(float2* s, const float2* __restric
     FixedStage<N, NL, LS, 1, BT, TEND>::run(s, tw);
 }
 
+// The last stage of NLINES line transforms of N points in LDS (stride LS),
+// in registers: every stage before it has run (fft_lds_fixed with TEND =
+// NST - 1), and each output (line, position p, value) with p < KEEP (a
+// multiple of N / r_last: the test folds at compile time) goes to `out`
+// instead of back to LDS.  The butterflies, twiddles and order of the stage
+// in FixedStage: the same bits.  No barrier at the end (nothing is written to
+// LDS).
+template <int N, int NLINES, int LS, int KEEP, class Out>
+__device__ __forceinline__ void fft_last_stage_out(const float2* s, const float2* __restrict__ tw, Out out) {
+    constexpr int NST = Plan<N>::NST;
+    static_assert(NST >= 2, "a transform of at least two stages");
+    constexpr int RL = plan_radix(Plan<N>::K, NST - 1), HL = N / RL;   // span N, step 1
+    constexpr int WL = NLINES * HL, ITL = (WL + kBlock - 1) / kBlock;
+    static_assert(KEEP % HL == 0 && KEEP <= N, "kept outputs: whole runs of the last stage");
+#pragma unroll
+    for (int it = 0; it < ITL; ++it) {
+        const int f = threadIdx.x + it * kBlock;
+        if (WL % kBlock == 0 || f < WL) {
+            const int line = f % NLINES, bf = f / NLINES;   // consecutive lanes: consecutive lines
+            const float2* v = s + line * LS + bf;
+            float2 z[RL];
+#pragma unroll
+            for (int q = 0; q < RL; ++q) z[q] = v[q * HL];
+            if (RL > 2 && bf > 0) {
+                int tq = 0;
+#pragma unroll
+                for (int q = 1; q < RL; ++q) {
+                    tq += bf;   // q * bf * step
+                    const float2 tv = tw[tq];
+                    const float tr = z[q].x * tv.x - z[q].y * tv.y;
+                    z[q].y = z[q].x * tv.y + z[q].y * tv.x;
+                    z[q].x = tr;
+                }
+            }
+            butterfly<RL>(z);
+#pragma unroll
+            for (int q = 0; q < RL; ++q)
+                if (q * HL < KEEP) out(line, bf + q * HL, z[q]);
+        }
+    }
+}
+
 // MAXL: the most lines any call of this instance passes (sizes the registers
 // that carry stage 0 across its barrier); LS: the line stride in float2.
 // Ends with a __syncthreads.

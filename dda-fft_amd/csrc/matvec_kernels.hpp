@@ -217,18 +217,30 @@ __global__ __launch_bounds__(kBlock) void k_zf(Dims d, ProArgs pa, const SolveSt
         }
         for (int f = threadIdx.x; f < NL; f += kBlock) stw[f] = tw[f];
         __syncthreads();
-        fft_lds<NL, 2 * TJ, LS>(s, 2 * tj, stw);
-        const int ltj = __builtin_ctz(tj);
         float2* dst = b1 + ((long long)c * d.ni + i) * (2 * NL) * d.nj + j0;
         const int lnj = __builtin_ctz(d.nj);
-        constexpr int NST = (2 * NL * TJ + kBlock - 1) / kBlock;
+        constexpr bool FULL = !std::is_same_v<decltype(tile_arg), int>;
+        if constexpr (FULL && Plan<NL>::NST >= 2 && NL <= 128) {
+            // the last stage in registers, stored straight to B1: row L' = k
+            // of the line's output k (n_l + k for the pre-twiddled line)
+            // (64^3, 2 RHS: 14.6 -> 14.2 us; 128^3: 75.6 -> 72 us)
+            fft_lds_fixed<NL, 2 * TJ, LS, kBlock, Plan<NL>::NST - 1>(s, stw);
+            fft_last_stage_out<NL, 2 * TJ, LS, NL>(s, stw, [&](int line, int k, float2 v) {
+                const bool odd = line >= TJ;
+                dst[((odd ? NL + k : k) << lnj) + (odd ? line - TJ : line)] = v;
+            });
+        } else {
+            fft_lds<NL, 2 * TJ, LS>(s, 2 * tj, stw);
+            const int ltj = __builtin_ctz(tj);
+            constexpr int NST = (2 * NL * TJ + kBlock - 1) / kBlock;
 #pragma unroll
-        for (int q = 0; q < NST; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < 2 * NL * tj) {   // row L' of B1: the even transform's outputs, then the odd one's
-                const int Lp = f >> ltj, t = f - (Lp << ltj);
-                const float2 v = Lp >= NL ? s[(tj + t) * LS + (Lp - NL)] : s[t * LS + Lp];
-                dst[(Lp << lnj) + t] = v;
+            for (int q = 0; q < NST; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < 2 * NL * tj) {   // row L' of B1: the even transform's outputs, then the odd one's
+                    const int Lp = f >> ltj, t = f - (Lp << ltj);
+                    const float2 v = Lp >= NL ? s[(tj + t) * LS + (Lp - NL)] : s[t * LS + Lp];
+                    dst[(Lp << lnj) + t] = v;
+                }
             }
         }
     };
@@ -442,18 +454,30 @@ __global__ __launch_bounds__(kBlock) void k_yf(Dims d, const SolveState* __restr
         }
         for (int f = threadIdx.x; f < NJ; f += kBlock) stw[f] = tw[f];
         __syncthreads();
-        fft_lds<NJ, 2 * TI, LS>(s, 2 * ti, stw);
         // row J of plane (c, i) of B2 [C][J][i][L'], from L0
         float2* dst = b2 + ((long long)c * (2 * NJ) * d.ni + i) * NL2 + L0;
         const int rowj = d.ni << lnl2;
-        constexpr int NST = (2 * NJ * TI + kBlock - 1) / kBlock;
+        constexpr bool FULL = !std::is_same_v<decltype(tile_arg), int>;
+        if constexpr (FULL && Plan<NJ>::NST >= 2 && NJ <= 128) {
+            // the last stage in registers, stored straight to B2: row
+            // J = 2 k (+1 for the pre-twiddled line) of output k (64^3, 2 RHS:
+            // 16.3 -> 15.9 us; 256-point lines stay in LDS: 1270 -> 1315 us)
+            fft_lds_fixed<NJ, 2 * TI, LS, kBlock, Plan<NJ>::NST - 1>(s, stw);
+            fft_last_stage_out<NJ, 2 * TI, LS, NJ>(s, stw, [&](int line, int k, float2 v) {
+                const bool odd = line >= TI;
+                dst[(2 * k + (odd ? 1 : 0)) * rowj + (odd ? line - TI : line)] = v;
+            });
+        } else {
+            fft_lds<NJ, 2 * TI, LS>(s, 2 * ti, stw);
+            constexpr int NST = (2 * NJ * TI + kBlock - 1) / kBlock;
 #pragma unroll
-        for (int q = 0; q < NST; ++q) {
-            const int f = threadIdx.x + q * kBlock;
-            if (f < 2 * NJ * ti) {
-                const int J = f >> lti, t = f - (J << lti);
-                const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
-                dst[J * rowj + t] = v;
+            for (int q = 0; q < NST; ++q) {
+                const int f = threadIdx.x + q * kBlock;
+                if (f < 2 * NJ * ti) {
+                    const int J = f >> lti, t = f - (J << lti);
+                    const float2 v = (J & 1) ? s[(ti + t) * LS + (J >> 1)] : s[t * LS + (J >> 1)];
+                    dst[J * rowj + t] = v;
+                }
             }
         }
     };
@@ -706,6 +730,8 @@ __device__ __forceinline__ void x_tile(const Dims& d, unsigned long long amask, 
             // keeps) stored straight to B3: the same butterflies and twiddles
             // as the LDS stage, without its LDS writes, barrier and read-back
             fft_lds_fixed<NI2, CB * TJ, LS2, kBlock, NSTI - 1>(s, stw2);
+            // (inline rather than fft_last_stage_out: through the helper the
+            // compiler takes 83 VGPRs instead of 73 at n_i = 32)
             constexpr int RL = plan_radix(Plan<NI2>::K, NSTI - 1), HL = NI2 / RL, NLI = CB * TJ;
             constexpr int WL = NLI * HL, ITL = (WL + kBlock - 1) / kBlock;
 #pragma unroll
@@ -845,36 +871,9 @@ __global__ __launch_bounds__(kBlock) void k_yi(Dims d, const SolveState* __restr
             __syncthreads();
             FixedStage<NJ2, TL, LS, 1, kBlock, NSTJ - 1>::run(s, stw);
             constexpr int NJ = NJ2 / 2;
-            constexpr int RL = plan_radix(Plan<NJ2>::K, NSTJ - 1), HL = NJ2 / RL;
-            constexpr int WL = TL * HL, ITL = (WL + kBlock - 1) / kBlock;
             const int lnl2 = __builtin_ctz(NL2);
             float2* dst = b4 + (((long long)c * d.ni + i) * NJ) * NL2 + L0;
-#pragma unroll
-            for (int it = 0; it < ITL; ++it) {
-                const int f = threadIdx.x + it * kBlock;
-                if (WL % kBlock == 0 || f < WL) {
-                    const int t = f % TL, bf = f / TL;   // butterfly bf of the last stage: span NJ2, step 1
-                    const float2* v = s + t * LS + bf;
-                    float2 z[RL];
-#pragma unroll
-                    for (int q = 0; q < RL; ++q) z[q] = v[q * HL];
-                    if (RL > 2 && bf > 0) {
-                        int tq = 0;
-#pragma unroll
-                        for (int q = 1; q < RL; ++q) {
-                            tq += bf;   // q * bf * step
-                            const float2 tv = stw[tq];
-                            const float tr = z[q].x * tv.x - z[q].y * tv.y;
-                            z[q].y = z[q].x * tv.y + z[q].y * tv.x;
-                            z[q].x = tr;
-                        }
-                    }
-                    butterfly<RL>(z);
-#pragma unroll
-                    for (int q = 0; q < RL; ++q)
-                        if (q * HL < NJ) dst[((bf + q * HL) << lnl2) + t] = z[q];
-                }
-            }
+            fft_last_stage_out<NJ2, TL, LS, NJ>(s, stw, [&](int t, int j, float2 v) { dst[(j << lnl2) + t] = v; });
         } else {
             {
                 constexpr int NLD = (TL * NJ2 + kBlock - 1) / kBlock;

@@ -96,6 +96,7 @@ def main():
     ok &= check_compact_ahat(orc)
     ok &= check_x_lines(orc)
     ok &= check_y_lines(orc)
+    ok &= check_z_lines(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -273,6 +274,40 @@ def check_y_lines(orc):
                 ok &= check(res[pol]["steps"] == ref["steps"] and
                             np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])),
                             "y lines solve %s pol %d" % (dims, pol))
+    return ok
+
+
+def check_z_lines(orc):
+    """The Z- and Y-forward passes (split, not fused) on their full-tile paths
+    at 16-, 32- and 64-point lines: convolutions (both conjugations) and a
+    BiCGStab solve (every prologue of its matvecs) bit for bit against the
+    oracle."""
+    rng = np.random.default_rng(37)
+    ok = True
+    for dims in ((4, 16, 16), (4, 16, 64), (4, 32, 32)):
+        kd = np.float32(0.35)
+        A = orc.build_ahat(dims, kd)
+        occ = (rng.random(dims) < 0.7).astype(np.int8)
+        mv = np.array([0, 1], np.int32)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.05j)], np.complex64)
+        with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+            ctx.set_target(occ, mv)
+            ctx.set_tuning(fuse_zy=0)
+            x = rnd(rng, dims + (3,))
+            for conj in (False, True):
+                y = ctx.convolution(x, conj)
+                ok &= check(np.array_equal(bits(y), bits(orc.convolution(A, x, conj))),
+                            "z lines conv %s %d" % (dims, conj))
+            kinc = np.array([0, 0, 1], np.float32)
+            einc = np.eye(3, dtype=np.complex64)[:2]
+            dirs = directions(np.arange(0, 181, 90), 0.0)
+            res = ctx.solve(2, np.stack([alpha, alpha]), kinc, einc, dirs=dirs, max_step=3, err_max=1e-9)
+            for pol in range(2):
+                ref = orc.dda_fft(2, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=3, err_max=1e-9,
+                                  return_p=True)
+                ok &= check(res[pol]["steps"] == ref["steps"] and
+                            np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])),
+                            "z lines solve %s pol %d" % (dims, pol))
     return ok
 
 
