@@ -232,6 +232,7 @@ struct ddafft_ctx {
     size_t fftab_cap = 0;
     int sum_mode = 0;                  // observables: 0 FP64 sums, 1 the reference's sequential float sums
     int ahat_mode = 0;                 // 0: full Â as the reference builds it; 1: one octant (Nyquist planes zeroed)
+    int ff_mfma = -1;                  // far field on the matrix cores: -1 default, 0 off, 1 on
     // launch schedule, 0 / -1: from the tuned table, else the built-in rule
     int x_blocks = 0;                  // X pass: blocks per launch up to which right-hand-side groups are spread
     int x_rb = 0;                      // X pass: right-hand sides per tile
@@ -1117,6 +1118,10 @@ int ddafft_set_tuning(ddafft_ctx* c, int key, int value) {
             if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "fuse_zy must be -1 (tuned), 0 or 1");
             c->fuse_zy = value;
             return DDAFFT_OK;
+        case DDAFFT_TUNE_FF_MFMA:
+            if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "ff_mfma must be -1 (default), 0 or 1");
+            c->ff_mfma = value;
+            return DDAFFT_OK;
         case DDAFFT_TUNE_GRAPH:
             if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "graph must be -1 (tuned), 0 or 1");
             c->use_graph = value;
@@ -1334,11 +1339,24 @@ struct Fin {
 // side (block sums, or the one sequential float sum of the reference-order
 // mode) and dipole chunks per far-field direction tile (enough blocks to
 // fill the chip).
+// The FP64-summed far field on the matrix cores (k_far_field_mfma): when the
+// context asks for it (or by default) and the phase rows of 16 directions fit
+// LDS (ni + nj + nl <= 384).
+constexpr int kFFMfmaDefault = 0;
+bool ff_mfma_on(const ddafft_ctx* c) {
+    const int mode = c->ff_mfma >= 0 ? c->ff_mfma : kFFMfmaDefault;
+    return mode == 1 && !c->sum_mode && c->d.ni + c->d.nj + c->d.nl <= 384;
+}
+
 void obs_blocks(const ddafft_ctx* c, int ndirs, int* cb, int* fchunk) {
     *cb = c->sum_mode ? 1 : (int)std::min<long long>((c->d.n + kBlock - 1) / kBlock, 512);
-    const int dtiles = (ndirs + kFD - 1) / kFD;
-    *fchunk = ndirs > 0 ? (int)std::max(1, std::min({64, (c->ncells + 4095) / 4096,
-                                                     (2048 + dtiles - 1) / std::max(dtiles, 1)})) : 0;
+    const bool mf = ff_mfma_on(c);
+    const int fd = mf ? kFDM : kFD;
+    const int dtiles = (ndirs + fd - 1) / fd;
+    // the MFMA form walks its chunk with dependent loads: shorter chunks, more blocks
+    const int per = mf ? 1024 : 4096, want = mf ? 4096 : 2048;
+    *fchunk = ndirs > 0 ? (int)std::max(1, std::min({64, (c->ncells + per - 1) / per,
+                                                     (want + dtiles - 1) / std::max(dtiles, 1)})) : 0;
 }
 
 // Sizes the downloads and reserves the pinned buffer (before anything of this
@@ -1423,7 +1441,18 @@ int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_s
             // phase rows in LDS sized for lattices up to 256 per axis, or the
             // larger form for 512-point axes
             const bool big = d.ni + d.nj + d.nl > kFFAxis;
-            if (pair && !big)
+            if (ff_mfma_on(c)) {
+                const dim3 gm(f.fchunk, (ndirs + kFDM - 1) / kFDM);
+                const bool small = d.ni + d.nj + d.nl <= 192;
+#define DDA_FFM(R, AXM)                                                                                 \
+    hipLaunchKernelGGL((k_far_field_mfma<R, AXM>), gm, kBlock, 0, c->stream, d, c->fftab, c->cells, c->ncells, \
+                       ndirs, r0, c->P, c->partials)
+                if (pair && small) DDA_FFM(2, 192);
+                else if (pair) DDA_FFM(2, 384);
+                else if (small) DDA_FFM(1, 192);
+                else DDA_FFM(1, 384);
+#undef DDA_FFM
+            } else if (pair && !big)
                 hipLaunchKernelGGL((k_far_field<2, kFFAxis>), gf, kBlock, 0, c->stream, d, c->fftab, c->cells,
                                    c->ncells, ndirs, r0, c->P, c->partials);
             else if (pair)

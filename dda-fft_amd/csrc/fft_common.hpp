@@ -21,6 +21,9 @@
 namespace ddafft {
 
 constexpr int kBlock = 256;   // threads per block for every pass kernel
+#ifndef DDA_FLOAT4_MFMA   // the CPU emulation of the kernels (tests/emul) defines its own
+#define DDA_FLOAT4_MFMA float __attribute__((ext_vector_type(4)))
+#endif
 constexpr int kLpad = 1;      // LDS line padding (float2) against bank conflicts
 
 // ---------------------------------------------------------------------------

@@ -383,6 +383,106 @@ __global__ __launch_bounds__(kBlock) void k_far_field(Dims d, const float2* __re
     }
 }
 
+// The same sums as k_far_field on the matrix cores.  A block takes 16
+// directions and a chunk of the dipoles; each wave, 4 dipoles per step: lane
+// (m = lane % 16, q = lane / 16) forms the phase of direction d0 + m at dipole
+// q (the same two complex products as k_far_field) and loads that dipole's
+// moment for column n = lane % 16 (complex column n / 2: right-hand side and
+// component; real or imaginary part n % 2).  Two 16x16x4 FP32 MFMAs per step,
+//   D += Re(phase) [Pr | Pi]  and  D += Im(phase) [-Pi | Pr],
+// accumulate the real and imaginary parts of sum phase * P (MFMA lane layout:
+// A[l % 16][l / 16], B[l / 16][l % 16], D[4 (l / 16) + v][l % 16]).  Eight
+// steps run per iteration, their cell and moment loads issued first; after
+// each, the FP32 accumulator is added into FP64 per lane; the waves' sums
+// meet in LDS in a fixed order.  Same partials layout as k_far_field, with
+// 16 directions per block row.  AXM: LDS row stride of the phase table
+// (ni + nj + nl <= AXM).
+constexpr int kFDM = 16;   // directions per block of the MFMA form
+template <int R, int AXM>
+__global__ __launch_bounds__(kBlock) void k_far_field_mfma(Dims d, const float2* __restrict__ ptab,
+                                                           const int* __restrict__ cells, int ncells, int ndirs,
+                                                           int rhs0, const float2* __restrict__ P,
+                                                           double* __restrict__ partials) {
+    typedef DDA_FLOAT4_MFMA f4;
+    constexpr int NC = 6 * R;   // real columns: (right-hand side, component) x (re, im)
+    static_assert(NC <= 16, "columns fit one MFMA tile");
+    constexpr int NW = kBlock / 64;
+    constexpr int TS = AXM + 1;   // odd row stride: the 16 directions' rows in different banks
+    constexpr int U = 8;          // steps per iteration, their loads issued first
+    __shared__ float2 tab[kFDM * TS];
+    __shared__ double red[NW * kFDM * 16];
+    const int nchunk = gridDim.x;
+    const int d0 = blockIdx.y * kFDM;
+    const int naxis = d.ni + d.nj + d.nl;
+    for (int f = threadIdx.x; f < kFDM * naxis; f += kBlock) {
+        const int q = f / naxis, a = f - q * naxis;
+        const int dd = min(d0 + q, ndirs - 1);
+        tab[q * TS + a] = ptab[dd * naxis + a];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, q = lane >> 4, n = lane & 15;
+    const int cpx = n >> 1, rr = cpx / 3, comp = cpx - rr * 3;
+    const bool col = n < NC;
+    const float2* t = tab + m * TS;
+    const long long pbase = (long long)(rhs0 + (col ? rr : 0)) * 3 * d.n + (long long)comp * d.n;
+    const int sl = __builtin_ctz(d.nl), sj = __builtin_ctz(d.nj);
+    const int per = (ncells + nchunk - 1) / nchunk;
+    const int lo = blockIdx.x * per, hi = min(ncells, lo + per);
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    double dacc[4] = {0.0, 0.0, 0.0, 0.0};
+    // every wave runs the same number of steps (lanes past the chunk add zero)
+    const int nstep = (max(hi - lo, 0) + 4 * NW - 1) / (4 * NW);
+    for (int st = 0; st < nstep; st += U) {
+        int k[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = lo + 4 * (wave + NW * (st + u)) + q;
+            ok[u] = e < hi;   // st + u < nstep too: e grows with the step
+            k[u] = ok[u] ? cells[e] : 0;
+        }
+        float2 pv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) pv[u] = (ok[u] && col) ? P[pbase + k[u]] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float a1 = 0.f, a2 = 0.f;
+            if (ok[u]) {
+                const int l = k[u] & (d.nl - 1), ij = k[u] >> sl;
+                const int j = ij & (d.nj - 1), i = ij >> sj;
+                const float2 ph = cmul_fma(cmul_fma(t[i], t[d.ni + j]), t[d.ni + d.nj + l]);
+                a1 = ph.x;
+                a2 = ph.y;
+            }
+            const float b1 = (n & 1) ? pv[u].y : pv[u].x, b2 = (n & 1) ? pv[u].x : -pv[u].y;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc, 0, 0, 0);
+        }
+        // FP32 runs of U steps (4 U dipoles per wave) go into FP64
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dacc[v] += (double)acc[v];
+        acc = f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dacc[v] += (double)acc[v];
+    // lane holds D[4 (lane / 16) + v][lane % 16]: direction row, real column
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[(wave * kFDM + 4 * q + v) * 16 + n] = dacc[v];
+    __syncthreads();
+    const int tid = threadIdx.x;
+    if (tid < kFDM * 16) {
+        const int row = tid >> 4, cn = tid & 15;
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sum += red[(w * kFDM + row) * 16 + cn];
+        if (cn < NC && d0 + row < ndirs) {
+            const int r = (cn >> 1) / 3, e6 = cn - r * 6;   // (component, re / im) of right-hand side r
+            partials[(((long long)(rhs0 + r) * nchunk + blockIdx.x) * ndirs + d0 + row) * 6 + e6] = sum;
+        }
+    }
+}
+
 // Sums the chunks and applies the transverse projection:
 // out[nrhs][ndirs][3] complex float.  Right-hand side r uses the directions
 // dirs + r * dir_stride (dir_stride 0: one set shared by all).

@@ -169,6 +169,12 @@ int ddafft_set_ahat_mode(ddafft_ctx* ctx, int mode);
  * i-plane in one kernel where n_j, n_l <= 64, 0 always as two passes, -1
  * (default) as the table says. */
 #define DDAFFT_TUNE_FUSE_ZY 3
+/* DDAFFT_TUNE_FF_MFMA: 1 sums the far field (FP64-summed mode) on the matrix
+ * cores, 16 directions per block, where the phase rows fit (ni + nj + nl <=
+ * 384); 0 with the direct kernel; -1 (default) as the library chooses
+ * (the direct kernel).  The two agree to about 1e-7 relative, not bit for
+ * bit. */
+#define DDAFFT_TUNE_FF_MFMA 4
 int ddafft_set_tuning(ddafft_ctx* ctx, int key, int value);
 /* One instance's solve for ddafft_solve_ex / ddafft_solve_batch: nrhs
  * right-hand sides, each with its own incidence (Monte-Carlo orientations x

@@ -220,6 +220,35 @@ inline double __shfl_xor(double v, int off, int width = 64) {
     return r;
 }
 
+// 16x16x4 FP32 matrix product of one wave (v_mfma_f32_16x16x4f32): lane l
+// gives A[l % 16][l / 16] and B[l / 16][l % 16] and gets back
+// D[4 (l / 16) + v][l % 16] (+ its accumulator), the layout the GPU has.  The
+// block's threads meet at a barrier to exchange their operands, so every
+// thread of the block must call it the same number of times.
+struct emul_f4 {
+    float v[4];
+    float& operator[](int i) { return v[i]; }
+    float operator[](int i) const { return v[i]; }
+};
+#define DDA_FLOAT4_MFMA emul_f4
+inline emul_f4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, emul_f4 c, int, int, int) {
+    static float A[emul::kMaxThreads], B[emul::kMaxThreads];
+    const int t = (int)threadIdx.x;
+    A[t] = a;
+    B[t] = b;
+    __syncthreads();
+    const int w = t / 64, l = t % 64;
+    emul_f4 d = c;
+    for (int v = 0; v < 4; ++v) {
+        const int row = 4 * (l / 16) + v, col = l % 16;
+        float acc = d.v[v];
+        for (int k = 0; k < 4; ++k) acc += A[w * 64 + 16 * k + row] * B[w * 64 + 16 * k + col];
+        d.v[v] = acc;
+    }
+    __syncthreads();
+    return d;
+}
+
 inline unsigned __brev(unsigned x) {
     unsigned r = 0;
     for (int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);

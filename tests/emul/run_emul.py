@@ -97,6 +97,7 @@ def main():
     ok &= check_x_lines(orc)
     ok &= check_y_lines(orc)
     ok &= check_z_lines(orc)
+    ok &= check_ff_mfma(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -308,6 +309,35 @@ def check_z_lines(orc):
                 ok &= check(res[pol]["steps"] == ref["steps"] and
                             np.array_equal(bits(ctx.dipoles(pol)), bits(ref["p"])),
                             "z lines solve %s pol %d" % (dims, pol))
+    return ok
+
+
+def check_ff_mfma(orc):
+    """The far field on the matrix cores (ddafft_set_tuning ff_mfma) against
+    the oracle's far field and against the direct kernel's."""
+    rng = np.random.default_rng(41)
+    ok = True
+    for dims in ((8, 8, 8), (4, 8, 16)):
+        kd = np.float32(0.5)
+        A = orc.build_ahat(dims, kd)
+        occ = (rng.random(dims) < 0.7).astype(np.int8)
+        mv = np.array([0, 1], np.int32)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.05j)], np.complex64)
+        kinc = np.array([0, 0, 1], np.float32)
+        einc = np.eye(3, dtype=np.complex64)[:2]
+        dirs = np.concatenate([directions(np.arange(0, 181, 10), p) for p in (0.0, 45.0)])   # 38: a partial tile
+        out = {}
+        with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+            ctx.set_target(occ, mv)
+            for mode in (0, 1):
+                ctx.set_tuning(ff_mfma=mode)
+                out[mode] = ctx.solve(2, np.stack([alpha, alpha]), kinc, einc, dirs=dirs, max_step=4, err_max=1e-9)
+        for pol in range(2):
+            ref = orc.dda_fft(2, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=4, err_max=1e-9)
+            e1 = np.max(np.abs(out[1][pol]["esc"] - ref["esc"])) / np.max(np.abs(ref["esc"]))
+            e01 = np.max(np.abs(out[1][pol]["esc"] - out[0][pol]["esc"])) / np.max(np.abs(out[0][pol]["esc"]))
+            ok &= check(e1 < 1e-5 and e01 < 2e-6, "ff mfma %s pol %d vs oracle %.1e vs direct %.1e" % (
+                dims, pol, e1, e01))
     return ok
 
 

@@ -415,3 +415,31 @@ def test_compact_ahat_converged_observables(orc64):
               "compact Â", e[1], flush=True)
         for k in ("cext", "cabs", "esc"):
             assert e[1][k] < max(3 * e[0][k], 3e-5), k
+
+
+@pytest.mark.parametrize("dims,npol", [((16, 16, 16), 2), ((32, 32, 32), 2), ((32, 32, 64), 1)])
+def test_far_field_mfma(orc32, dims, npol):
+    """The far field summed on the matrix cores (ddafft_set_tuning ff_mfma):
+    within 1e-5 of the oracle's far field and within 2e-6 of the direct
+    kernel's, with a direction count that leaves a partial 16-direction tile."""
+    kd = np.float32(0.3)
+    occ = target(orc32, dims, 1.0, 0.5)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, 1.5 + 0.02j)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:npol]
+    dirs = np.concatenate([directions(np.arange(0, 181, 2), p) for p in (0.0, 30.0, 90.0)])   # 273
+    out = {}
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        A = ctx.ahat()
+        for mode in (0, 1):
+            ctx.set_tuning(ff_mfma=mode)
+            out[mode] = ctx.solve(2, alpha, kinc, einc, dirs=dirs, max_step=6, err_max=1e-6)
+    for pol in range(npol):
+        ref = orc32.dda_fft(2, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=6, err_max=1e-6)
+        e = _relerr(out[1][pol]["esc"], ref["esc"])
+        e01 = _relerr(out[1][pol]["esc"], out[0][pol]["esc"])
+        print("ff mfma", dims, pol, "vs oracle %.2e, vs direct %.2e" % (e, e01), flush=True)
+        assert e < 1e-5 and e01 < 2e-6
+        assert out[1][pol]["cext"] == out[0][pol]["cext"]
