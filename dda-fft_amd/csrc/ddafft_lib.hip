@@ -190,6 +190,8 @@ struct ddafft_ctx {
     int nmat = 0;
     int* cells = nullptr;   // dipole cell indices (far field)
     int ncells = 0;
+    int2* ffseg = nullptr;  // far-field column segments {first dipole, (i nj + j) << 6 | count}
+    int nseg = 0;
     long long ndip = 0;
     std::vector<long long> mat_count;  // dipoles per material index
     bool have_target = false;
@@ -233,6 +235,11 @@ struct ddafft_ctx {
     int sum_mode = 0;                  // observables: 0 FP64 sums, 1 the reference's sequential float sums
     int ahat_mode = 0;                 // 0: full Â as the reference builds it; 1: one octant (Nyquist planes zeroed)
     int ff_mfma = -1;                  // far field on the matrix cores: -1 default, 0 off, 1 on
+    int ff_group = -1;                 // far field by groups of directions sharing z: -1 default (on), 0 off, 1 on
+    int ffg_n = 0, ffg_S = 0;          // the resident direction set as ffg_n groups of ffg_S (0: no such split)
+    bool ffg_off_call = false;         // set by an allocation-free solve whose scratch cannot hold the grouped form
+    float2* ffpc = nullptr;            // dipole moments in dipole-list order for the grouped far field
+    size_t ffpc_cap = 0;
     // launch schedule, 0 / -1: from the tuned table, else the built-in rule
     int x_blocks = 0;                  // X pass: blocks per launch up to which right-hand-side groups are spread
     int x_rb = 0;                      // X pass: right-hand sides per tile
@@ -926,7 +933,8 @@ int ddafft_destroy(ddafft_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void* bufs[] = {c->ahat, c->tables, c->occ, c->dipf, c->mv, c->cells, c->b1, c->b2, c->b3, c->P, c->Pmin, c->R,
                     c->S, c->G, c->H, c->W, c->X, c->Y, c->small, c->hist, c->partials,
-                    c->dirs, c->esc, c->fftab, c->Pf, c->phase, c->gaux, c->inc_cell, c->seqbuf};
+                    c->dirs, c->esc, c->fftab, c->Pf, c->phase, c->gaux, c->inc_cell, c->seqbuf,
+                    c->ffseg, c->ffpc};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->state_ring) hipHostFree(c->state_ring);
@@ -1078,13 +1086,36 @@ int ddafft_set_target(ddafft_ctx* c, const signed char* occ, int nmat, const int
         if (mv[m]) cells.push_back((int)k);
     }
     if (cells.empty()) return fail(DDAFFT_EINVAL, "target has no dipoles");
+    // far-field segments: each run of dipoles in one (i, j) column, cut into
+    // near-equal pieces of at most kFFSeg
+    std::vector<int2> segs;
+    {
+        const int lnl = __builtin_ctz(c->d.nl);
+        size_t e = 0;
+        while (e < cells.size()) {
+            const int ij = cells[e] >> lnl;
+            size_t e1 = e + 1;
+            while (e1 < cells.size() && (cells[e1] >> lnl) == ij) ++e1;
+            const int len = (int)(e1 - e), np = (len + kFFSeg - 1) / kFFSeg;
+            for (int q = 0, at = 0; q < np; ++q) {
+                const int cnt = (len - at) / (np - q);
+                segs.push_back(make_int2((int)e + at, (ij << 6) | cnt));
+                at += cnt;
+            }
+            e = e1;
+        }
+    }
     HIPCHECK(hipStreamSynchronize(c->stream));
     if (c->mv) hipFree(c->mv);
     if (c->cells) hipFree(c->cells);
+    if (c->ffseg) hipFree(c->ffseg);
     c->mv = nullptr;
     c->cells = nullptr;
+    c->ffseg = nullptr;
+    c->nseg = 0;
     int rc = dev_alloc(&c->mv, (size_t)nmat);
     if (!rc) rc = dev_alloc(&c->cells, cells.size());
+    if (!rc) rc = dev_alloc(&c->ffseg, segs.size());
     if (rc) return rc;
     c->nmat = nmat;
     c->mat_count.assign(nmat, 0);
@@ -1097,6 +1128,8 @@ int ddafft_set_target(ddafft_ctx* c, const signed char* occ, int nmat, const int
     if ((rc = upload(c, c->dipf, c->dipf_h.data(), (size_t)n))) return rc;
     if ((rc = upload(c, c->mv, mv, (size_t)nmat * sizeof(int)))) return rc;
     if ((rc = upload(c, c->cells, cells.data(), cells.size() * sizeof(int)))) return rc;
+    if ((rc = upload(c, c->ffseg, segs.data(), segs.size() * sizeof(int2)))) return rc;
+    c->nseg = (int)segs.size();
     HIPCHECK(hipStreamSynchronize(c->stream));
     c->have_target = true;
     return DDAFFT_OK;
@@ -1121,6 +1154,10 @@ int ddafft_set_tuning(ddafft_ctx* c, int key, int value) {
         case DDAFFT_TUNE_FF_MFMA:
             if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "ff_mfma must be -1 (default), 0 or 1");
             c->ff_mfma = value;
+            return DDAFFT_OK;
+        case DDAFFT_TUNE_FF_GROUP:
+            if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "ff_group must be -1 (default), 0 or 1");
+            c->ff_group = value;
             return DDAFFT_OK;
         case DDAFFT_TUNE_GRAPH:
             if (value < -1 || value > 1) return fail(DDAFFT_EINVAL, "graph must be -1 (tuned), 0 or 1");
@@ -1149,6 +1186,30 @@ int ddafft_set_sum_mode(ddafft_ctx* c, int mode) {
     return DDAFFT_OK;
 }
 
+namespace {
+
+// A set of ng * S directions whose z components repeat with period ng
+// (directions d and d + ng share z bit for bit) splits into ng groups of S
+// (group g: g, g + ng, ..., g + (S - 1) ng).  Returns ng of the smallest
+// such period with 2 <= S <= kFFDirs (S in *S), else 0.
+int dir_groups(const float* dirs, int ndirs, int* S) {
+    *S = 0;
+    if (!dirs || ndirs < 2) return 0;
+    for (int ng = std::max(1, (ndirs + kFFDirs - 1) / kFFDirs); ng <= ndirs / 2; ++ng) {
+        if (ndirs % ng) continue;
+        bool ok = true;
+        for (int dd = ng; dd < ndirs && ok; ++dd)
+            ok = std::memcmp(&dirs[3 * dd + 2], &dirs[3 * (dd - ng) + 2], sizeof(float)) == 0;
+        if (ok) {
+            *S = ndirs / ng;
+            return ng;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
 int ddafft_set_directions(ddafft_ctx* c, int ndirs, const float* dirs) {
     if (!c || ndirs < 0 || (ndirs > 0 && !dirs)) return fail(DDAFFT_EINVAL, "bad directions");
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1161,6 +1222,7 @@ int ddafft_set_directions(ddafft_ctx* c, int ndirs, const float* dirs) {
         c->dirs_cap = ndirs;
     }
     c->ndirs = ndirs;
+    c->ffg_n = dir_groups(dirs, ndirs, &c->ffg_S);
     if (ndirs) {
         HIPCHECK(hipMemcpy(c->dirs, dirs, (size_t)ndirs * 3 * sizeof(float), hipMemcpyHostToDevice));
     }
@@ -1331,7 +1393,7 @@ int iteration(Seq& q, int solver, int o, float tau) {
 // after the stream (or an event after the copies) has completed.
 struct Fin {
     int nrhs = 0, ndirs = 0, cb = 0, fchunk = 0, max_step = 0;
-    bool state = false, hist = false;
+    bool state = false, hist = false, grouped = false;
     size_t o_st = 0, o_hist = 0, o_part = 0, o_esc = 0, total = 0;
 };
 
@@ -1348,8 +1410,26 @@ bool ff_mfma_on(const ddafft_ctx* c) {
     return mode == 1 && !c->sum_mode && c->d.ni + c->d.nj + c->d.nl <= 384;
 }
 
-void obs_blocks(const ddafft_ctx* c, int ndirs, int* cb, int* fchunk) {
+// The far field by groups of directions sharing z (k_far_field_grp): when a
+// set of ndirs directions splits into ng groups of S = 2..16 (dir_groups)
+// and the lattice's phase rows fit the kernel's LDS (n_i + n_j <= 512,
+// n_l <= 256).
+bool ff_group_ok(const ddafft_ctx* c, int ndirs, int ng, int S) {
+    const int mode = c->ff_group >= 0 ? c->ff_group : 1;
+    return mode == 1 && !c->sum_mode && !ff_mfma_on(c) && ndirs > 0 && ng > 0 && S >= 2 && S <= kFFDirs &&
+           ng * S == ndirs && c->nseg > 0 && c->d.ni + c->d.nj <= 512 && c->d.nl <= 256;
+}
+// ... for the resident set
+bool ff_group_on(const ddafft_ctx* c, int ndirs) {
+    return !c->ffg_off_call && ndirs == c->ndirs && ff_group_ok(c, ndirs, c->ffg_n, c->ffg_S);
+}
+
+void obs_blocks(const ddafft_ctx* c, int ndirs, bool grouped, int* cb, int* fchunk) {
     *cb = c->sum_mode ? 1 : (int)std::min<long long>((c->d.n + kBlock - 1) / kBlock, 512);
+    if (grouped) {   // one chunk per 256 segments, at most 64
+        *fchunk = (int)std::max(1, std::min(64, (c->nseg + kBlock - 1) / kBlock));
+        return;
+    }
     const bool mf = ff_mfma_on(c);
     const int fd = mf ? kFDM : kFD;
     const int dtiles = (ndirs + fd - 1) / fd;
@@ -1368,7 +1448,8 @@ int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int
     f.state = state;
     f.hist = state && hist;
     f.max_step = max_step;
-    obs_blocks(c, ndirs, &f.cb, &f.fchunk);
+    f.grouped = ff_group_on(c, ndirs);
+    obs_blocks(c, ndirs, f.grouped, &f.cb, &f.fchunk);
     size_t off = 0;
     auto take = [&](size_t b) {
         const size_t o = off;
@@ -1389,6 +1470,14 @@ int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int
         TRY(dev_alloc(&c->esc, (size_t)nrhs * ndirs * 3));
         c->esc_cap = nrhs * ndirs * 3;
     }
+    if (f.grouped && (size_t)c->ncells * 6 > c->ffpc_cap) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->ffpc) hipFree(c->ffpc);
+        c->ffpc = nullptr;
+        c->ffpc_cap = 0;
+        TRY(dev_alloc(&c->ffpc, (size_t)c->ncells * 6));
+        c->ffpc_cap = (size_t)c->ncells * 6;
+    }
     const size_t ntab = (size_t)ndirs * (c->d.ni + c->d.nj + c->d.nl);
     if (ndirs > 0 && ntab > c->fftab_cap) {
         HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1406,6 +1495,48 @@ int fin_layout(ddafft_ctx* c, int nrhs, bool state, bool hist, int max_step, int
 // directions ddirs + r * dir_stride.  Queues the kernels and the copies of
 // their results into the pinned buffer.
 int obs_enqueue_seq(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_stride);
+
+// The grouped far field of right-hand sides r0 .. r0 + R - 1 (R = 1, 2):
+// their dipole moments gathered in list order, then k_far_field_grp with as
+// many groups per block as fit 16 directions and 24 T values per thread.
+int ff_grouped_enqueue(ddafft_ctx* c, const Fin& f, int r0, int R, int ndirs) {
+    Dims d = c->d;
+    const int S = c->ffg_S, ng = c->ffg_n;
+    if (!c->ffpc || !c->ffseg || (size_t)c->ncells * 3 * R > c->ffpc_cap)
+        return fail(DDAFFT_ESTATE, "internal: grouped far-field buffers not allocated");
+    const int gb = std::min((c->ncells + kBlock - 1) / kBlock, 2048);
+    if (R == 2)
+        hipLaunchKernelGGL(k_ff_gather<2>, gb, kBlock, 0, c->stream, d, c->cells, c->ncells, r0, c->P, c->ffpc);
+    else
+        hipLaunchKernelGGL(k_ff_gather<1>, gb, kBlock, 0, c->stream, d, c->cells, c->ncells, r0, c->P, c->ffpc);
+    HIPCHECK(hipGetLastError());
+    int G = 1;
+    while (2 * G * S <= kFFDirs && 2 * G * 3 * R <= 24) G *= 2;
+    const dim3 grid(f.fchunk, (ng + G - 1) / G);
+    const bool small = d.ni + d.nj <= 128 && d.nl <= 64;
+#define DDA_FFG(RR, GG)                                                                                        \
+    do {                                                                                                        \
+        if (small)                                                                                              \
+            hipLaunchKernelGGL((k_far_field_grp<RR, GG, 128, 64>), grid, kBlock, 0, c->stream, d, c->fftab,      \
+                               c->ffpc, c->cells, c->ffseg, c->nseg, ng, S, ndirs, r0, c->partials);            \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_far_field_grp<RR, GG, 512, 256>), grid, kBlock, 0, c->stream, d, c->fftab,     \
+                               c->ffpc, c->cells, c->ffseg, c->nseg, ng, S, ndirs, r0, c->partials);            \
+    } while (0)
+    if (R == 2) {
+        if (G == 4) DDA_FFG(2, 4);
+        else if (G == 2) DDA_FFG(2, 2);
+        else DDA_FFG(2, 1);
+    } else {
+        if (G == 8) DDA_FFG(1, 8);
+        else if (G == 4) DDA_FFG(1, 4);
+        else if (G == 2) DDA_FFG(1, 2);
+        else DDA_FFG(1, 1);
+    }
+#undef DDA_FFG
+    HIPCHECK(hipGetLastError());
+    return DDAFFT_OK;
+}
 
 int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_stride) {
     if (c->sum_mode) return obs_enqueue_seq(c, f, ddirs, dir_stride);
@@ -1441,7 +1572,9 @@ int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_s
             // phase rows in LDS sized for lattices up to 256 per axis, or the
             // larger form for 512-point axes
             const bool big = d.ni + d.nj + d.nl > kFFAxis;
-            if (ff_mfma_on(c)) {
+            if (f.grouped) {
+                TRY(ff_grouped_enqueue(c, f, r0, pair ? 2 : 1, ndirs));
+            } else if (ff_mfma_on(c)) {
                 const dim3 gm(f.fchunk, (ndirs + kFDM - 1) / kFDM);
                 const bool small = d.ni + d.nj + d.nl <= 192;
 #define DDA_FFM(R, AXM)                                                                                 \
@@ -1831,6 +1964,7 @@ int job_prepare(ddafft_ctx* c, const ddafft_job* j, int solver, int max_step, co
             }
             TRY(upload(c, c->dirs, j->dirs, per * sets * sizeof(float)));
             c->ndirs = j->dirs_per_rhs ? 0 : ndirs;   // per-RHS sets are not resident for later calls
+            c->ffg_n = j->dirs_per_rhs ? 0 : dir_groups(j->dirs, ndirs, &c->ffg_S);
             if (j->dirs_per_rhs) {
                 *dir_stride = (long long)per;
                 c->dir_pairs.assign((size_t)(nrhs + 1) / 2, 0);
@@ -1921,30 +2055,51 @@ int ddafft_solve_noalloc(ddafft_ctx* c, int nrhs, int solver, float tau, int max
     if (c->prof) return fail(DDAFFT_ESTATE, "profiling records events: disable it for the allocation-free solve");
     const Dims& d = c->d;
     // this call's pieces of the scratch
+    // the far field's form, as the solve will choose it for these directions;
+    // the grouped form needs the dipole moments gathered in the scratch
+    bool grouped = false;
+    {
+        int S = c->ffg_S;
+        const int ng = dirs ? dir_groups(dirs, ndirs, &S) : c->ffg_n;
+        grouped = ff_group_ok(c, ndirs, ng, S);
+    }
     int cb = 0, fchunk = 0;
-    obs_blocks(c, ndirs, &cb, &fchunk);
-    size_t off = 0;
+    size_t off = 0, res = 0, n_part = 0, n_hist = 0, n_esc = 0, n_tab = 0, n_dirs = 0, n_pc = 0;
+    size_t o_part = 0, o_hist = 0, o_esc = 0, o_tab = 0, o_dirs = 0, o_res = 0, o_pc = 0;
     auto take = [&](size_t b) {
         const size_t o = off;
         off += (b + 255) & ~(size_t)255;
         return o;
     };
-    const size_t n_part = std::max({(size_t)nrhs * std::max(zi_blocks(d), update_blocks(d)) * kSlots,
-                                    (size_t)nrhs * cb * 2, (size_t)nrhs * fchunk * std::max(ndirs, 1) * 6});
-    const size_t n_hist = (size_t)c->nrhs_max * (max_step + 1);
-    const size_t n_esc = (size_t)nrhs * ndirs * 3, n_tab = (size_t)ndirs * (d.ni + d.nj + d.nl);
-    const size_t n_dirs = dirs ? (size_t)ndirs * 3 : 0;
-    size_t res = 0;   // results, laid out as fin_layout lays them out
-    {
-        auto t = [&](size_t b) { res += (b + 255) & ~(size_t)255; };
-        t((size_t)nrhs * sizeof(SolveState));
-        if (err_hist) t((size_t)nrhs * (max_step + 1) * sizeof(double));
-        t((size_t)nrhs * cb * 2 * sizeof(double));
-        if (ndirs > 0) t((size_t)nrhs * ndirs * 3 * sizeof(float2));
+    // the plan of the scratch; the grouped far field only where its pieces fit
+    for (;;) {
+        off = 0;
+        res = 0;   // results, laid out as fin_layout lays them out
+        obs_blocks(c, ndirs, grouped, &cb, &fchunk);
+        n_part = std::max({(size_t)nrhs * std::max(zi_blocks(d), update_blocks(d)) * kSlots, (size_t)nrhs * cb * 2,
+                           (size_t)nrhs * fchunk * std::max(ndirs, 1) * 6});
+        n_hist = (size_t)c->nrhs_max * (max_step + 1);
+        n_esc = (size_t)nrhs * ndirs * 3;
+        n_tab = (size_t)ndirs * (d.ni + d.nj + d.nl);
+        n_dirs = dirs ? (size_t)ndirs * 3 : 0;
+        n_pc = grouped ? (size_t)c->ncells * 6 : 0;
+        {
+            auto t = [&](size_t b) { res += (b + 255) & ~(size_t)255; };
+            t((size_t)nrhs * sizeof(SolveState));
+            if (err_hist) t((size_t)nrhs * (max_step + 1) * sizeof(double));
+            t((size_t)nrhs * cb * 2 * sizeof(double));
+            if (ndirs > 0) t((size_t)nrhs * ndirs * 3 * sizeof(float2));
+        }
+        o_part = take(n_part * sizeof(double));
+        o_hist = take(n_hist * sizeof(double));
+        o_esc = take(n_esc * sizeof(float2));
+        o_tab = take(n_tab * sizeof(float2));
+        o_dirs = take(n_dirs * sizeof(float));
+        o_res = take(res);
+        o_pc = take(n_pc * sizeof(float2));
+        if (!grouped || off <= kScratchBytes) break;
+        grouped = false;
     }
-    const size_t o_part = take(n_part * sizeof(double)), o_hist = take(n_hist * sizeof(double));
-    const size_t o_esc = take(n_esc * sizeof(float2)), o_tab = take(n_tab * sizeof(float2));
-    const size_t o_dirs = take(n_dirs * sizeof(float)), o_res = take(res);
     size_t o_seq = 0, n_seq = 0;
     if (c->sum_mode) {   // the sequential-sum terms get the rest
         o_seq = take(0);
@@ -1964,11 +2119,13 @@ int ddafft_solve_noalloc(ddafft_ctx* c, int nrhs, int solver, float tau, int max
         double* hist; int hist_cap;
         float2* esc; int esc_cap;
         float2* fftab; size_t fftab_cap;
-        float* dirs; int dirs_cap, ndirs;
+        float* dirs; int dirs_cap, ndirs, ffg_n, ffg_S;
+        float2* ffpc; size_t ffpc_cap;
         float* seqbuf; size_t seqbuf_cap;
         char *dn, *dn_dev; size_t dn_cap;
     } sv{c->partials, c->partials_cap, c->hist, c->hist_cap, c->esc, c->esc_cap, c->fftab, c->fftab_cap, c->dirs,
-         c->dirs_cap, c->ndirs, c->seqbuf, c->seqbuf_cap, c->dn, c->dn_dev, c->dn_cap};
+         c->dirs_cap, c->ndirs, c->ffg_n, c->ffg_S, c->ffpc, c->ffpc_cap, c->seqbuf, c->seqbuf_cap, c->dn, c->dn_dev,
+         c->dn_cap};
     HIPCHECK(hipStreamSynchronize(c->stream));   // nothing of an earlier call still uses the old buffers
     char* sc = c->scratch;
     c->partials = reinterpret_cast<double*>(sc + o_part);
@@ -1983,6 +2140,11 @@ int ddafft_solve_noalloc(ddafft_ctx* c, int nrhs, int solver, float tau, int max
         c->dirs = reinterpret_cast<float*>(sc + o_dirs);
         c->dirs_cap = ndirs;
     }
+    if (grouped) {
+        c->ffpc = reinterpret_cast<float2*>(sc + o_pc);
+        c->ffpc_cap = n_pc;
+    }
+    c->ffg_off_call = !grouped;   // the solve's own choice then agrees with this plan
     if (c->sum_mode) {
         c->seqbuf = reinterpret_cast<float*>(sc + o_seq);
         c->seqbuf_cap = n_seq;
@@ -1994,6 +2156,7 @@ int ddafft_solve_noalloc(ddafft_ctx* c, int nrhs, int solver, float tau, int max
     const int rc = ddafft_solve(c, nrhs, solver, tau, max_step, err_max, alpha, kinc, einc, ndirs, dirs, esc, cabs,
                                 cext, err_hist, info);
     t_noalloc = false;
+    c->ffg_off_call = false;
     if (rc) hipStreamSynchronize(c->stream);   // a failed call may have left work queued on the scratch
     c->partials = sv.partials;
     c->partials_cap = sv.partials_cap;
@@ -2006,6 +2169,10 @@ int ddafft_solve_noalloc(ddafft_ctx* c, int nrhs, int solver, float tau, int max
     c->dirs = sv.dirs;
     c->dirs_cap = sv.dirs_cap;
     c->ndirs = sv.ndirs;
+    c->ffg_n = sv.ffg_n;
+    c->ffg_S = sv.ffg_S;
+    c->ffpc = sv.ffpc;
+    c->ffpc_cap = sv.ffpc_cap;
     c->seqbuf = sv.seqbuf;
     c->seqbuf_cap = sv.seqbuf_cap;
     c->dn = sv.dn;

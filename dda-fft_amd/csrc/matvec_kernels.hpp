@@ -1018,6 +1018,10 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ 
         const long long rb = (long long)rhs * 3 * d.n;
         const long long k0 = ((long long)i * d.nj + j0) * NLH;   // first cell of the block
         const int nep = tj * NLH;
+        // off the dipoles only the Simple solver's first step reads x (every
+        // other epilogue writes zeros there or leaves the cell): the operand
+        // loads wait on the flags as the polarizability waits on occ
+        const bool x_everywhere = kind == EK_SIM_R0;
         int kq[NEP];
         bool dip[NEP];
         float2 a[NEP], xr[NEP][3], yr[NEP][3], o2[NEP][3];
@@ -1027,16 +1031,20 @@ __global__ __launch_bounds__(kBlock) void k_zi(Dims d, SolveState* __restrict__ 
             kq[q] = f;   // cell k0 + f: row t = f / nl, l = f % nl
             dip[q] = false;
             a[q] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int c3 = 0; c3 < 3; ++c3) xr[q][c3] = yr[q][c3] = o2[q][c3] = make_float2(0.f, 0.f);
             if (f < nep) {
                 const int m = occ[k0 + f];
                 dip[q] = dipf[k0 + f] != 0;
                 a[q] = al[m];
+                const bool ldx = need_x && (dip[q] || x_everywhere);
+                const bool ldd = dip[q];
 #pragma unroll
                 for (int c3 = 0; c3 < 3; ++c3) {
                     const long long ck = rb + c3 * d.n + k0 + f;
-                    if (need_x) xr[q][c3] = ea.x[ck];
-                    if (need_y) yr[q][c3] = ea.y[ck];
-                    if (need_o2) o2[q][c3] = ea.out2[ck];
+                    if (ldx) xr[q][c3] = ea.x[ck];
+                    if (need_y && ldd) yr[q][c3] = ea.y[ck];
+                    if (need_o2 && ldd) o2[q][c3] = ea.out2[ck];
                 }
             }
         }

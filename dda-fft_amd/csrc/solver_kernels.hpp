@@ -117,40 +117,40 @@ __global__ __launch_bounds__(kBlock) void k_update(Dims d, SolveState* __restric
     double acc[kSlots] = {0, 0, 0, 0, 0, 0};
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < d.n;
          k += (long long)gridDim.x * blockDim.x) {
-        const bool dip = dipf[k] != 0;
+        if (dipf[k] == 0) {
+            // off the dipoles R keeps its value, and so does P except in
+            // BiCGStab, which sets it to zero: nothing is read there
+            if (ua.kind == UK_BICG) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) ua.P[rb + q * d.n + k] = make_float2(0.f, 0.f);
+            }
+            continue;
+        }
         double nrm = 0;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const long long ck = rb + q * d.n + k;
             float2 p = ua.P[ck], r = ua.R[ck];
             if (ua.kind == UK_BICG) {
-                if (dip) {
-                    const float2 t2 = cadd(p, cmul(ncA, ua.S[ck]));
-                    p = cadd(cmul(ncW, r), t2);
-                    r = cadd(r, cmul(ncW, ua.W[ck]));
-                } else {
-                    p = make_float2(0.f, 0.f);
-                }
-                if (dip) dot_acc(r, ua.G[ck], acc[0], acc[1]);
+                const float2 t2 = cadd(p, cmul(ncA, ua.S[ck]));
+                p = cadd(cmul(ncW, r), t2);
+                r = cadd(r, cmul(ncW, ua.W[ck]));
+                dot_acc(r, ua.G[ck], acc[0], acc[1]);
             } else if (ua.kind == UK_CG) {
-                if (dip) {
-                    const float2 w = ua.W[ck];
-                    p = cadd(p, cmul(ncA, ua.S[ck]));
-                    r = cadd(r, cmul(ncA, w));
-                    dot_acc(r, w, acc[0], acc[1]);
-                }
+                const float2 w = ua.W[ck];
+                p = cadd(p, cmul(ncA, ua.S[ck]));
+                r = cadd(r, cmul(ncA, w));
+                dot_acc(r, w, acc[0], acc[1]);
             } else {
-                if (dip) {
-                    r = cadd(r, rmul(rA, ua.W[ck]));
-                    p = cadd(p, rmul(rA, ua.S[ck]));
-                    acc[4] += norm_term(r);
-                }
+                r = cadd(r, rmul(rA, ua.W[ck]));
+                p = cadd(p, rmul(rA, ua.S[ck]));
+                acc[4] += norm_term(r);
             }
             ua.P[ck] = p;
             ua.R[ck] = r;
-            if (dip) nrm += norm_term(r);
+            nrm += norm_term(r);
         }
-        if (dip) acc[5] += sqrt(nrm);
+        acc[5] += sqrt(nrm);
     }
     block_sum<kSlots>(acc, red);
     if (threadIdx.x == 0) {
@@ -480,6 +480,153 @@ __global__ __launch_bounds__(kBlock) void k_far_field_mfma(Dims d, const float2*
             const int r = (cn >> 1) / 3, e6 = cn - r * 6;   // (component, re / im) of right-hand side r
             partials[(((long long)(rhs0 + r) * nchunk + blockIdx.x) * ndirs + d0 + row) * 6 + e6] = sum;
         }
+    }
+}
+
+// Far field of a direction set made of groups that share their z component
+// (the scattering planes of a fixed orientation are rotations of one plane
+// about z, so every polar angle gives one group with one direction per
+// plane).  With X, Y, Z the per-axis phase factors,
+//   S(n) = sum_(i,j) X_n[i] Y_n[j] T_g(i,j),   T_g(i,j) = sum_l Z_g[l] P(i,j,l),
+// so the l-sums are formed once per group instead of once per direction.
+// Group g holds the directions g, g + ngroups, ..., g + (S - 1) ngroups.
+// The dipole list is cut into segments of at most kFFSeg consecutive dipoles
+// of one (i, j) column (seg: {first dipole, (i nj + j) << 6 | count}).
+// Stage 1: one thread per segment sums its T for the block's G groups, from
+// the right-hand sides' dipole moments gathered in list order (pc, by
+// k_ff_gather).  Stage 2: the T of 256 segments go through LDS to 16 threads
+// per direction, which add X Y T.  Partials: those of k_far_field.
+// ---------------------------------------------------------------------------
+
+constexpr int kFFSeg = 16;    // dipoles per segment at most (below 64)
+constexpr int kFFDirs = 16;   // directions per block at most (G * S)
+
+// pc[e][r * 3 + c] = P of right-hand side rhs0 + r, component c, at dipole e
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_ff_gather(Dims d, const int* __restrict__ cells, int ncells, int rhs0,
+                                                      const float2* __restrict__ P, float2* __restrict__ pc) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ncells; e += gridDim.x * blockDim.x) {
+        const int k = cells[e];
+        float2 v[3 * R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[r * 3 + c] = P[(long long)(rhs0 + r) * 3 * d.n + c * d.n + k];
+#pragma unroll
+        for (int q = 0; q < 3 * R; ++q) pc[(long long)e * 3 * R + q] = v[q];
+    }
+}
+
+// AXY >= ni + nj, AZ >= nl: the phase rows held in LDS
+template <int R, int G, int AXY, int AZ>
+__global__ __launch_bounds__(kBlock) void k_far_field_grp(Dims d, const float2* __restrict__ ptab,
+                                                          const float2* __restrict__ pc, const int* __restrict__ cells,
+                                                          const int2* __restrict__ seg, int nseg, int ngroups, int S,
+                                                          int ndirs, int rhs0, double* __restrict__ partials) {
+    constexpr int NV = 3 * R;
+    constexpr int GT = G > 2 ? G : 2;   // the T rows also hold the FP64 sums at the end
+    constexpr int XS = AXY + 1;         // row stride: the direction slots of a wave in different banks
+    __shared__ float2 sxy[kFFDirs * XS];   // X and Y rows of the block's directions
+    __shared__ float2 sz[G * AZ];           // Z rows of its groups
+    __shared__ float2 sT[GT * NV * kBlock];
+    __shared__ int sij[kBlock];
+    static_assert(kFFDirs * 2 * NV * 16 * sizeof(double) <= sizeof(sT), "direction sums fit the T rows");
+    const int naxis = d.ni + d.nj + d.nl, nxy = d.ni + d.nj;
+    const int g0 = blockIdx.y * G, ndt = G * S;
+    for (int f = threadIdx.x; f < ndt * nxy; f += kBlock) {
+        const int q = f / nxy, a = f - q * nxy;
+        const int g = min(g0 + q / S, ngroups - 1), sd = q - (q / S) * S;
+        sxy[q * XS + a] = ptab[(long long)(g + sd * ngroups) * naxis + a];
+    }
+    for (int f = threadIdx.x; f < G * d.nl; f += kBlock) {
+        const int q = f / d.nl, a = f - q * d.nl;
+        const int g = min(g0 + q, ngroups - 1);
+        sz[q * AZ + a] = ptab[(long long)g * naxis + nxy + a];   // the same in every direction of the group
+    }
+    // stage-2 role: direction slot q (group qg, member qs), 16 threads per slot
+    const int q = threadIdx.x >> 4, sub = threadIdx.x & 15;
+    const int qg = q / S;
+    const int lnj = __builtin_ctz(d.nj);
+    float2 acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = make_float2(0.f, 0.f);
+    __syncthreads();
+    for (int b = blockIdx.x; b * kBlock < nseg; b += gridDim.x) {
+        const int s = b * kBlock + threadIdx.x;
+        float2 T[G][NV];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) T[g][v] = make_float2(0.f, 0.f);
+        int ij = -1;
+        if (s < nseg) {
+            const int2 sd = seg[s];
+            const int cnt = sd.y & 63;
+            ij = sd.y >> 6;
+            const float2* p = pc + (long long)sd.x * NV;
+            const int* ce = cells + sd.x;
+            // four dipoles per step, their loads issued together
+            for (int t0 = 0; t0 < cnt; t0 += 4) {
+                float2 pv[4][NV];
+                int lv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    lv[u] = 0;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) pv[u][v] = make_float2(0.f, 0.f);
+                    if (t0 + u < cnt) {
+                        lv[u] = ce[t0 + u] & (d.nl - 1);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) pv[u][v] = p[(t0 + u) * NV + v];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (t0 + u < cnt)
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const float2 z = sz[g * AZ + lv[u]];
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) cmac_fma(T[g][v], z, pv[u][v]);
+                        }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sT[(g * NV + v) * kBlock + threadIdx.x] = T[g][v];
+        sij[threadIdx.x] = ij;
+        __syncthreads();
+        if (q < ndt) {
+            const float2* xr = sxy + q * XS;
+            for (int m = sub; m < kBlock; m += 16) {
+                const int ijm = sij[m];
+                if (ijm < 0) break;   // the segments of a batch are a prefix of its slots
+                const int i = ijm >> lnj, j = ijm & (d.nj - 1);
+                const float2 ph = cmul_fma(xr[i], xr[d.ni + j]);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) cmac_fma(acc[v], ph, sT[(qg * NV + v) * kBlock + m]);
+            }
+        }
+        __syncthreads();   // sT and sij are refilled by the next batch
+    }
+    // FP64 sums of the 16 threads of each direction, in a fixed order
+    double* sums = reinterpret_cast<double*>(sT);   // [slot][2 NV][16]
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        sums[(q * 2 * NV + 2 * v) * 16 + sub] = acc[v].x;
+        sums[(q * 2 * NV + 2 * v + 1) * 16 + sub] = acc[v].y;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ndt * 2 * NV; t += kBlock) {
+        const int qq = t / (2 * NV), w = t - qq * 2 * NV;
+        const int g = g0 + qq / S, sd = qq - (qq / S) * S;
+        if (g >= ngroups) continue;
+        double a = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a += sums[t * 16 + u];
+        const int r = w / 6, e = w - r * 6;
+        partials[(((long long)(rhs0 + r) * gridDim.x + blockIdx.x) * ndirs + g + sd * ngroups) * 6 + e] = a;
     }
 }
 

@@ -169,7 +169,7 @@ class Context:
         Nyquist planes zeroed, mirrored in the X pass (ddafft_set_ahat_mode)."""
         _check(_lib.ddafft_set_ahat_mode(self._h, int(mode)))
 
-    def set_tuning(self, x_blocks=None, x_rb=None, graph=None, fuse_zy=None, ff_mfma=None):
+    def set_tuning(self, x_blocks=None, x_rb=None, graph=None, fuse_zy=None, ff_mfma=None, ff_group=None):
         """Launch schedule (ddafft_set_tuning); None leaves a setting as it
         is.  x_blocks: X-pass launch size up to which right-hand-side groups
         are spread over blocks (0: the tuned table's); x_rb: right-hand sides
@@ -177,7 +177,9 @@ class Context:
         captured HIP graphs (True / False, or -1: tuned); fuse_zy: z- and
         y-forward transforms of an i-plane in one kernel where it fits (True /
         False, or -1: tuned); ff_mfma: the far field on the matrix cores
-        (True / False, or -1: the library's choice)."""
+        (True / False, or -1: the library's choice); ff_group: the far field
+        by groups of directions sharing their z component where the set
+        splits so (True / False, or -1: the default, on)."""
         if x_blocks is not None:
             _check(_lib.ddafft_set_tuning(self._h, 0, int(x_blocks)))
         if x_rb is not None:
@@ -188,6 +190,8 @@ class Context:
             _check(_lib.ddafft_set_tuning(self._h, 3, -1 if fuse_zy == -1 else int(bool(fuse_zy))))
         if ff_mfma is not None:
             _check(_lib.ddafft_set_tuning(self._h, 4, -1 if ff_mfma == -1 else int(bool(ff_mfma))))
+        if ff_group is not None:
+            _check(_lib.ddafft_set_tuning(self._h, 5, -1 if ff_group == -1 else int(bool(ff_group))))
 
     def set_directions(self, dirs):
         dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)

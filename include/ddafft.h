@@ -175,6 +175,14 @@ int ddafft_set_ahat_mode(ddafft_ctx* ctx, int mode);
  * (the direct kernel).  The two agree to about 1e-7 relative, not bit for
  * bit. */
 #define DDAFFT_TUNE_FF_MFMA 4
+/* DDAFFT_TUNE_FF_GROUP: 1 (default, -1) sums the far field (FP64-summed mode)
+ * of a direction set whose z components repeat with a period ng, i.e. ng
+ * groups of 2..16 directions sharing z (the scattering planes of a fixed
+ * orientation), by groups: the sum over l once per group, then per direction
+ * over the (i, j) columns; lattices with n_i + n_j <= 512 and n_l <= 256.
+ * 0 always uses the per-direction kernel.  The two agree to about 1e-7
+ * relative, not bit for bit. */
+#define DDAFFT_TUNE_FF_GROUP 5
 int ddafft_set_tuning(ddafft_ctx* ctx, int key, int value);
 /* One instance's solve for ddafft_solve_ex / ddafft_solve_batch: nrhs
  * right-hand sides, each with its own incidence (Monte-Carlo orientations x

@@ -51,6 +51,10 @@ struct float4 {
     float x, y, z, w;
 };
 inline float2 make_float2(float a, float b) { return float2{a, b}; }
+struct int2 {
+    int x, y;
+};
+inline int2 make_int2(int a, int b) { return int2{a, b}; }
 
 struct dim3 {
     unsigned x, y, z;

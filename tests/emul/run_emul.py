@@ -98,6 +98,7 @@ def main():
     ok &= check_y_lines(orc)
     ok &= check_z_lines(orc)
     ok &= check_ff_mfma(orc)
+    ok &= check_ff_group(orc)
     if not quick:
         ok &= check_gradient()
         ok &= check_long_axes(orc)
@@ -341,6 +342,44 @@ def check_ff_mfma(orc):
     return ok
 
 
+def check_ff_group(orc):
+    """The far field by groups of directions sharing z (ddafft_set_tuning
+    ff_group) against the per-direction kernel and the oracle: 2 and 3
+    planes (a partial last tile of groups), one and two right-hand sides,
+    columns cut into several segments (n_l = 32), and a set that does not
+    split (every direction its own z), which takes the per-direction kernel."""
+    rng = np.random.default_rng(43)
+    ok = True
+    cases = (((8, 8, 8), (0.0, 45.0), 2), ((4, 4, 32), (0.0, 60.0, 120.0), 2), ((4, 8, 16), (0.0, 60.0, 120.0), 1),
+             ((8, 4, 8), None, 2))
+    for dims, planes, nrhs in cases:
+        kd = np.float32(0.5)
+        A = orc.build_ahat(dims, kd)
+        occ = (rng.random(dims) < 0.8).astype(np.int8)
+        mv = np.array([0, 1], np.int32)
+        alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.05j)], np.complex64)
+        kinc = np.array([0, 0, 1], np.float32)
+        einc = np.eye(3, dtype=np.complex64)[:nrhs]
+        if planes is None:
+            v = rng.standard_normal((21, 3))
+            dirs = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+        else:
+            dirs = np.concatenate([directions(np.arange(0, 181, 10), p) for p in planes])
+        out = {}
+        with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+            ctx.set_target(occ, mv)
+            for mode in (0, 1):
+                ctx.set_tuning(ff_group=mode)
+                out[mode] = ctx.solve(2, np.stack([alpha] * nrhs), kinc, einc, dirs=dirs, max_step=4, err_max=1e-9)
+        for pol in range(nrhs):
+            ref = orc.dda_fft(2, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=4, err_max=1e-9)
+            e1 = np.max(np.abs(out[1][pol]["esc"] - ref["esc"])) / np.max(np.abs(ref["esc"]))
+            e01 = np.max(np.abs(out[1][pol]["esc"] - out[0][pol]["esc"])) / np.max(np.abs(out[0][pol]["esc"]))
+            ok &= check(e1 < 1e-5 and e01 < 2e-6, "ff group %s planes %s rhs %d/%d vs oracle %.1e vs direct %.1e" % (
+                dims, planes, pol, nrhs, e1, e01))
+    return ok
+
+
 def check_long_axes(orc):
     """256- and 128-point axes (the transform sizes of the 256^3 workload, 512
     points padded): A-hat, both convolutions and a short solve, bit for bit."""
@@ -423,7 +462,8 @@ def check_graph(orc):
 
 def check_noalloc(orc):
     """ddafft_solve_noalloc: the results of ddafft_solve bit for bit, in both
-    sum modes, twice in a row (the second smaller)."""
+    sum modes, twice in a row (the second smaller); two scattering planes, so
+    the FP64 mode takes the grouped far field with its moments in the scratch."""
     ok = True
     for dims, nrhs in (((8, 8, 8), 2), ((4, 4, 8), 1)):
         kd = np.float32(0.6)
@@ -431,7 +471,7 @@ def check_noalloc(orc):
         alpha = np.array([0, orc.polarizability(1, kd, 1.5 + 0.01j)], np.complex64)
         kinc = np.array([0, 0, 1], np.float32)
         einc = np.eye(3, dtype=np.complex64)[:nrhs]
-        dirs = directions(np.arange(0, 181, 30))
+        dirs = np.concatenate([directions(np.arange(0, 181, 30), p) for p in (0.0, 90.0)])
         for mode in (0, 1):
             with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
                 ctx.set_target(occ, [0, 1])

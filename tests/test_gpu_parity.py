@@ -417,6 +417,46 @@ def test_compact_ahat_converged_observables(orc64):
             assert e[1][k] < max(3 * e[0][k], 3e-5), k
 
 
+@pytest.mark.parametrize("dims,npol,oracle_too", [((16, 16, 16), 2, True), ((32, 32, 32), 2, True),
+                                                  ((32, 32, 64), 1, True), ((64, 64, 64), 2, False),
+                                                  ((128, 128, 128), 2, False), ((64, 32, 128), 1, False)])
+def test_far_field_grouped(orc32, dims, npol, oracle_too):
+    """The far field by groups of directions sharing z (the default where the
+    direction set splits so): within 2e-6 of the per-direction kernel
+    (ddafft_set_tuning ff_group 0) and, at sizes the oracle finishes quickly,
+    within 1e-5 of the oracle's far field.  Three planes of 91 angles leave a
+    partial tile of groups; four planes of 181 are the scattering workload's
+    set; 128-point axes take the kernel's larger phase-row form."""
+    kd = np.float32(0.3)
+    occ = target(orc32, dims, 1.0, 0.5)
+    mv = np.array([0, 1], np.int32)
+    alpha = np.array([0, orc32.polarizability(1, kd, 1.5 + 0.02j)], np.complex64)
+    kinc = np.array([0, 0, 1], np.float32)
+    einc = np.eye(3, dtype=np.complex64)[:npol]
+    if oracle_too:
+        dirs = np.concatenate([directions(np.arange(0, 181, 2), p) for p in (0.0, 30.0, 90.0)])   # 273
+    else:
+        dirs = np.concatenate([directions(np.arange(0, 181, 1), p) for p in (0.0, 45.0, 90.0, 135.0)])   # 724
+    out = {}
+    with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
+        ctx.set_target(occ, mv)
+        A = ctx.ahat()
+        for mode in (0, 1):
+            ctx.set_tuning(ff_group=mode)
+            out[mode] = ctx.solve(2, alpha, kinc, einc, dirs=dirs, max_step=6, err_max=1e-6)
+    for pol in range(npol):
+        e01 = _relerr(out[1][pol]["esc"], out[0][pol]["esc"])
+        e = float("nan")
+        if oracle_too:
+            ref = orc32.dda_fft(2, occ, mv, alpha, kd, kinc, einc[pol], A, dirs, max_step=6, err_max=1e-6)
+            e = _relerr(out[1][pol]["esc"], ref["esc"])
+            assert e < 1e-5
+        print("ff grouped", dims, pol, "vs oracle %.2e, vs per-direction %.2e" % (e, e01), flush=True)
+        assert e01 < 2e-6
+        assert out[1][pol]["cext"] == out[0][pol]["cext"]
+        assert out[1][pol]["steps"] == out[0][pol]["steps"]
+
+
 @pytest.mark.parametrize("dims,npol", [((16, 16, 16), 2), ((32, 32, 32), 2), ((32, 32, 64), 1)])
 def test_far_field_mfma(orc32, dims, npol):
     """The far field summed on the matrix cores (ddafft_set_tuning ff_mfma):
@@ -432,6 +472,7 @@ def test_far_field_mfma(orc32, dims, npol):
     out = {}
     with ddafft.Context(dims, kd, nrhs_max=2) as ctx:
         ctx.set_target(occ, mv)
+        ctx.set_tuning(ff_group=0)   # against the per-direction kernel
         A = ctx.ahat()
         for mode in (0, 1):
             ctx.set_tuning(ff_mfma=mode)
