@@ -1600,7 +1600,8 @@ int obs_enqueue(ddafft_ctx* c, const Fin& f, const float* ddirs, long long dir_s
             HIPCHECK(hipGetLastError());
             r0 += pair ? 2 : 1;
         }
-        hipLaunchKernelGGL(k_far_field_sum, (nrhs * ndirs + kBlock - 1) / kBlock, kBlock, 0, c->stream, c->partials,
+        // one thread per (right-hand side, direction), in blocks of 64 to spread over the CUs
+        hipLaunchKernelGGL(k_far_field_sum, (nrhs * ndirs + 63) / 64, 64, 0, c->stream, c->partials,
                            f.fchunk, ndirs, nrhs, ddirs, dir_stride, reinterpret_cast<float2*>(c->dn_dev + f.o_esc));
         HIPCHECK(hipGetLastError());
     }

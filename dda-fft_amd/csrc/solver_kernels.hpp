@@ -500,6 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_far_field_mfma(Dims d, const float2*
 
 constexpr int kFFSeg = 16;    // dipoles per segment at most (below 64)
 constexpr int kFFDirs = 16;   // directions per block at most (G * S)
+constexpr int kFFStep = 4;    // dipoles of a segment whose loads are in flight together
 
 // pc[e][r * 3 + c] = P of right-hand side rhs0 + r, component c, at dipole e
 template <int R>
@@ -565,12 +566,12 @@ __global__ __launch_bounds__(kBlock) void k_far_field_grp(Dims d, const float2* 
             ij = sd.y >> 6;
             const float2* p = pc + (long long)sd.x * NV;
             const int* ce = cells + sd.x;
-            // four dipoles per step, their loads issued together
-            for (int t0 = 0; t0 < cnt; t0 += 4) {
-                float2 pv[4][NV];
-                int lv[4];
+            // kFFStep dipoles per step, their loads issued together
+            for (int t0 = 0; t0 < cnt; t0 += kFFStep) {
+                float2 pv[kFFStep][NV];
+                int lv[kFFStep];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kFFStep; ++u) {
                     lv[u] = 0;
 #pragma unroll
                     for (int v = 0; v < NV; ++v) pv[u][v] = make_float2(0.f, 0.f);
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_far_field_grp(Dims d, const float2* 
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < kFFStep; ++u)
                     if (t0 + u < cnt)
 #pragma unroll
                         for (int g = 0; g < G; ++g) {
@@ -599,9 +600,10 @@ __global__ __launch_bounds__(kBlock) void k_far_field_grp(Dims d, const float2* 
         __syncthreads();
         if (q < ndt) {
             const float2* xr = sxy + q * XS;
-            for (int m = sub; m < kBlock; m += 16) {
+            const int nv = min(kBlock, nseg - b * kBlock);   // the batch's segments fill its first slots
+#pragma unroll 4
+            for (int m = sub; m < nv; m += 16) {
                 const int ijm = sij[m];
-                if (ijm < 0) break;   // the segments of a batch are a prefix of its slots
                 const int i = ijm >> lnj, j = ijm & (d.nj - 1);
                 const float2 ph = cmul_fma(xr[i], xr[d.ni + j]);
 #pragma unroll
@@ -640,11 +642,24 @@ __global__ __launch_bounds__(kBlock) void k_far_field_sum(const double* __restri
     if (idx >= nrhs * ndirs) return;
     const int dd = idx % ndirs, rhs = idx / ndirs;
     double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < nchunk; ++c) {
-        const double* p = partials + (((long long)rhs * nchunk + c) * ndirs + dd) * 6;
+    const double* p0 = partials + ((long long)rhs * nchunk * ndirs + dd) * 6;
+    const long long cs = (long long)ndirs * 6;   // chunk stride
+    // chunks in order, the loads of four in flight together
+    int c = 0;
+    for (; c + 4 <= nchunk; c += 4) {
+        double t[4][6];
 #pragma unroll
-        for (int e = 0; e < 6; ++e) s[e] += p[e];
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int e = 0; e < 6; ++e) t[u][e] = p0[(c + u) * cs + e];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int e = 0; e < 6; ++e) s[e] += t[u][e];
     }
+    for (; c < nchunk; ++c)
+#pragma unroll
+        for (int e = 0; e < 6; ++e) s[e] += p0[c * cs + e];
     const float* dv = dirs + rhs * dir_stride + 3 * dd;
     const double n[3] = {dv[0], dv[1], dv[2]};
     const double dr = n[0] * s[0] + n[1] * s[2] + n[2] * s[4];
