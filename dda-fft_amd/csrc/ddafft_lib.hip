@@ -1355,7 +1355,9 @@ int step0(Seq& q, int solver) {
     return DDAFFT_OK;
 }
 
-int iteration(Seq& q, int solver, int o, float tau) {
+// part 0: the whole iteration; BiCGStab also in two parts, 1 (the first
+// matvec and its scalar step) and 2 (the rest)
+int iteration(Seq& q, int solver, int o, float tau, int part = 0) {
     ddafft_ctx* c = q.c;
     const int zb = zi_blocks(c->d);
     const int ub = update_blocks(c->d);
@@ -1377,9 +1379,13 @@ int iteration(Seq& q, int solver, int o, float tau) {
             TRY(q.mv(o, pro(PRO_COPY, c->R), epi(EK_AA_SUPD, c->R, nullptr, c->S), 1));
             break;
         case DDAFFT_SOLVER_BICGSTAB:
-            TRY(q.mv(o, snap(pro(PRO_LC4, c->R, c->S, c->H)), epi(EK_BICG_H, c->S, c->H, nullptr, nullptr, c->G), 0, FK_BICG_A));
-            TRY(q.mv(o, pro(PRO_LC1, c->R, c->H), epi(EK_BICG_W, c->R, c->W), 0, FK_BICG_W));
-            TRY(q.upd(o, UpdArgs{UK_BICG, c->P, c->R, c->S, c->W, c->G, nullptr}, FK_BICG_E));
+            if (part != 2)
+                TRY(q.mv(o, snap(pro(PRO_LC4, c->R, c->S, c->H)), epi(EK_BICG_H, c->S, c->H, nullptr, nullptr, c->G), 0,
+                         FK_BICG_A));
+            if (part != 1) {
+                TRY(q.mv(o, pro(PRO_LC1, c->R, c->H), epi(EK_BICG_W, c->R, c->W), 0, FK_BICG_W));
+                TRY(q.upd(o, UpdArgs{UK_BICG, c->P, c->R, c->S, c->W, c->G, nullptr}, FK_BICG_E));
+            }
             break;
         case DDAFFT_SOLVER_SIMPLE:
             TRY(q.mv(o, snap(pro(PRO_COPY, c->P)), epi(EK_SIM, c->P, c->R, c->P, nullptr, c->S, tau), 0, FK_SIM_E));
@@ -1848,6 +1854,35 @@ int run_wait(Run& r) {
     return DDAFFT_OK;
 }
 
+// One more iteration of a single solve.  BiCGStab iterations launched
+// eagerly go in two parts: the first matvec and its scalar step are queued
+// before the host reads the stopping decisions of the previous iteration,
+// the rest only if the solve goes on.  After the last iteration only that
+// first part runs idle (its kernels exit at once) instead of a whole
+// iteration, and the GPU has the first part to run while the host decides.
+// Other solvers and graph replays queue whole iterations one ahead.
+int run_step(Run& r) {
+    ddafft_ctx* c = r.c;
+    const int o = r.o;
+    const bool graph = schedule(c, r.nrhs).graph && !c->prof && !t_noalloc;
+    if (r.solver != DDAFFT_SOLVER_BICGSTAB || graph) {
+        TRY(run_enqueue(r));
+        return run_wait(r);
+    }
+    TRY(iteration(r.q, r.solver, o, r.tau, 1));
+    // the decisions of iteration o - 1 (step 0 for o = 1), published to its ring slot
+    HIPCHECK(hipEventSynchronize(c->ev[(o - 1) & 3]));
+    const SolveState* sr = c->state_ring + (size_t)((o - 1) & 3) * c->nrhs_max;
+    int m = 0;
+    for (int q = 0; q < r.nrhs; ++q) m = std::max(m, sr[q].stop_at);
+    r.known_stop = std::min(r.known_stop, m);
+    if (o >= r.known_stop) return DDAFFT_OK;   // iteration o does not run: run_more ends the loop
+    TRY(iteration(r.q, r.solver, o, r.tau, 2));
+    HIPCHECK(hipEventRecord(c->ev[o & 3], c->stream));
+    r.o = o + 1;
+    return DDAFFT_OK;
+}
+
 // best-iterate restore, which also writes the final states into the pinned
 // result buffer; then the copy of the history, when asked for
 int fin_state_enqueue(Run& r, const Fin& f) {
@@ -1921,10 +1956,7 @@ int solve_core(ddafft_ctx* c, int nrhs, int solver, float tau, int max_step, flo
     r.max_step = max_step;
     r.err_max = err_max;
     TRY(run_begin(r));
-    while (run_more(r)) {
-        TRY(run_enqueue(r));
-        TRY(run_wait(r));
-    }
+    while (run_more(r)) TRY(run_step(r));
     return run_finish(r, err_hist, info, sf);
 }
 
@@ -2004,10 +2036,7 @@ int ddafft_solve_ex(ddafft_ctx* c, const ddafft_job* job, int solver, float tau,
     r.max_step = max_step;
     r.err_max = err_max;
     TRY(run_begin(r));
-    while (run_more(r)) {
-        TRY(run_enqueue(r));
-        TRY(run_wait(r));
-    }
+    while (run_more(r)) TRY(run_step(r));
     Fin f;
     TRY(fin_layout(c, job->nrhs, true, err_hist != nullptr, max_step, job->ndirs, f));
     TRY(fin_state_enqueue(r, f));
@@ -2497,10 +2526,7 @@ int ddafft_gradient(ddafft_ctx* c, int npol, int nobj, int solver, float tau, in
     ra.max_step = max_step;
     ra.err_max = err_max;
     TRY(run_begin(ra));
-    while (run_more(ra)) {
-        TRY(run_enqueue(ra));
-        TRY(run_wait(ra));
-    }
+    while (run_more(ra)) TRY(run_step(ra));
     Fin fa;
     TRY(fin_layout(c, nadj, true, false, max_step, 0, fa));
     TRY(fin_state_enqueue(ra, fa));
