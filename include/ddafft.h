@@ -103,10 +103,13 @@ int ddafft_solve(ddafft_ctx* ctx, int nrhs, int solver, float tau, int max_step,
 /* ddafft_solve that allocates nothing.  No counterpart in the reference: it
  * is for callers that cannot allocate memory during a solve (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).
- * Same arguments and results as ddafft_solve, bit for bit.  Its scratch
- * (block partials, discrepancy history, far-field partials and phase tables,
- * per-call directions, results on their way back, reference-order sum terms)
- * lives in a fixed 32 MiB device array, one per GPU, that every such call on
+ * Same arguments and results as ddafft_solve, bit for bit (except the far
+ * field of a direction set of scattering planes when the grouped far field's
+ * gathered dipole moments, 48 B per dipole for two right-hand sides, do not
+ * fit the scratch: then the per-direction kernel, within about 1e-7).  Its
+ * scratch (block partials, discrepancy history, far-field partials and phase
+ * tables, per-call directions, results on their way back, reference-order
+ * sum terms, gathered moments) lives in a fixed 32 MiB device array, one per GPU, that every such call on
  * that GPU shares: calls that overlap on one GPU race.  A solve whose
  * scratch would exceed it fails with DDAFFT_EINVAL and a message giving the
  * size needed; results are limited to 1 MiB per call.  Inputs are copied from
